@@ -1,0 +1,200 @@
+"""Drives any library that exports the Synaptogenesis C-ABI through the same
+sequence of calls, so one case definition runs on the reference's serial CPU
+build (to make fixtures) and on the MI355X library (to check it).
+
+Inputs are injected only through the reference's own public hooks, because its
+random generator cannot be seeded (util/tools.h:8):
+
+* weights and delays -- a ``"callback"`` / ``"delay callback"`` entry in the
+  weight config (state/weight_matrix.cpp:853-869, 930-949);
+* initial neuron state -- writes through ``get_neuron_data`` views
+  (state/state.cpp:706-718, extern.cpp:214-224);
+* per-step input -- a ``"callback"`` input module (io/impl/callback_module.cpp:29-51);
+* per-step capture -- one ``"callback"`` output module per (layer, key).
+"""
+
+import copy
+import hashlib
+
+import numpy as np
+
+from synaptogenesis_amd import syngen
+
+
+class Case:
+    """One seeded workload: a network dict plus everything injected into it.
+
+    ``weights`` / ``delays`` map connection name -> host-layout array.
+    ``init`` maps (structure, layer, key) -> array written before the run.
+    ``inputs`` maps (structure, layer) -> callable(step) -> float32 array.
+    ``capture`` lists (structure, layer, key) recorded after every step.
+    ``final_neuron`` / ``final_conn`` list data read after the run.
+    """
+
+    def __init__(self, name, net, steps, learning=True, weights=None,
+                 delays=None, init=None, inputs=None, capture=(),
+                 final_neuron=(), final_conn=(), keep_steps=None):
+        self.name = name
+        self.net = net
+        self.steps = steps
+        self.learning = learning
+        self.weights = weights or {}
+        self.delays = delays or {}
+        self.init = init or {}
+        self.inputs = inputs or {}
+        self.capture = list(capture)
+        self.final_neuron = list(final_neuron)
+        self.final_conn = list(final_conn)
+        self.keep_steps = keep_steps
+
+
+def digest(arr):
+    return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
+
+
+_UINT_KEYS = ("output",)
+_INT_KEYS = ("time since spike", "delays")
+
+
+def _as_dtype(key):
+    if key in _UINT_KEYS:
+        return np.uint32
+    if key in _INT_KEYS:
+        return np.int32
+    return np.float32
+
+
+def _view(parr, key):
+    """Copy a library-owned array out as the dtype its key holds."""
+    n = parr.size
+    if n == 0 or not parr.data:
+        return np.zeros(0, dtype=_as_dtype(key))
+    raw = np.ctypeslib.as_array(
+        syngen.cast(parr.data, syngen.POINTER(syngen.c_float)), shape=(n,))
+    return raw.view(_as_dtype(key)).copy()
+
+
+def _writable(parr, key):
+    raw = np.ctypeslib.as_array(
+        syngen.cast(parr.data, syngen.POINTER(syngen.c_float)),
+        shape=(parr.size,))
+    return raw.view(_as_dtype(key))
+
+
+def run_case(case, lib, run_args=None):
+    """Run ``case`` on ``lib``; return (per-step captures, finals, report).
+
+    captures: dict (structure, layer, key) -> list of per-step arrays
+    finals:   dict ('neuron', structure, layer, key) / ('conn', name, key)
+              -> array
+    """
+    net = copy.deepcopy(case.net)
+    tag = "%s_%x" % (case.name, id(case) & 0xFFFFFF)
+
+    # Weight / delay injection through weight-config callbacks.
+    conns = {c.get("name", ""): c for c in net.get("connections", [])}
+    wlist = []
+    for cname in sorted(set(case.weights) | set(case.delays)):
+        c = conns[cname]
+        wc = dict(c.get("weight config", {}))
+        wc.setdefault("type", "flat")
+        wc.setdefault("weight", "0")
+        wc["id"] = str(len(wlist))
+        if cname in case.weights:
+            wc["callback"] = "W_" + tag
+        if cname in case.delays:
+            wc["delay callback"] = "D_" + tag
+        c["weight config"] = wc
+        wlist.append(cname)
+
+    def weight_cb(ID, size, ptr):
+        w = case.weights[wlist[ID]]
+        assert w.size == size, (wlist[ID], w.size, size)
+        np.ctypeslib.as_array(
+            syngen.cast(ptr, syngen.POINTER(syngen.c_float)),
+            shape=(size,))[:] = w
+
+    def delay_cb(ID, size, wptr, dptr):
+        d = case.delays[wlist[ID]]
+        assert d.size == size, (wlist[ID], d.size, size)
+        np.ctypeslib.as_array(
+            syngen.cast(dptr, syngen.POINTER(syngen.c_int)),
+            shape=(size,))[:] = d
+
+    keep = []
+    if case.weights:
+        keep.append(lib.create_weight_callback("W_" + tag, weight_cb))
+    if case.delays:
+        keep.append(lib.create_delay_weight_callback("D_" + tag, delay_cb))
+
+    network = syngen.Network(net, lib)
+    assert network.obj, "network construction failed"
+    network.build_state()
+    assert network.state, "state construction failed"
+
+    for (s, l, key), arr in case.init.items():
+        view = _writable(network.get_neuron_data(s, l, key), key)
+        arr = np.asarray(arr).reshape(-1)
+        assert view.size == arr.size, (s, l, key, view.size, arr.size)
+        view[:] = arr.astype(view.dtype)
+
+    # Environment: one input module for all input layers, one output module
+    # per captured (layer, key).
+    modules = []
+    in_layers = list(case.inputs.keys())
+    counters = [0] * len(in_layers)
+
+    def in_cb(ID, size, ptr):
+        vals = case.inputs[in_layers[ID]](counters[ID])
+        counters[ID] += 1
+        np.ctypeslib.as_array(
+            syngen.cast(ptr, syngen.POINTER(syngen.c_float)),
+            shape=(size,))[:] = vals
+
+    if in_layers:
+        keep.append(lib.create_io_callback("IN_" + tag, in_cb))
+        modules.append({
+            "type": "callback",
+            "layers": [{"structure": s, "layer": l, "input": "true",
+                        "function": "IN_" + tag, "id": str(i)}
+                       for i, (s, l) in enumerate(in_layers)]})
+
+    captures = {k: [] for k in case.capture}
+    for j, (s, l, key) in enumerate(case.capture):
+        def out_cb(ID, size, ptr, _k=(s, l, key)):
+            raw = np.ctypeslib.as_array(
+                syngen.cast(ptr, syngen.POINTER(syngen.c_float)),
+                shape=(size,))
+            captures[_k].append(raw.view(_as_dtype(_k[2])).copy())
+        name = "OUT%d_%s" % (j, tag)
+        keep.append(lib.create_io_callback(name, out_cb))
+        modules.append({
+            "type": "callback",
+            "layers": [{"structure": s, "layer": l, "output": "true",
+                        "function": name, "id": "0",
+                        "output keys": [key]}]})
+
+    args = {"iterations": str(case.steps), "worker threads": "0",
+            "multithreaded": "false",
+            "learning flag": "true" if case.learning else "false",
+            "verbose": "false"}
+    if run_args:
+        args.update(run_args)
+    report = None
+    if case.steps > 0:
+        report = network.run({"modules": modules}, args)
+
+    finals = {}
+    for (s, l, key) in case.final_neuron:
+        finals[("neuron", s, l, key)] = _view(
+            network.get_neuron_data(s, l, key), key)
+    for (cname, key) in case.final_conn:
+        finals[("conn", cname, key)] = _view(
+            network.get_weight_matrix(cname, key), key)
+    network.free()
+    del keep
+    return captures, finals, report
+
+
+def finals_key(k):
+    return "|".join(k)

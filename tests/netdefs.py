@@ -1,0 +1,348 @@
+"""Seeded parity workloads, defined once and run on both libraries.
+
+Every case is a network in the reference's JSON vocabulary plus seeded
+injections (see harness.py).  Sizes are chosen so that the reference's serial
+CPU build finishes each case in seconds; the full benchmark sizes are built by
+the same functions with larger arguments.
+"""
+
+import numpy as np
+
+from synaptogenesis_amd.inputs import Stream
+from harness import Case
+
+
+def S(x):
+    if isinstance(x, bool):
+        return "true" if x else "false"
+    return str(x)
+
+
+def layer(name, model, rows, cols, **kw):
+    d = {"name": name, "neural model": model, "rows": S(rows),
+         "columns": S(cols)}
+    d.update({k.replace("_", " "): S(v) for k, v in kw.items()})
+    return d
+
+
+def conn(name, frm, to, ctype, opcode="add", plastic=False, delay=0,
+         arbor=None, subset=None, weight=None, **kw):
+    d = {"name": name, "from layer": frm, "to layer": to, "type": ctype,
+         "opcode": opcode, "plastic": S(plastic), "delay": S(delay)}
+    if arbor is not None:
+        d["arborized config"] = {k: S(v) for k, v in arbor.items()}
+    if subset is not None:
+        d["subset config"] = {k: S(v) for k, v in subset.items()}
+    if weight is not None:
+        d["weight config"] = {k: S(v) for k, v in weight.items()}
+    d.update({k.replace("_", " "): S(v) for k, v in kw.items()})
+    return d
+
+
+def net(structure, layers, conns, cluster="parallel"):
+    return {"structures": [{"name": structure, "type": cluster,
+                            "layers": layers}],
+            "connections": conns}
+
+
+# -- connection geometry (num_weights as in network/connection.cpp:7-61) ----
+
+def _arbor_dims(a):
+    rf = int(a.get("row field size", a.get("field size", 0)))
+    cf = int(a.get("column field size", a.get("field size", 0)))
+    return rf, cf
+
+
+def num_weights(c, layers):
+    sizes = {l["name"]: int(l["rows"]) * int(l["columns"]) for l in layers}
+    t = c["type"]
+    if t == "fully connected":
+        return sizes[c["from layer"]] * sizes[c["to layer"]]
+    if t == "one to one":
+        return sizes[c["to layer"]]
+    if t == "subset":
+        s = {k: int(v) for k, v in c["subset config"].items()}
+        fs = (s["from row end"] - s["from row start"]) * \
+            (s["from column end"] - s["from column start"])
+        ts = (s["to row end"] - s["to row start"]) * \
+            (s["to column end"] - s["to column start"])
+        return fs * ts
+    rf, cf = _arbor_dims(c["arborized config"])
+    if c.get("convolutional", "false") == "true":
+        return rf * cf
+    return sizes[c["to layer"]] * rf * cf
+
+
+# -- cases -------------------------------------------------------------------
+
+def conn_zoo(steps=12):
+    """Base activator over every connection type, opcode, FLOAT and BIT
+    sources, word-0 and word-1 delays, wrap/no-wrap, stride, spacing,
+    offset and convolutional kernels (engine/kernel/synapse_kernel.h)."""
+    st = Stream(101)
+    L = [layer("src", "relay", 12, 10),
+         layer("bsrc", "game_of_life", 12, 10),
+         layer("d_fc", "relay", 7, 9),
+         layer("d_fcd", "relay", 5, 5),
+         layer("d_sub", "relay", 12, 10),
+         layer("d_one", "relay", 12, 10),
+         layer("d_conv", "relay", 6, 5),
+         layer("d_convk", "relay", 12, 10),
+         layer("d_div", "relay", 23, 19),
+         layer("d_divk", "relay", 25, 21),
+         layer("d_div2", "relay", 3, 4)]
+    C = [
+        conn("gol", "bsrc", "bsrc", "convergent", convolutional=True,
+             arbor={"field size": 3, "wrap": True},
+             weight={"type": "flat", "weight": 1, "diagonal": False}),
+        conn("fc_f", "src", "d_fc", "fully connected"),
+        conn("fc_b", "bsrc", "d_fc", "fully connected", "sub", delay=5),
+        conn("fcd_f", "src", "d_fcd", "fully connected", delay=2),
+        conn("sub_f", "src", "d_sub", "subset",
+             subset={"from row start": 2, "from row end": 10,
+                     "from column start": 1, "from column end": 8,
+                     "to row start": 3, "to row end": 12,
+                     "to column start": 2, "to column end": 9}),
+        conn("sub_b", "bsrc", "d_sub", "subset", "pool", delay=40,
+             subset={"from row start": 0, "from row end": 6,
+                     "from column start": 0, "from column end": 10,
+                     "to row start": 6, "to row end": 12,
+                     "to column start": 0, "to column end": 4}),
+        conn("one_f", "src", "d_one", "one to one"),
+        conn("one_b", "bsrc", "d_one", "one to one", "mult", delay=3),
+        conn("conv_f", "src", "d_conv", "convergent",
+             arbor={"field size": 3, "stride": 2}),
+        conn("conv_b", "bsrc", "d_conv", "convergent", "sub_heaviside",
+             delay=33,
+             arbor={"row field size": 5, "column field size": 3,
+                    "stride": 2, "spacing": 2, "offset": -3, "wrap": True}),
+        conn("convk_f", "src", "d_convk", "convergent", convolutional=True,
+             arbor={"field size": 5, "wrap": True}),
+        conn("convk_b", "bsrc", "d_convk", "convergent", "add_heaviside",
+             convolutional=True, arbor={"field size": 3, "wrap": True}),
+        conn("div_f", "src", "d_div", "divergent",
+             arbor={"field size": 3, "stride": 2}),
+        conn("div_b", "bsrc", "d_div", "divergent", "mult_heaviside",
+             arbor={"field size": 4, "stride": 3, "wrap": True}),
+        conn("divk_f", "src", "d_divk", "divergent", convolutional=True,
+             arbor={"field size": 5, "stride": 2, "wrap": True}),
+        conn("divk_b", "bsrc", "d_divk", "divergent", "sub",
+             convolutional=True,
+             arbor={"row field size": 3, "column field size": 4,
+                    "row stride": 2, "column stride": 1, "wrap": True}),
+        conn("div2_a", "src", "d_div2", "fully connected"),
+        conn("div2_d", "src", "d_div2", "fully connected", "div"),
+    ]
+    n = net("s", L, C)
+    weights = {}
+    for i, c in enumerate(C):
+        if c["name"] == "gol":
+            continue
+        nw = num_weights(c, L)
+        weights[c["name"]] = st.uniform(nw, step=0, stream=1000 + i,
+                                        low=-1.0, high=1.0)
+    init = {("s", "bsrc", "output"): st.bits(2 * 120, step=0, stream=1)}
+    inputs = {("s", "src"): lambda t: st.normal(120, step=t + 1, stream=2)}
+    dests = [l["name"] for l in L if l["name"].startswith("d_")]
+    capture = [("s", d, "input") for d in dests] + [("s", "bsrc", "output")]
+    return Case("conn_zoo", n, steps, weights=weights, init=init,
+                inputs=inputs, capture=capture,
+                final_neuron=[("s", "bsrc", "output")])
+
+
+def gol(name, rows, cols, steps, workload, wrap=True, conway_init=False):
+    """Game of life on one layer: 3x3 convolutional kernel, weight 1, centre
+    cleared by "diagonal": false (state/weight_matrix.cpp:951-977)."""
+    st = Stream(workload)
+    L = [layer("board", "game_of_life", rows, cols)]
+    C = [conn("life", "board", "board", "convergent", convolutional=True,
+              arbor={"field size": 3, "wrap": wrap},
+              weight={"type": "flat", "weight": 1, "diagonal": False})]
+    n = rows * cols
+    if conway_init:
+        words = st.bernoulli(n, 0.5, stream=1).astype(np.uint32) << 31
+    else:
+        words = st.bits(n, stream=1)
+    return Case(name, net("gol", L, C), steps,
+                init={("gol", "board", "output"): words.astype(np.uint32)},
+                capture=[("gol", "board", "output")],
+                final_neuron=[("gol", "board", "output"),
+                              ("gol", "board", "input")])
+
+
+def iz_fc(size=1000, steps=200, name="iz_fc"):
+    """Two Izhikevich layers, FULLY_CONNECTED size->size direct, weights
+    U[0,0.5), input N(5,2) into layer 1, v0=-65, u0=b*v0."""
+    st = Stream(1)
+    L = [layer("L1", "izhikevich", 1, size, params="regular"),
+         layer("L2", "izhikevich", 1, size, params="regular")]
+    C = [conn("fc", "L1", "L2", "fully connected", direct=True,
+              myelinated=True)]
+    v0 = np.full(size, -65.0, np.float32)
+    u0 = np.full(size, np.float32(0.2) * np.float32(-65.0), np.float32)
+    init = {("izfc", "L1", "voltage"): v0, ("izfc", "L1", "recovery"): u0,
+            ("izfc", "L2", "voltage"): v0, ("izfc", "L2", "recovery"): u0}
+    w = st.uniform(size * size, stream=1, low=0.0, high=0.5)
+    inputs = {("izfc", "L1"):
+              lambda t: st.normal(size, step=t + 1, stream=2, mean=5.0,
+                                  std=2.0)}
+    return Case(name, net("izfc", L, C), steps, weights={"fc": w},
+                init=init, inputs=inputs,
+                capture=[("izfc", "L1", "output"), ("izfc", "L2", "output"),
+                         ("izfc", "L2", "voltage")],
+                final_neuron=[("izfc", l, k) for l in ("L1", "L2")
+                              for k in ("voltage", "recovery", "output",
+                                        "post trace", "time since spike")])
+
+
+def iz_sheet(e_rows=32, e_cols=40, steps=60, name="iz_sheet",
+             noise=(0.0, 3.0), workload=3):
+    """Excitatory/inhibitory Izhikevich sheet: E (a .02, d 8) and I at half
+    resolution (a .1, d 2).  E->E convergent 15x15 stride 1 wrap on the
+    conductance path with STDP + short-term plasticity (learning rate 0.01)
+    and per-synapse delays U{0..31}; E->I convergent 15x15 stride 2 wrap
+    direct; I->E divergent 9x9 stride 2 wrap, opcode sub, direct.  Weights
+    U[0,0.5), noise N(mean, std) into E and I every step."""
+    st = Stream(workload)
+    i_rows, i_cols = e_rows // 2, e_cols // 2
+    ne, ni = e_rows * e_cols, i_rows * i_cols
+    L = [layer("E", "izhikevich", e_rows, e_cols, params="regular"),
+         layer("I", "izhikevich", i_rows, i_cols, params="fast")]
+    C = [conn("EE", "E", "E", "convergent", plastic=True,
+              arbor={"field size": 15, "stride": 1, "wrap": True},
+              learning_rate=0.01),
+         conn("EI", "E", "I", "convergent",
+              arbor={"field size": 15, "stride": 2, "wrap": True},
+              direct=True, myelinated=True),
+         conn("IE", "I", "E", "divergent", "sub",
+              arbor={"field size": 9, "stride": 2, "wrap": True},
+              direct=True, myelinated=True)]
+    weights, delays = {}, {}
+    for i, c in enumerate(C):
+        nw = num_weights(c, L)
+        weights[c["name"]] = st.uniform(nw, stream=10 + i, low=0.0, high=0.5)
+    delays["EE"] = st.integers(num_weights(C[0], L), 0, 31, stream=20)
+    init = {}
+    for lname, n, b in (("E", ne, 0.2), ("I", ni, 0.2)):
+        init[("sheet", lname, "voltage")] = np.full(n, -65.0, np.float32)
+        init[("sheet", lname, "recovery")] = np.full(
+            n, np.float32(b) * np.float32(-65.0), np.float32)
+    mean, std = noise
+    inputs = {("sheet", "E"): lambda t: st.normal(ne, step=t + 1, stream=3,
+                                                  mean=mean, std=std),
+              ("sheet", "I"): lambda t: st.normal(ni, step=t + 1, stream=4,
+                                                  mean=mean, std=std)}
+    ee_keys = ["weights", "short trace", "long trace", "presyn trace",
+               "short term R", "short term w", "weight derivatives",
+               "time since spike"]
+    return Case(name, net("sheet", L, C), steps, weights=weights,
+                delays=delays, init=init, inputs=inputs,
+                capture=[("sheet", "E", "output"), ("sheet", "I", "output")],
+                final_neuron=[("sheet", l, k) for l in ("E", "I")
+                              for k in ("voltage", "recovery", "output",
+                                        "post trace", "time since spike",
+                                        "ampa", "nmda", "gabaa", "gabab")],
+                final_conn=[("EE", k) for k in ee_keys])
+
+
+def iz_ops(steps=80):
+    """Izhikevich conductance activators for every opcode and connection
+    type, both plastic updaters (STDP add, iSTDP sub), gap junctions, a
+    direct connection and a myelinated delay."""
+    st = Stream(104)
+    L = [layer("A", "izhikevich", 8, 10, params="regular"),
+         layer("B", "izhikevich", 8, 10, params="fast"),
+         layer("C", "izhikevich", 6, 6, params="bursting")]
+    C = [conn("AB_fc", "A", "B", "fully connected", "add", plastic=True,
+              learning_rate=0.05),
+         conn("AB_sub", "A", "B", "subset", "sub", plastic=True,
+              subset={"from row start": 1, "from row end": 7,
+                      "from column start": 0, "from column end": 10,
+                      "to row start": 0, "to row end": 8,
+                      "to column start": 2, "to column end": 8}),
+         conn("AB_one", "A", "B", "one to one", "mult"),
+         conn("AB_conv", "A", "B", "convergent", "reward",
+              arbor={"field size": 3, "wrap": True}),
+         conn("BB_div", "B", "B", "divergent", "add", plastic=True,
+              arbor={"field size": 3, "stride": 1}),
+         conn("AA_gap", "A", "A", "fully connected", "gap"),
+         conn("AC_dir", "A", "C", "convergent", "add",
+              arbor={"field size": 3, "stride": 2}, direct=True,
+              myelinated=True, delay=2),
+         conn("BC_sub", "B", "C", "fully connected", "sub", plastic=True,
+              short_term_plasticity=False)]
+    weights, delays = {}, {}
+    for i, c in enumerate(C):
+        nw = num_weights(c, L)
+        hi = 0.02 if c["opcode"] == "gap" else 0.6
+        weights[c["name"]] = st.uniform(nw, stream=10 + i, low=0.0, high=hi)
+        if c.get("direct") != "true":
+            delays[c["name"]] = st.integers(nw, 0, 31, stream=40 + i)
+    init = {}
+    for lname, n, b in (("A", 80, 0.2), ("B", 80, 0.2), ("C", 36, 0.2)):
+        init[("ops", lname, "voltage")] = np.full(n, -65.0, np.float32)
+        init[("ops", lname, "recovery")] = np.full(
+            n, np.float32(b) * np.float32(-65.0), np.float32)
+    inputs = {("ops", "A"): lambda t: st.normal(80, step=t + 1, stream=3,
+                                                mean=4.0, std=6.0)}
+    plastic = ["AB_fc", "AB_sub", "BB_div", "BC_sub"]
+    keys = ["weights", "presyn trace", "short term R", "short term w",
+            "weight derivatives", "time since spike", "short trace",
+            "long trace"]
+    return Case("iz_ops", net("ops", L, C), steps, weights=weights,
+                delays=delays, init=init, inputs=inputs,
+                capture=[("ops", l, k) for l in ("A", "B", "C")
+                         for k in ("output", "voltage")],
+                final_neuron=[("ops", l, k) for l in ("A", "B", "C")
+                              for k in ("voltage", "recovery", "ampa",
+                                        "nmda", "gabaa", "gabab", "mult",
+                                        "dopamine", "post trace",
+                                        "time since spike")],
+                final_conn=[(c, k) for c in plastic for k in keys])
+
+
+def rate(steps=30):
+    """Rate-encoding layers (rectified tanh) behind FULLY_CONNECTED and
+    CONVERGENT connections from a relay input layer."""
+    st = Stream(105)
+    L = [layer("in", "relay", 8, 8),
+         layer("h", "rate_encoding", 1, 48),
+         layer("c", "rate_encoding", 4, 4),
+         layer("o", "rate_encoding", 1, 10)]
+    C = [conn("in_h", "in", "h", "fully connected"),
+         conn("in_c", "in", "c", "convergent",
+              arbor={"field size": 3, "stride": 2}),
+         conn("h_o", "h", "o", "fully connected", delay=1),
+         conn("c_o", "c", "o", "fully connected", "sub")]
+    weights = {}
+    for i, c in enumerate(C):
+        nw = num_weights(c, L)
+        weights[c["name"]] = st.normal(nw, stream=10 + i, std=0.5)
+    inputs = {("rate", "in"): lambda t: st.uniform(64, step=t + 1, stream=2)}
+    return Case("rate", net("rate", L, C), steps, weights=weights,
+                inputs=inputs,
+                capture=[("rate", l, "output") for l in ("h", "c", "o")],
+                final_neuron=[("rate", l, "output") for l in ("h", "c", "o")])
+
+
+def iz_init():
+    """Default Izhikevich state after build_state (no run): parameter
+    tables and the constructor's settle loop
+    (state/impl/izhikevich_attributes.cpp:28-35,626-649)."""
+    names = ["default", "regular", "bursting", "chattering", "fast",
+             "low_threshold", "thalamo_cortical", "resonator"]
+    L = [layer(p, "izhikevich", 1, 3, params=p) for p in names]
+    keys = ["voltage", "recovery", "a", "b", "c", "d", "post trace",
+            "time since spike"]
+    return Case("iz_init", net("init", L, []), 0,
+                final_neuron=[("init", p, k) for p in names for k in keys])
+
+
+def all_cases():
+    return [conn_zoo(), gol("gol_small", 24, 20, 80, 102),
+            gol("gol_conway", 20, 24, 60, 103, conway_init=True),
+            gol("gol_nowrap", 16, 18, 40, 106, wrap=False),
+            iz_fc(), iz_sheet(),
+            iz_sheet(24, 32, 80, "iz_sheet_hot", noise=(2.0, 5.0),
+                     workload=107),
+            iz_ops(), rate(), iz_init()]
