@@ -1,0 +1,190 @@
+/*
+ * sg_hip.h -- the thin kernel C-ABI of the MI355X simulation core.
+ *
+ * Everything here is plain C: POD descriptors, raw device pointers, an
+ * opaque stream (a hipStream_t, or NULL for the default stream) and int
+ * error codes (0 == hipSuccess).  libsyngen.so (the C++ host layer behind
+ * the reference-compatible C-ABI in syngen.h) drives the simulation through
+ * these entry points, and Python loads the same library with ctypes for
+ * kernel-level parity tests.
+ *
+ * Each entry point replaces one family of the reference's per-timestep
+ * kernels; the reference file:line it restates is cited beside it.  Arrays
+ * called "device layout" hold per-synapse data k-major ([k][destination]),
+ * the transpose of the reference's host layout ([destination][k]), so one
+ * lane per destination reads consecutive addresses.
+ */
+#ifndef SG_HIP_H
+#define SG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Connection types (util/constants.h:71-78). */
+enum { SG_FULLY_CONNECTED = 0, SG_SUBSET = 1, SG_ONE_TO_ONE = 2,
+       SG_CONVERGENT = 3, SG_DIVERGENT = 4 };
+
+/* Source output types (util/constants.h:15-19). */
+enum { SG_OUT_FLOAT = 0, SG_OUT_BIT = 1, SG_OUT_INT = 2 };
+
+/* Opcodes (util/constants.h:109-121). */
+enum { SG_ADD = 0, SG_SUB = 1, SG_MULT = 2, SG_DIV = 3, SG_POOL = 4,
+       SG_REWARD = 5, SG_MODULATE = 6, SG_GAP = 7, SG_ADD_HEAVISIDE = 8,
+       SG_SUB_HEAVISIDE = 9, SG_MULT_HEAVISIDE = 10 };
+
+/* One connection as the kernels see it (engine/kernel/synapse_data.h:16-57
+ * plus the ArborizedConfig / SubsetConfig it carries). */
+typedef struct sg_conn {
+    int type;            /* SG_FULLY_CONNECTED .. SG_DIVERGENT            */
+    int opcode;          /* aggregator for the base activator             */
+    int out_type;        /* SG_OUT_* of the source layer                  */
+    int delay;           /* connection delay                              */
+    int convolutional;   /* shared kernel of row/column field size        */
+    int from_size, from_rows, from_cols;
+    int to_size, to_rows, to_cols;
+    /* arborized config (network/connection_config.cpp:8-45) */
+    int row_field, col_field, row_stride, col_stride;
+    int row_spacing, col_spacing, row_offset, col_offset, wrap;
+    /* subset config (network/connection_config.cpp:112-134) */
+    int from_row_start, from_row_end, from_col_start, from_col_end;
+    int to_row_start, to_row_end, to_col_start, to_col_end;
+    /* weights per destination slot in device layout, number of slots */
+    int k_per_dest;
+    int num_dest;
+    int num_weights;
+    /* source output at word delay/32 (BIT) or register delay (FLOAT/INT) */
+    const uint32_t *src;
+    /* destination input register */
+    float *dst;
+    /* weights, device layout */
+    const float *weights;
+} sg_conn;
+
+/* Per-synapse and per-source state of an Izhikevich conductance connection
+ * (state/impl/izhikevich_attributes.h:45-66). */
+typedef struct sg_iz_syn {
+    float *weights;
+    float *short_traces, *long_traces, *presyn_traces;
+    float *stp_r, *stp_w, *weight_derivatives;
+    /* delay & 31 in bits 0-7, the number of later visits of the same
+     * source in the reference's serial order (capped at 32) in bits 8-15 */
+    const uint16_t *delay_rank;
+    /* time since spike per source: value after the last activation, and
+     * the buffer the next activation writes */
+    const int32_t *tss_cur;
+    int32_t *tss_next;
+    /* min(32, visits per source) */
+    const uint8_t *visits;
+    float baseline_conductance;
+    float learning_rate;
+    float max_weight;
+    int stp_flag;
+} sg_iz_syn;
+
+/* Destination-side data an Izhikevich updater reads, plus the conductance
+ * arrays an activator accumulates into. */
+typedef struct sg_iz_dest {
+    const uint32_t *out_word0;      /* destination spike history word 0 */
+    const float *post_trace;
+    const int32_t *time_since_spike;
+    float *short_cond;             /* ampa / gabaa / mult / dopamine    */
+    float *long_cond;              /* nmda / gabab, or NULL             */
+    float short_tau, long_tau;
+    const float *voltage;          /* gap junctions                     */
+    float *input;                  /* gap junctions                     */
+} sg_iz_dest;
+
+/* Izhikevich conductance activator kinds (izhikevich_attributes.cpp:288-366). */
+enum { SG_IZ_ADD = 0, SG_IZ_SUB = 1, SG_IZ_MULT = 2, SG_IZ_REWARD = 3,
+       SG_IZ_GAP = 4 };
+
+/* One Izhikevich layer (izhikevich_attributes.h:6-43). */
+typedef struct sg_iz_layer {
+    int size, history;
+    float *input;
+    uint32_t *output;             /* [history][size] */
+    float *ampa, *nmda, *gabaa, *gabab, *mult, *dopamine;
+    float *voltage, *recovery, *post_trace;
+    int32_t *time_since_spike;
+    const float *a, *b, *c, *d;
+} sg_iz_layer;
+
+/* Base activator: every connection type, extractor and aggregator
+ * (engine/kernel/synapse_kernel.h:47-912, kernel.cpp:217,
+ * extractor.cpp:8-15, aggregator.cpp:15-48). */
+int sg_hip_activate_base(const sg_conn *c, void *stream);
+
+/* time-since-spike pre-pass: tss_next = min(32, tss_cur + visits). */
+int sg_hip_tss_prepass(int32_t *tss_next, const int32_t *tss_cur,
+                       const uint8_t *visits, int n, void *stream);
+
+/* Izhikevich conductance / gap activators
+ * (izhikevich_attributes.cpp:219-366). */
+int sg_hip_iz_activate(const sg_conn *c, const sg_iz_syn *s,
+                       const sg_iz_dest *d, int kind, void *stream);
+
+/* Izhikevich STDP/iSTDP + short-term-plasticity updaters
+ * (izhikevich_attributes.cpp:404-520).  sub != 0 selects update_iz_sub.
+ * src_table / dst_table hold the iSTDP factor for time-since-spike 0..31. */
+int sg_hip_iz_update(const sg_conn *c, const sg_iz_syn *s,
+                     const sg_iz_dest *d, int sub, const float *src_table,
+                     const float *dst_table, void *stream);
+
+/* The updater of step t-1 fused with the conductance activator of step t
+ * over one pass of the per-synapse arrays (same results as running
+ * sg_hip_iz_update then sg_hip_iz_activate). */
+int sg_hip_iz_update_activate(const sg_conn *c, const sg_iz_syn *s,
+                              const sg_iz_dest *d, int kind, int sub,
+                              const float *src_table, const float *dst_table,
+                              void *stream);
+
+/* Neuron models (izhikevich_attributes.cpp:102-213,
+ * game_of_life_attributes.cpp:14-33, rate_encoding_attributes.cpp:14-23,
+ * relay_attributes.cpp:10-18) including the history shift
+ * (state/attributes.h:316-333). */
+int sg_hip_iz_neuron(const sg_iz_layer *l, void *stream);
+int sg_hip_gol_neuron(float *input, uint32_t *output, int size, int history,
+                      int survival_min, int survival_max, int birth_min,
+                      int birth_max, void *stream);
+int sg_hip_rate_neuron(float *input, float *output, int size, int history,
+                       void *stream);
+int sg_hip_relay_neuron(float *input, float *output, int size, int history,
+                        int ramp, void *stream);
+
+/* Game of life fused step for the one-connection 3x3 convolutional case:
+ * activation + rule + shift in one pass over double-buffered words. */
+int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
+                    float *input, int rows, int cols, int wrap,
+                    const float *kernel9, int survival_min, int survival_max,
+                    int birth_min, int birth_max, void *stream);
+
+/* Random init configs (engine/kernel/kernel.cpp:43-158 via
+ * engine/instruction.h:208-282): dist 0 = uniform[a,b), 1 = normal(a, b),
+ * 2 = poisson (fires `val` with probability a).  Counter-based Philox keyed
+ * by (seed, step), so runs are reproducible. */
+int sg_hip_init_random(float *p, int n, int dist, float a, float b, float val,
+                       int overwrite, uint64_t seed, uint64_t step,
+                       void *stream);
+
+/* set_data (engine/kernel/kernel.cpp:14-40). */
+int sg_hip_set_f32(float *p, float val, long n, int overwrite, void *stream);
+
+/* Layout changes: 32-bit element matrix transpose [rows][cols] ->
+ * [cols][rows]. */
+int sg_hip_transpose32(const void *src, void *dst, int rows, int cols,
+                       void *stream);
+
+/* Device properties: number of visible devices, -1 on error. */
+int sg_hip_device_count(void);
+
+/* Last error string for a code. */
+const char *sg_hip_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

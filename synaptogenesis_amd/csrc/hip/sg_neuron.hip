@@ -1,0 +1,290 @@
+// Neuron-model kernels with the spike / value history shift, plus the small
+// utility kernels (set, transpose).
+//
+// One lane per neuron; neuron state is structure-of-arrays so every access
+// is a coalesced 4-byte stream.  The Izhikevich update keeps the reference's
+// mixed precision exactly: the expressions that its double literals promote
+// to FP64 are evaluated in FP64 here too, in the same order, and rounded to
+// FP32 where the reference stores to a float (compiled with
+// -ffp-contract=off, correctly rounded division, denormals preserved).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sg_hip.h"
+
+namespace sg {
+
+constexpr int kNeuronBlock = 256;
+
+static inline int nblocks(long n) {
+    return (int)((n + kNeuronBlock - 1) / kNeuronBlock);
+}
+
+// SHIFT_BIT_OUTPUTS (state/attributes.h:321-333): the history words of one
+// neuron form one long shift register, newest bit at the MSB of word 0.
+// Returns the bit that left word 0 (prev_bit).
+__device__ __forceinline__ bool shift_bits(uint32_t *out, int size, int nid,
+                                           int history, uint32_t new_bit) {
+    uint32_t curr = out[(long)size * (history - 1) + nid];
+    for (int i = history - 1; i > 0; --i) {
+        const uint32_t prev = out[(long)size * (i - 1) + nid];
+        out[(long)size * i + nid] = (curr >> 1) | (prev << 31);
+        curr = prev;
+    }
+    out[nid] = (curr >> 1) | (new_bit << 31);
+    return (curr >> 31) != 0;
+}
+
+// SHIFT_FLOAT_OUTPUTS (state/attributes.h:316-319).
+__device__ __forceinline__ void shift_floats(float *out, int size, int nid,
+                                             int history, float v) {
+    for (int i = history - 1; i > 0; --i)
+        out[(long)size * i + nid] = out[(long)size * (i - 1) + nid];
+    out[nid] = v;
+}
+
+// iz_attribute_kernel (state/impl/izhikevich_attributes.cpp:102-213).
+__global__ __launch_bounds__(kNeuronBlock) void iz_neuron_kernel(
+        sg_iz_layer l) {
+    const int nid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nid >= l.size) return;
+
+    const float ampa = l.ampa[nid];
+    const float nmda = l.nmda[nid];
+    const float gabaa = l.gabaa[nid];
+    const float gabab = l.gabab[nid];
+    const float mult = l.mult[nid];
+    float voltage = l.voltage[nid];
+    float recovery = l.recovery[nid];
+    const float base_current = l.input[nid];
+    const float a = l.a[nid];
+    const float b = l.b[nid];
+
+    for (int i = 0; i < 10 && (double)voltage < 30.0; ++i) {
+        float current = -ampa * voltage;
+        // powf(x, 2) is folded to x * x by the reference's compiler.
+        const float x = (voltage + 80.0f) / 60.0f;
+        const float temp = x * x;
+        current = current - nmda * (temp / (1.0f + temp)) * voltage;
+        current = current - gabaa * (voltage + 70.0f);
+        current = current - gabab * (voltage + 90.0f);
+        current = current * (1.0f + mult);
+        current = current + base_current;
+
+        const double dv = (0.04 * (double)voltage) * (double)voltage +
+                          (double)(5.0f * voltage) + 140.0 -
+                          (double)recovery + (double)current;
+        const float delta_v = (float)dv;
+
+        const float sum_g = ampa + nmda + gabaa + gabab;
+        const float drive = delta_v + (gabaa * -70.0f) + (gabab * -90.0f);
+        voltage = (float)(((double)voltage + 0.1 * (double)drive) /
+                          (1.0 + 0.1 * (double)sum_g));
+        voltage = (voltage != voltage) ? 30.0f : voltage;
+
+        const float adjusted_tau = ((double)voltage > 30.0)
+            ? (float)((double)delta_v /
+                      (30.0 - (double)voltage + (double)delta_v) * 0.1)
+            : (float)0.1;
+        recovery = recovery + a * adjusted_tau * ((b * voltage) - recovery);
+    }
+
+    l.ampa[nid] = 0.0f;
+    l.nmda[nid] = 0.0f;
+    l.gabaa[nid] = 0.0f;
+    l.gabab[nid] = 0.0f;
+    l.mult[nid] = 0.0f;
+    l.dopamine[nid] = (float)((double)l.dopamine[nid] * 0.95);
+
+    const uint32_t spike = (double)voltage >= 30.0 ? 1u : 0u;
+    const bool prev_bit = shift_bits(l.output, l.size, nid, l.history, spike);
+
+    l.post_trace[nid] = prev_bit
+        ? (float)0.004
+        : (float)((double)l.post_trace[nid] * 0.95);
+    l.time_since_spike[nid] = prev_bit
+        ? 0 : min(32, l.time_since_spike[nid] + 1);
+    l.voltage[nid] = spike ? l.c[nid] : voltage;
+    l.recovery[nid] = recovery + ((float)spike * l.d[nid]);
+}
+
+// gol_attribute_kernel (state/impl/game_of_life_attributes.cpp:14-33):
+// survival reads bit 0 of word 0, the state 32 steps back.
+__global__ __launch_bounds__(kNeuronBlock) void gol_neuron_kernel(
+        const float *input, uint32_t *output, int size, int history,
+        int smin, int smax, int bmin, int bmax) {
+    const int nid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nid >= size) return;
+    const int in = (int)input[nid];
+    const bool prev_live = output[nid] & 1u;
+    const bool live = prev_live ? (in >= smin && in <= smax)
+                                : (in >= bmin && in <= bmax);
+    shift_bits(output, size, nid, history, live ? 1u : 0u);
+}
+
+// re_attribute_kernel (state/impl/rate_encoding_attributes.cpp:14-23):
+// out = input > 0 ? tanh(input) : 0.  tanh is evaluated in FP64 and
+// rounded once to FP32.
+__global__ __launch_bounds__(kNeuronBlock) void rate_neuron_kernel(
+        const float *input, float *output, int size, int history) {
+    const int nid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nid >= size) return;
+    const float in = input[nid];
+    const float v = ((double)in > 0.0) ? (float)tanh((double)in) : 0.0f;
+    shift_floats(output, size, nid, history, v);
+}
+
+// relay_attribute_kernel (state/impl/relay_attributes.cpp:10-18).
+__global__ __launch_bounds__(kNeuronBlock) void relay_neuron_kernel(
+        const float *input, float *output, int size, int history, int ramp) {
+    const int nid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (nid >= size) return;
+    const float in = input[nid];
+    const float v = ramp ? (float)fmax(0.0, (double)in) : in;
+    shift_floats(output, size, nid, history, v);
+}
+
+__global__ __launch_bounds__(kNeuronBlock) void set_f32_kernel(
+        float *p, float val, long n, int overwrite) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = overwrite ? val : p[i] + val;
+}
+
+// Philox4x32-10 (Salmon et al., SC'11) for the random init configs.
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0,
+                                       uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__global__ __launch_bounds__(kNeuronBlock) void init_random_kernel(
+        float *p, int n, int dist, float a, float b, float val, int overwrite,
+        uint64_t seed, uint64_t step) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c[4] = {(uint32_t)step, (uint32_t)i, (uint32_t)(step >> 32), 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = ((double)(c[0] >> 8) + 1.0) * (1.0 / 16777216.0);
+    const double u2 = (double)(c[1] >> 8) * (1.0 / 16777216.0);
+    const float u = (float)((double)(c[0] >> 8) * (1.0 / 16777216.0));
+    if (dist == 0) {
+        const float v = u * (b - a) + a;
+        p[i] = overwrite ? v : p[i] + v;
+    } else if (dist == 1) {
+        const float z = (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        const float v = z * b + a;
+        p[i] = overwrite ? v : p[i] + v;
+    } else {
+        const bool fire = u < a;
+        if (overwrite) p[i] = fire ? val : 0.0f;
+        else if (fire) p[i] += val;
+    }
+}
+
+// 32-bit transpose through a padded LDS tile.
+constexpr int kTile = 64;
+constexpr int kRowsPerPass = 4;
+
+__global__ __launch_bounds__(kTile * kRowsPerPass) void transpose32_kernel(
+        const uint32_t *src, uint32_t *dst, int rows, int cols) {
+    __shared__ uint32_t tile[kTile][kTile + 1];
+    const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int j = ty; j < kTile; j += kRowsPerPass) {
+        const int r = by + j, c = bx + tx;
+        if (r < rows && c < cols) tile[j][tx] = src[(long)r * cols + c];
+    }
+    __syncthreads();
+    for (int j = ty; j < kTile; j += kRowsPerPass) {
+        const int r = bx + j, c = by + tx;   // output row = input column
+        if (r < cols && c < rows) dst[(long)r * rows + c] = tile[tx][j];
+    }
+}
+
+}  // namespace sg
+
+using namespace sg;
+
+extern "C" int sg_hip_iz_neuron(const sg_iz_layer *l, void *stream) {
+    if (l->size <= 0) return 0;
+    hipLaunchKernelGGL(iz_neuron_kernel, dim3(nblocks(l->size)),
+                       dim3(kNeuronBlock), 0, (hipStream_t)stream, *l);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_gol_neuron(float *input, uint32_t *output, int size,
+                                 int history, int smin, int smax, int bmin,
+                                 int bmax, void *stream) {
+    if (size <= 0) return 0;
+    hipLaunchKernelGGL(gol_neuron_kernel, dim3(nblocks(size)),
+                       dim3(kNeuronBlock), 0, (hipStream_t)stream, input,
+                       output, size, history, smin, smax, bmin, bmax);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_rate_neuron(float *input, float *output, int size,
+                                  int history, void *stream) {
+    if (size <= 0) return 0;
+    hipLaunchKernelGGL(rate_neuron_kernel, dim3(nblocks(size)),
+                       dim3(kNeuronBlock), 0, (hipStream_t)stream, input,
+                       output, size, history);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_relay_neuron(float *input, float *output, int size,
+                                   int history, int ramp, void *stream) {
+    if (size <= 0) return 0;
+    hipLaunchKernelGGL(relay_neuron_kernel, dim3(nblocks(size)),
+                       dim3(kNeuronBlock), 0, (hipStream_t)stream, input,
+                       output, size, history, ramp);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_init_random(float *p, int n, int dist, float a,
+                                  float b, float val, int overwrite,
+                                  uint64_t seed, uint64_t step,
+                                  void *stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(init_random_kernel, dim3(nblocks(n)),
+                       dim3(kNeuronBlock), 0, (hipStream_t)stream, p, n, dist,
+                       a, b, val, overwrite, seed, step);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_set_f32(float *p, float val, long n, int overwrite,
+                              void *stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(set_f32_kernel, dim3(nblocks(n)), dim3(kNeuronBlock),
+                       0, (hipStream_t)stream, p, val, n, overwrite);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_transpose32(const void *src, void *dst, int rows,
+                                  int cols, void *stream) {
+    if (rows <= 0 || cols <= 0) return 0;
+    dim3 g((cols + kTile - 1) / kTile, (rows + kTile - 1) / kTile);
+    hipLaunchKernelGGL(transpose32_kernel, g, dim3(kTile, kRowsPerPass), 0,
+                       (hipStream_t)stream, (const uint32_t *)src,
+                       (uint32_t *)dst, rows, cols);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+    return n;
+}
+
+extern "C" const char *sg_hip_error_string(int code) {
+    return hipGetErrorString((hipError_t)code);
+}

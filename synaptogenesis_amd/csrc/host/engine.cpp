@@ -1,0 +1,644 @@
+#include "engine.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+
+#include "callbacks.h"
+
+namespace sg {
+
+namespace {
+
+std::mutex g_engine_lock;
+bool g_running = false;
+std::atomic<bool> g_interrupt(false);
+
+enum { IO_INPUT = 1, IO_OUTPUT = 2 };
+
+// Pinned host staging buffer of one (layer, key).
+struct HostBuf {
+    float *p = nullptr;
+    int n = 0;
+    bool dirty = false;
+    ~HostBuf() { if (p) (void)hipHostFree(p); }
+    void alloc(int size) {
+        n = size;
+        check_hip(hipHostMalloc((void **)&p, (size_t)size * 4,
+                                hipHostMallocDefault), "hipHostMalloc");
+        std::memset(p, 0, (size_t)size * 4);
+    }
+};
+
+struct ModuleLayer {
+    LayerState *ls;
+    int io = 0;
+    std::set<std::string> in_keys, out_keys;
+    IoCallback fn = nullptr;
+    int id = 0;
+};
+
+// The "callback" module (io/impl/callback_module.cpp, io/module.cpp:84-110).
+struct Module {
+    std::vector<ModuleLayer> layers;
+    bool clear_first = false;
+    int start_delay = 0, cutoff = 0, rate = 1, curr = 0;
+};
+
+void check_layer_model_supports(LayerState *ls, const std::string &key) {
+    if (key == "input" || key == "output") return;
+    if (!ls->vars.count(key))
+        throw Error("Failed to retrieve neuron data \"" + key + "\"");
+}
+
+}  // namespace
+
+void Engine::interrupt() {
+    std::lock_guard<std::mutex> g(g_engine_lock);
+    if (g_running) g_interrupt = true;
+}
+
+Props *Engine::run(const Props &args) {
+    {
+        std::lock_guard<std::mutex> g(g_engine_lock);
+        if (g_running) throw Error("Cannot run more than one engine at once!");
+        g_running = true;
+        g_interrupt = false;
+    }
+    struct Release {
+        ~Release() {
+            std::lock_guard<std::mutex> g(g_engine_lock);
+            g_running = false;
+            g_interrupt = false;
+        }
+    } release;
+
+    // Devices: GPUs are 0..n-1, the host is n (resource_manager.h:53).
+    int ngpu = sg_hip_device_count();
+    if (ngpu <= 0)
+        throw Error("No HIP device available: this library runs the "
+                    "simulation on an MI355X GPU");
+    std::vector<std::string> devs;
+    if (args.has("devices")) devs.push_back(args.get("devices"));
+    for (auto &d : args.get_array("devices")) devs.push_back(d);
+    int device = 0;
+    for (auto &d : devs) {
+        int id = std::stoi(d);
+        if (id >= ngpu)
+            throw Error("Device " + d + " is not a GPU: this library "
+                        "simulates on GPUs only");
+        device = id;
+    }
+    check_hip(hipSetDevice(device), "hipSetDevice");
+
+    state_->to_device();
+
+    const bool learning = args.get_bool("learning flag", true);
+    const bool suppress = args.get_bool("suppress output", false);
+    const int env_rate = std::max(1, args.get_int("environment rate", 1));
+    long iterations = args.has("iterations") ? args.get_int("iterations", 1)
+                                             : 0;
+
+    // ---- environment: modules and buffers --------------------------------
+    std::vector<Module> modules;
+    std::map<LayerState *, int> io_types;
+    std::map<LayerState *, std::set<std::string>> in_keys, out_keys;
+    if (env_) {
+        for (auto *mc : env_->get_child_array("modules")) {
+            if (mc->get_bool("skip", false)) continue;
+            const std::string type = mc->get("type", "");
+            if (type != "callback")
+                throw Error("Unrecognized module: " + type + "!");
+            Module m;
+            m.clear_first = mc->get_bool("clear", false);
+            m.start_delay = mc->get_int("delay", 0);
+            m.cutoff = mc->get_int("cutoff", 0);
+            m.rate = mc->get_int("rate", 1);
+            if (m.start_delay < 0 || m.cutoff < 0 || m.rate < 1)
+                throw Error("Invalid module delay / cutoff / rate");
+            for (auto *lc : mc->get_child_array("layers")) {
+                Layer *l = lc->has("structure")
+                    ? (net_->structure(lc->get("structure"))
+                           ? net_->structure(lc->get("structure"))
+                                 ->layer(lc->get("layer"))
+                           : nullptr)
+                    : net_->layer(lc->get("layer"));
+                if (!l) throw Error("Could not find layer " + lc->get("layer"));
+                ModuleLayer ml;
+                ml.ls = state_->layer_state(l);
+                if (lc->get_bool("input", false)) ml.io |= IO_INPUT;
+                if (lc->get_bool("output", false)) ml.io |= IO_OUTPUT;
+                for (auto &k : lc->get_array("input keys")) {
+                    ml.in_keys.insert(k);
+                    ml.io |= IO_INPUT;
+                }
+                for (auto &k : lc->get_array("output keys")) {
+                    ml.out_keys.insert(k);
+                    ml.io |= IO_OUTPUT;
+                }
+                if (ml.io == 0)
+                    throw Error("Module has zero IO type for layer " + l->name);
+                if (ml.in_keys.empty() && (ml.io & IO_INPUT))
+                    ml.in_keys.insert("input");
+                if (ml.out_keys.empty() && (ml.io & IO_OUTPUT))
+                    ml.out_keys.insert("output");
+                for (auto &k : ml.in_keys) check_layer_model_supports(ml.ls, k);
+                for (auto &k : ml.out_keys) check_layer_model_supports(ml.ls, k);
+                if (!lc->has("function"))
+                    throw Error("Unspecified callback function for layer " +
+                                l->name + " in CallbackModule!");
+                if (!lc->has("id"))
+                    throw Error("Unspecified callback id for layer " +
+                                l->name + " in CallbackModule!");
+                auto &cb = Callbacks::get();
+                ml.fn = cb.find(cb.io, lc->get("function"), "io");
+                ml.id = lc->get_int("id", 0);
+                io_types[ml.ls] |= ml.io;
+                for (auto &k : ml.in_keys) in_keys[ml.ls].insert(k);
+                for (auto &k : ml.out_keys) out_keys[ml.ls].insert(k);
+                m.layers.push_back(ml);
+            }
+            if (!m.layers.empty()) modules.push_back(m);
+        }
+    }
+    for (auto &kv : io_types) {
+        if ((kv.second & IO_INPUT) && in_keys[kv.first].empty())
+            in_keys[kv.first].insert("input");
+        if ((kv.second & IO_OUTPUT) && out_keys[kv.first].empty())
+            out_keys[kv.first].insert("output");
+    }
+    auto is_input = [&](LayerState *ls) {
+        auto it = io_types.find(ls);
+        return it != io_types.end() && (it->second & IO_INPUT);
+    };
+
+    std::map<std::pair<LayerState *, std::string>, std::unique_ptr<HostBuf>>
+        in_bufs, out_bufs;
+    std::map<LayerState *, float *> inter;   // last fed input, on device
+    std::vector<void *> dev_allocs;
+    for (auto &kv : in_keys)
+        for (auto &k : kv.second) {
+            in_bufs[{kv.first, k}].reset(new HostBuf());
+            in_bufs[{kv.first, k}]->alloc(kv.first->layer->size);
+            if (k == "input") {
+                float *d = nullptr;
+                check_hip(hipMalloc((void **)&d,
+                                    (size_t)kv.first->layer->size * 4),
+                          "hipMalloc");
+                check_hip(hipMemset(d, 0, (size_t)kv.first->layer->size * 4),
+                          "hipMemset");
+                inter[kv.first] = d;
+                dev_allocs.push_back(d);
+            }
+        }
+    for (auto &kv : out_keys)
+        for (auto &k : kv.second) {
+            out_bufs[{kv.first, k}].reset(new HostBuf());
+            out_bufs[{kv.first, k}]->alloc(kv.first->layer->size);
+        }
+    struct FreeAll {
+        std::vector<void *> &v;
+        ~FreeAll() { for (auto *p : v) (void)hipFree(p); }
+    } free_all{dev_allocs};
+
+    hipStream_t stream;
+    check_hip(hipStreamCreate(&stream), "hipStreamCreate");
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } sg_{stream};
+
+    // ---- per-connection launch helpers -----------------------------------
+    auto conn_desc = [&](ConnState &cs) {
+        Connection &c = *cs.conn;
+        sg_conn d;
+        std::memset(&d, 0, sizeof(d));
+        d.type = c.type;
+        d.opcode = c.opcode;
+        d.out_type = cs.from->out_type;
+        d.delay = c.delay;
+        d.convolutional = c.convolutional;
+        d.from_size = c.from->size; d.from_rows = c.from->rows;
+        d.from_cols = c.from->cols;
+        d.to_size = c.to->size; d.to_rows = c.to->rows; d.to_cols = c.to->cols;
+        d.row_field = c.arbor.row_field; d.col_field = c.arbor.col_field;
+        d.row_stride = c.arbor.row_stride; d.col_stride = c.arbor.col_stride;
+        d.row_spacing = c.arbor.row_spacing;
+        d.col_spacing = c.arbor.col_spacing;
+        d.row_offset = c.arbor.row_offset; d.col_offset = c.arbor.col_offset;
+        d.wrap = c.arbor.wrap;
+        d.from_row_start = c.subset.from_row_start;
+        d.from_row_end = c.subset.from_row_end;
+        d.from_col_start = c.subset.from_col_start;
+        d.from_col_end = c.subset.from_col_end;
+        d.to_row_start = c.subset.to_row_start;
+        d.to_row_end = c.subset.to_row_end;
+        d.to_col_start = c.subset.to_col_start;
+        d.to_col_end = c.subset.to_col_end;
+        d.k_per_dest = cs.k_per_dest;
+        d.num_dest = cs.num_dest;
+        d.num_weights = c.num_weights;
+        const int per_word = cs.from->out_type == SG_OUT_BIT ? 32 : 1;
+        d.src = cs.from->output.du() +
+                (size_t)(c.delay / per_word) * c.from->size;
+        d.dst = cs.to->input.df();
+        d.weights = cs.arr("weights").df();
+        return d;
+    };
+    auto iz_syn = [&](ConnState &cs) {
+        sg_iz_syn s;
+        std::memset(&s, 0, sizeof(s));
+        s.weights = cs.arr("weights").df();
+        s.short_traces = cs.arr("short trace").df();
+        s.long_traces = cs.arr("long trace").df();
+        s.presyn_traces = cs.arr("presyn trace").df();
+        s.stp_r = cs.arr("short term R").df();
+        s.stp_w = cs.arr("short term w").df();
+        s.weight_derivatives = cs.arr("weight derivatives").df();
+        s.delay_rank = cs.d_delay_rank;
+        s.tss_cur = cs.d_tss[cs.tss_cur];
+        s.tss_next = cs.d_tss[1 - cs.tss_cur];
+        s.visits = cs.d_visits;
+        s.baseline_conductance = cs.baseline_conductance;
+        s.learning_rate = cs.learning_rate;
+        s.max_weight = cs.conn->max_weight;
+        s.stp_flag = cs.stp_flag;
+        return s;
+    };
+    auto iz_dest = [&](ConnState &cs, int kind) {
+        sg_iz_dest d;
+        std::memset(&d, 0, sizeof(d));
+        LayerState &t = *cs.to;
+        d.out_word0 = t.output.du();
+        d.post_trace = t.var("post trace").df();
+        d.time_since_spike = t.var("time since spike").di();
+        d.voltage = t.var("voltage").df();
+        d.input = t.input.df();
+        // ACTIV_EXTRACTIONS_SHORT / _LONG taus are floats
+        // (izhikevich_attributes.cpp:219-249).
+        switch (kind) {
+            case SG_IZ_ADD:
+                d.short_cond = t.var("ampa").df(); d.short_tau = (float)0.8;
+                d.long_cond = t.var("nmda").df(); d.long_tau = (float)0.993;
+                break;
+            case SG_IZ_SUB:
+                d.short_cond = t.var("gabaa").df(); d.short_tau = (float)0.833;
+                d.long_cond = t.var("gabab").df(); d.long_tau = (float)0.993;
+                break;
+            case SG_IZ_MULT:
+                d.short_cond = t.var("mult").df(); d.short_tau = (float)0.95;
+                break;
+            case SG_IZ_REWARD:
+                d.short_cond = t.var("dopamine").df(); d.short_tau = (float)0.95;
+                break;
+        }
+        return d;
+    };
+    auto fusable = [&](ConnState &cs) {
+        return cs.iz && !cs.direct && cs.iz_kind >= 0 &&
+               cs.iz_kind != SG_IZ_GAP && cs.update_kind >= 0;
+    };
+    auto activate = [&](ConnState &cs) {
+        if (cs.fused_gol) return;
+        sg_conn d = conn_desc(cs);
+        if (!cs.iz || cs.direct) {
+            check_hip(sg_hip_activate_base(&d, stream), "activate");
+            return;
+        }
+        sg_iz_dest dd = iz_dest(cs, cs.iz_kind);
+        sg_iz_syn s = iz_syn(cs);
+        if (cs.iz_kind == SG_IZ_GAP) {
+            check_hip(sg_hip_iz_activate(&d, &s, &dd, SG_IZ_GAP, stream),
+                      "gap activate");
+            return;
+        }
+        check_hip(sg_hip_tss_prepass(s.tss_next, s.tss_cur, cs.d_visits,
+                                     cs.conn->from->size, stream),
+                  "tss prepass");
+        if (cs.pending_update) {
+            check_hip(sg_hip_iz_update_activate(
+                          &d, &s, &dd, cs.iz_kind, cs.update_kind,
+                          state_->d_src_table(), state_->d_dst_table(),
+                          stream),
+                      "update+activate");
+            cs.pending_update = false;
+        } else {
+            check_hip(sg_hip_iz_activate(&d, &s, &dd, cs.iz_kind, stream),
+                      "iz activate");
+        }
+        cs.tss_cur = 1 - cs.tss_cur;
+    };
+    auto update = [&](ConnState &cs) {
+        sg_conn d = conn_desc(cs);
+        sg_iz_syn s = iz_syn(cs);
+        sg_iz_dest dd = iz_dest(cs, cs.iz_kind >= 0 ? cs.iz_kind : SG_IZ_ADD);
+        check_hip(sg_hip_iz_update(&d, &s, &dd, cs.update_kind,
+                                   state_->d_src_table(),
+                                   state_->d_dst_table(), stream),
+                  "iz update");
+    };
+
+    // ---- game-of-life fast path -------------------------------------------
+    // A game-of-life layer whose only input is one 3x3 convolutional
+    // self-connection (stride 1, offsets -1, add, delay 0), with one history
+    // word and no environment I/O, is stepped by sg_hip_gol_step.
+    std::map<LayerState *, ConnState *> gol_fast;
+    std::map<LayerState *, uint32_t *> gol_spare, gol_home;
+    for (auto &lsp : state_->layers()) {
+        LayerState *ls = lsp.get();
+        if (ls->model != Model::GAME_OF_LIFE || ls->history != 1) continue;
+        if (io_types.count(ls) || ls->layer->config.has_child("init config"))
+            continue;
+        if (ls->layer->inputs.size() != 1) continue;
+        Connection *c = ls->layer->inputs[0];
+        const ArborConfig &a = c->arbor;
+        if (c->from != ls->layer || c->type != SG_CONVERGENT ||
+            !c->convolutional || c->opcode != SG_ADD || c->delay != 0 ||
+            a.row_field != 3 || a.col_field != 3 || a.row_stride != 1 ||
+            a.col_stride != 1 || a.row_spacing != 1 || a.col_spacing != 1 ||
+            a.row_offset != -1 || a.col_offset != -1)
+            continue;
+        ConnState *cs = state_->conn_state(c);
+        cs->fused_gol = true;
+        gol_fast[ls] = cs;
+        uint32_t *spare = nullptr;
+        check_hip(hipMalloc((void **)&spare, ls->output.bytes()), "hipMalloc");
+        dev_allocs.push_back(spare);
+        gol_spare[ls] = spare;
+        gol_home[ls] = ls->output.du();
+    }
+    struct GolReset {
+        std::map<LayerState *, ConnState *> &m;
+        ~GolReset() { for (auto &kv : m) kv.second->fused_gol = false; }
+    } gol_reset{gol_fast};
+
+    // ---- init configs (instruction.cpp:3-30) ------------------------------
+    auto init_layer = [&](LayerState *ls, bool overwrite_default, long step) {
+        Props *ic = ls->layer->config.get_child("init config");
+        const int n = ls->layer->size;
+        float *in = ls->input.df();
+        if (!ic) {
+            if (overwrite_default)
+                check_hip(sg_hip_set_f32(in, 0.0f, n, 1, stream), "clear");
+            return;
+        }
+        const bool ow = overwrite_default;
+        const std::string type = ic->get("type", "normal");
+        const uint64_t seed = ls->layer->id;
+        if (type == "flat") {
+            check_hip(sg_hip_set_f32(in, ic->get_float("value", 1.0f), n, ow,
+                                     stream), "init flat");
+        } else if (type == "uniform") {
+            check_hip(sg_hip_init_random(in, n, 0, ic->get_float("min", 0.0f),
+                                         ic->get_float("max", 1.0f), 0, ow,
+                                         seed, step, stream), "init uniform");
+        } else if (type == "normal") {
+            check_hip(sg_hip_init_random(in, n, 1,
+                                         ic->get_float("mean", 1.0f),
+                                         ic->get_float("std dev", 0.1f), 0, ow,
+                                         seed, step, stream), "init normal");
+        } else if (type == "poisson") {
+            const float rate = (float)(0.001 * ic->get_float("rate", 1.0f));
+            check_hip(sg_hip_init_random(in, n, 2, rate, 0.0f,
+                                         ic->get_float("value", 20.0f), ow,
+                                         seed, step, stream), "init poisson");
+        } else {
+            throw Error("Unrecognized init type: " + type);
+        }
+    };
+
+    // ---- main loop --------------------------------------------------------
+    auto &layers = state_->layers();
+    long done = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool interrupted = false;
+    for (long it = 0; iterations == 0 || it < iterations; ++it) {
+        // 1. pre-input
+        for (auto &lsp : layers) {
+            LayerState *ls = lsp.get();
+            if (is_input(ls) || gol_fast.count(ls)) continue;
+            init_layer(ls, true, it);
+            for (auto *c : ls->layer->inputs) activate(*state_->conn_state(c));
+        }
+        // 2. environment input
+        bool any_input = false;
+        for (auto &m : modules) {
+            if (m.start_delay > 0) continue;
+            for (auto &ml : m.layers) {
+                if (!(ml.io & IO_INPUT)) continue;
+                if (m.cutoff == 0 || m.curr < m.cutoff) {
+                    if (m.curr % m.rate != 0) continue;
+                    for (auto &k : ml.in_keys) {
+                        if (!any_input) {
+                            check_hip(hipStreamSynchronize(stream), "sync");
+                            any_input = true;
+                        }
+                        HostBuf &b = *in_bufs.at({ml.ls, k});
+                        if (m.clear_first) std::memset(b.p, 0, (size_t)b.n * 4);
+                        ml.fn(ml.id, ml.ls->layer->size, b.p);
+                        b.dirty = true;
+                    }
+                } else if (m.curr == m.cutoff) {
+                    if (!any_input) {
+                        check_hip(hipStreamSynchronize(stream), "sync");
+                        any_input = true;
+                    }
+                    HostBuf &b = *in_bufs.at({ml.ls, "input"});
+                    std::memset(b.p, 0, (size_t)b.n * 4);
+                    b.dirty = true;
+                }
+            }
+        }
+        for (auto &kv : in_keys) {
+            LayerState *ls = kv.first;
+            const size_t bytes = (size_t)ls->layer->size * 4;
+            for (auto &k : kv.second) {
+                HostBuf &b = *in_bufs.at({ls, k});
+                if (k == "input") {
+                    if (b.dirty)
+                        check_hip(hipMemcpyAsync(inter[ls], b.p, bytes,
+                                                 hipMemcpyHostToDevice, stream),
+                                  "input H2D");
+                    b.dirty = false;
+                    check_hip(hipMemcpyAsync(ls->input.dev, inter[ls], bytes,
+                                             hipMemcpyDeviceToDevice, stream),
+                              "input D2D");
+                } else {
+                    check_hip(hipMemcpyAsync(ls->var(k).dev, b.p, bytes,
+                                             hipMemcpyHostToDevice, stream),
+                              "aux input H2D");
+                }
+            }
+        }
+        // 3. post-input
+        for (auto &lsp : layers) {
+            LayerState *ls = lsp.get();
+            if (!is_input(ls)) continue;
+            init_layer(ls, false, it);
+            for (auto *c : ls->layer->inputs) activate(*state_->conn_state(c));
+        }
+        // 4. neuron updates
+        const bool last = iterations != 0 && it + 1 == iterations;
+        for (auto &lsp : layers) {
+            LayerState *ls = lsp.get();
+            const int n = ls->layer->size;
+            switch (ls->model) {
+                case Model::IZHIKEVICH: {
+                    sg_iz_layer l;
+                    l.size = n; l.history = ls->history;
+                    l.input = ls->input.df(); l.output = ls->output.du();
+                    l.ampa = ls->var("ampa").df(); l.nmda = ls->var("nmda").df();
+                    l.gabaa = ls->var("gabaa").df();
+                    l.gabab = ls->var("gabab").df();
+                    l.mult = ls->var("mult").df();
+                    l.dopamine = ls->var("dopamine").df();
+                    l.voltage = ls->var("voltage").df();
+                    l.recovery = ls->var("recovery").df();
+                    l.post_trace = ls->var("post trace").df();
+                    l.time_since_spike = ls->var("time since spike").di();
+                    l.a = ls->var("a").df(); l.b = ls->var("b").df();
+                    l.c = ls->var("c").df(); l.d = ls->var("d").df();
+                    check_hip(sg_hip_iz_neuron(&l, stream), "iz neuron");
+                    break;
+                }
+                case Model::GAME_OF_LIFE: {
+                    auto gf = gol_fast.find(ls);
+                    if (gf != gol_fast.end()) {
+                        uint32_t *spare = gol_spare[ls];
+                        float k9[9];
+                        std::memcpy(k9, gf->second->arr("weights").f(), 36);
+                        check_hip(sg_hip_gol_step(
+                                      ls->output.du(), spare,
+                                      last ? ls->input.df() : nullptr,
+                                      ls->layer->rows, ls->layer->cols,
+                                      gf->second->conn->arbor.wrap, k9,
+                                      ls->survival_min, ls->survival_max,
+                                      ls->birth_min, ls->birth_max, stream),
+                                  "gol step");
+                        gol_spare[ls] = ls->output.du();
+                        ls->output.dev = spare;
+                    } else {
+                        check_hip(sg_hip_gol_neuron(
+                                      ls->input.df(), ls->output.du(), n,
+                                      ls->history, ls->survival_min,
+                                      ls->survival_max, ls->birth_min,
+                                      ls->birth_max, stream),
+                                  "gol neuron");
+                    }
+                    break;
+                }
+                case Model::RATE_ENCODING:
+                    check_hip(sg_hip_rate_neuron(ls->input.df(),
+                                                 ls->output.df(), n,
+                                                 ls->history, stream),
+                              "rate neuron");
+                    break;
+                case Model::RELAY:
+                    check_hip(sg_hip_relay_neuron(ls->input.df(),
+                                                  ls->output.df(), n,
+                                                  ls->history, ls->ramp,
+                                                  stream),
+                              "relay neuron");
+                    break;
+            }
+        }
+        // 5. weight updates
+        if (learning) {
+            for (auto &csp : state_->conns()) {
+                ConnState &cs = *csp;
+                if (cs.update_kind < 0) continue;
+                if (fusable(cs)) cs.pending_update = true;
+                else update(cs);
+            }
+        }
+        // 6. output
+        if (it % env_rate == 0 && !suppress && !out_keys.empty()) {
+            for (auto &kv : out_keys) {
+                LayerState *ls = kv.first;
+                const size_t bytes = (size_t)ls->layer->size * 4;
+                for (auto &k : kv.second) {
+                    HostBuf &b = *out_bufs.at({ls, k});
+                    const void *src = k == "output" ? ls->output.dev
+                                    : k == "input" ? ls->input.dev
+                                    : ls->var(k).dev;
+                    check_hip(hipMemcpyAsync(b.p, src, bytes,
+                                             hipMemcpyDeviceToHost, stream),
+                              "output D2H");
+                }
+            }
+            check_hip(hipStreamSynchronize(stream), "output sync");
+            for (auto &m : modules) {
+                if (m.start_delay > 0) continue;
+                if (!(m.cutoff == 0 || m.curr < m.cutoff)) continue;
+                if (m.curr % m.rate != 0) continue;
+                for (auto &ml : m.layers) {
+                    if (!(ml.io & IO_OUTPUT)) continue;
+                    for (auto &k : ml.out_keys)
+                        ml.fn(ml.id, ml.ls->layer->size,
+                              out_bufs.at({ml.ls, k})->p);
+                }
+            }
+        }
+        // modules cycle (io/module.cpp:98-108)
+        for (auto &m : modules) {
+            if (m.start_delay <= 0) {
+                if (m.cutoff == 0 || m.curr < m.cutoff) ++m.curr;
+                else if (m.cutoff != 0 && m.curr == m.cutoff) ++m.curr;
+            } else {
+                --m.start_delay;
+            }
+        }
+        check_hip(hipGetLastError(), "step");
+        ++done;
+        if (g_interrupt) {
+            interrupted = true;
+            break;
+        }
+    }
+    // Apply updates still deferred into a next activation.
+    for (auto &csp : state_->conns())
+        if (csp->pending_update) {
+            update(*csp);
+            csp->pending_update = false;
+        }
+    // Leave double-buffered game-of-life words in the state's own buffer.
+    for (auto &kv : gol_fast) {
+        LayerState *ls = kv.first;
+        if (ls->output.dev != gol_home[ls]) {
+            check_hip(hipMemcpyAsync(gol_home[ls], ls->output.dev,
+                                     ls->output.bytes(),
+                                     hipMemcpyDeviceToDevice, stream),
+                      "gol restore");
+            ls->output.dev = gol_home[ls];
+        }
+    }
+    check_hip(hipStreamSynchronize(stream), "final sync");
+    const double secs = std::chrono::duration<double>(
+        std::chrono::steady_clock::now() - t0).count();
+
+    // Report (report.cpp:8-16, engine.cpp:584-590).
+    Props *r = new Props();
+    r->set("iterations", std::to_string(done));
+    r->set("total time", std::to_string((float)secs));
+    r->set("average time", std::to_string((float)(secs / (double)done)));
+    r->set("fps", std::to_string((float)((double)done / secs)));
+    r->set("network bytes", std::to_string(state_->network_bytes()));
+    size_t buf_bytes = 0;
+    for (auto &kv : in_bufs) buf_bytes += (size_t)kv.second->n * 4;
+    for (auto &kv : out_bufs) buf_bytes += (size_t)kv.second->n * 4;
+    r->set("state buffer bytes", std::to_string(inter.size() ? buf_bytes : 0));
+    r->set("engine buffer bytes", std::to_string(buf_bytes));
+    r->set_child("args", args);
+    r->set("interrupted", interrupted ? "true" : "false");
+    Props mem;
+    mem.set("device", std::to_string(device));
+    mem.set("bytes", std::to_string(state_->network_bytes()));
+    r->add_to_child_array("memory usage", mem);
+    return r;
+}
+
+}  // namespace sg

@@ -1,0 +1,88 @@
+"""Comparison of a run against a golden fixture written by
+tests/golden/make_golden.py."""
+
+import json
+import os
+
+import numpy as np
+
+import harness
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load(name):
+    return np.load(os.path.join(HERE, "golden", name + ".npz"))
+
+
+def compare(case, captures, finals, fixture, exact=True, rtol=0.0,
+            atol=0.0):
+    """Return a list of human-readable mismatches (empty when equal)."""
+    problems = []
+    for k in case.capture:
+        key = "cap|" + "|".join(k)
+        want = fixture[key + "|digests"]
+        got = captures[k]
+        if len(got) != len(want):
+            problems.append("%s: %d steps, fixture has %d" %
+                            (key, len(got), len(want)))
+            continue
+        for t, arr in enumerate(got):
+            if harness.digest(arr).encode() == want[t]:
+                continue
+            msg = "%s: first differing step %d" % (key, t)
+            if key + "|data" in fixture.files:
+                steps = (fixture[key + "|steps"].tolist()
+                         if key + "|steps" in fixture.files
+                         else list(range(len(want))))
+                if t in steps:
+                    ref = fixture[key + "|data"][steps.index(t)]
+                    msg += _diff(arr, ref)
+            if not exact and key + "|data" in fixture.files:
+                ref_all = fixture[key + "|data"].view(np.float32)
+                steps = (fixture[key + "|steps"].tolist()
+                         if key + "|steps" in fixture.files
+                         else list(range(len(want))))
+                if all(np.allclose(got[s].view(np.float32), ref_all[i],
+                                   rtol=rtol, atol=atol, equal_nan=True)
+                       for i, s in enumerate(steps)):
+                    break
+            problems.append(msg)
+            break
+    for k, arr in finals.items():
+        key = "fin|" + "|".join(k)
+        want = fixture[key + "|digest"][0]
+        if harness.digest(arr).encode() == want:
+            continue
+        ref = fixture[key + "|data"]
+        got = arr[fixture[key + "|index"]] if key + "|index" in \
+            fixture.files else arr
+        if not exact and np.allclose(got.view(np.float32),
+                                     ref.view(np.float32), rtol=rtol,
+                                     atol=atol, equal_nan=True):
+            continue
+        problems.append("%s differs%s" % (key, _diff(got, ref)))
+    return problems
+
+
+def _diff(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape:
+        return " (shape %s vs %s)" % (a.shape, b.shape)
+    ne = np.flatnonzero(a.view(np.uint32) != b.view(np.uint32)) \
+        if a.dtype.itemsize == 4 else np.flatnonzero(a != b)
+    if ne.size == 0:
+        return ""
+    i = ne[0]
+    s = " (%d/%d elements; first at %d: %r vs %r" % (ne.size, a.size, i,
+                                                       a[i], b[i])
+    if a.dtype == np.float32:
+        with np.errstate(invalid="ignore"):
+            d = np.abs(a.astype(np.float64) - b.astype(np.float64))
+            s += "; max abs diff %.3g" % np.nanmax(d)
+    return s + ")"
+
+
+def meta(fixture):
+    return json.loads(str(fixture["meta"]))

@@ -1,0 +1,58 @@
+"""GPU parity: every golden case run through the C-ABI of the MI355X
+library and compared with the reference's serial CPU results bit for bit
+(rate-encoding: to the tolerance of its tanh, see below)."""
+
+import copy
+
+import numpy as np
+import pytest
+
+import golden_check
+import harness
+import netdefs
+
+CASES = {c.name: c for c in netdefs.all_cases()}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", [n for n in CASES if n != "rate"])
+def test_golden_bit_exact(gpu_lib, name):
+    case = CASES[name]
+    print("case", name, flush=True)
+    captures, finals, report = harness.run_case(case, gpu_lib)
+    if case.steps:
+        assert report is not None and report.get("iterations") == \
+            str(case.steps)
+    problems = golden_check.compare(case, captures, finals,
+                                    golden_check.load(name))
+    assert not problems, "\n".join(problems)
+
+
+@pytest.mark.gpu
+def test_golden_rate_encoding(gpu_lib):
+    """rate_encoding uses tanh; the reference calls glibc tanhf, the kernel
+    rounds an FP64 tanh once to FP32.  Both are within 1 ulp of the true
+    value, so outputs may differ by an ulp and propagate through the next
+    layer's weighted sum: tolerance rtol 1e-5, atol 1e-6."""
+    case = CASES["rate"]
+    captures, finals, _ = harness.run_case(case, gpu_lib)
+    problems = golden_check.compare(case, captures, finals,
+                                    golden_check.load("rate"), exact=False,
+                                    rtol=1e-5, atol=1e-6)
+    assert not problems, "\n".join(problems)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["gol_small", "gol_conway", "gol_nowrap"])
+def test_golden_gol_fused_path(gpu_lib, name):
+    """Without per-step capture the engine steps a 3x3 convolutional
+    game-of-life layer with the fused kernel; its final words must equal the
+    fixture's last captured step."""
+    case = copy.copy(CASES[name])
+    case.capture = []
+    case.final_neuron = [("gol", "board", "output")]
+    _, finals, _ = harness.run_case(case, gpu_lib)
+    fx = golden_check.load(name)
+    last = fx["cap|gol|board|output|digests"][-1]
+    got = finals[("neuron", "gol", "board", "output")]
+    assert harness.digest(got).encode() == last
