@@ -1,0 +1,193 @@
+"""Throughput of the MI355X simulation core on the Izhikevich E/I sheet.
+
+One "step" is one simulation timestep of the whole network: every synapse
+activation (E->E on the conductance path with STDP + short-term plasticity
+fused into the same pass, E->I and I->E direct), every neuron update and
+every weight update, with the state resident in HBM.  The metric is synaptic
+updates per second (visited (source, destination) pairs per step x steps /
+time); neuron-steps per second is reported beside it.
+
+Timed region: one run() of --steps iterations through the C-ABI after a
+--warmup run that moved the state to the GPU.  A second run of the same
+length records HIP events around every kernel launch on the engine's stream
+to get the dominant kernel's average duration for the roofline.  The CPU
+baseline times the reference's serial build (oracle/_ref, compiled from the
+reference's sources) on a bounded number of steps of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from synaptogenesis_amd import syngen, workloads  # noqa: E402
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libsyngen_ref.so")
+HBM_PEAK_GBPS = 8000.0
+
+# Algorithmic bytes of one launch of the fused E->E update+activate kernel:
+# per synapse RW weight, stp R, stp w, short trace, long trace, presyn trace,
+# weight derivative (7 x 8 B) + R delay (4 B) = 60 B; per source neuron the
+# history word (4 B) and time-since-spike RW (8 B); per destination the
+# history word, post trace, time since spike (12 B) and ampa/nmda RW (16 B).
+def fused_bytes(ne):
+    return ne * 225 * 60 + ne * 12 + ne * 28
+
+
+# Algorithmic bytes of one whole step (every live array read once, written
+# once if mutated): E->E 60 B/synapse, E->I and I->E weights 4 B/synapse,
+# 112 B per Izhikevich neuron, source history words, E->E time since spike.
+def step_bytes(ne, ni):
+    return (ne * 225 * 60 + ni * 225 * 4 + ne * 16 * 4 + (ne + ni) * 112 +
+            ne * 8 + (ne + ne + ni) * 4)
+
+
+def build(lib, wl):
+    keep = []
+    net = json.loads(json.dumps(wl["net"]))
+    names = sorted(set(wl["weights"]) | set(wl["delays"]))
+    tag = "bench%x" % (id(wl) & 0xFFFF)
+    for i, c in enumerate(net["connections"]):
+        if c["name"] not in names:
+            continue
+        wc = dict(c.get("weight config", {}))
+        wc.update({"type": "flat", "weight": "0",
+                   "id": str(names.index(c["name"]))})
+        if c["name"] in wl["weights"]:
+            wc["callback"] = "W" + tag
+        if c["name"] in wl["delays"]:
+            wc["delay callback"] = "D" + tag
+        c["weight config"] = wc
+
+    def wcb(ID, size, ptr):
+        w = wl["weights"][names[ID]]()
+        assert w.size == size
+        np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)),
+                              shape=(size,))[:] = w
+
+    def dcb(ID, size, wptr, dptr):
+        d = wl["delays"][names[ID]]()
+        assert d.size == size
+        np.ctypeslib.as_array(ctypes.cast(dptr, ctypes.POINTER(ctypes.c_int)),
+                              shape=(size,))[:] = d
+
+    keep.append(lib.create_weight_callback("W" + tag, wcb))
+    keep.append(lib.create_delay_weight_callback("D" + tag, dcb))
+    network = syngen.Network(net, lib)
+    if not network.obj:
+        raise RuntimeError("network construction failed")
+    network.build_state()
+    for (s, l, key), arr in wl["init"].items():
+        view = network.get_neuron_data(s, l, key)
+        raw = np.ctypeslib.as_array(
+            ctypes.cast(view.data, ctypes.POINTER(ctypes.c_uint32)),
+            shape=(view.size,))
+        raw[:] = np.asarray(arr).view(np.uint32)
+    network._keep = keep
+    return network
+
+
+def run(network, steps, extra=None):
+    args = {"iterations": str(steps), "worker threads": "0",
+            "multithreaded": "false", "suppress output": "true",
+            "learning flag": "true", "verbose": "false"}
+    args.update(extra or {})
+    t0 = time.perf_counter()
+    report = network.run(None, args)
+    dt = time.perf_counter() - t0
+    if report.obj is None or report.get("iterations") != str(steps):
+        raise RuntimeError("run failed")
+    return dt, report
+
+
+def cpu_baseline(wl, steps):
+    if not os.path.exists(REF_LIB):
+        return None
+    lib = syngen.load_library(REF_LIB)
+    lib.set_suppress_output(True)
+    lib.set_warnings(False)
+    network = build(lib, wl)
+    run(network, 1)                      # first step outside the sample
+    dt, _ = run(network, steps)
+    network.free()
+    return {"value": wl["visits_per_step"] * steps / dt,
+            "unit": "synaptic updates/s", "cores": 1, "kind": "reference",
+            "sample": "%d steps of the full %s on the reference's serial "
+                      "CPU build (worker threads 0, single thread), "
+                      "%.2f s/step" % (steps, wl["name"], dt / steps)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build runs a network on a "
+              "single GPU (multi-GPU sharding is a design note only); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    wl = workloads.sheet()
+    lib = syngen.load_library()
+    lib.set_suppress_output(True)
+    lib.set_warnings(False)
+    network = build(lib, wl)
+    run(network, a.warmup)
+    dt, report = run(network, a.steps)
+    _, kreport = run(network, a.steps, {"kernel timing": "true"})
+    timing = kreport.children["kernel timing"].properties
+    network.free()
+
+    ne, ni = 256 * 320, 128 * 160
+    ms_per_step = dt * 1000.0 / a.steps
+    syn_per_s = wl["visits_per_step"] * a.steps / dt
+    fused_ms = float(timing["iz_update_activate ms"]) / \
+        float(timing["iz_update_activate launches"])
+    achieved = fused_bytes(ne) / (fused_ms * 1e-3) / 1e9
+    kernels = {k[:-3]: float(v) / float(timing[k[:-3] + " launches"])
+               for k, v in timing.items() if k.endswith(" ms")}
+
+    cpu = None if a.no_cpu else cpu_baseline(wl, a.cpu_steps)
+    out = {
+        "metric": "synaptic updates/sec (neuron-steps/sec beside it) on 1 "
+                  "MI355X; % HBM roofline",
+        "value": syn_per_s,
+        "unit": "synaptic updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (Philox-seeded weights U[0,0.5), delays "
+                "U{0..31}; device-drawn N(0,3) noise)",
+        "config": {"workload": wl["name"] + " (E->E conv 15x15 s1 wrap "
+                   "STDP+STP, E->I conv 15x15 s2 direct, I->E div 9x9 s2 "
+                   "direct)", "synapse_visits_per_step": wl["visits_per_step"],
+                   "neurons": wl["neurons"]},
+        "neuron_steps_per_s": wl["neurons"] * a.steps / dt,
+        "step_hbm_fraction": step_bytes(ne, ni) / (ms_per_step * 1e-3) /
+                             1e9 / HBM_PEAK_GBPS,
+        "roofline": {"bound": "hbm", "kernel": "iz_update_activate (E->E)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None, "avg_ms": fused_ms},
+        "kernel_avg_ms": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

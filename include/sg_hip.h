@@ -63,21 +63,26 @@ typedef struct sg_conn {
     const float *weights;
 } sg_conn;
 
+/* Fields of the interleaved per-synapse record of a tiled Izhikevich
+ * connection: for destination tile t and visit k the 7 fields' 64 lanes are
+ * consecutive, ((t * K + k) * SG_IZ_FIELDS + field) * 64 + lane. */
+enum { SG_IZ_W = 0, SG_IZ_R = 1, SG_IZ_SW = 2, SG_IZ_PRE = 3, SG_IZ_DW = 4,
+       SG_IZ_ST = 5, SG_IZ_LT = 6, SG_IZ_FIELDS = 7 };
+
 /* Per-synapse and per-source state of an Izhikevich conductance connection
- * (state/impl/izhikevich_attributes.h:45-66). */
+ * (state/impl/izhikevich_attributes.h:45-66).  Tiled connection types use
+ * `record` (interleaved, above); one-to-one connections use the separate
+ * arrays. */
 typedef struct sg_iz_syn {
+    float *record;
     float *weights;
     float *short_traces, *long_traces, *presyn_traces;
     float *stp_r, *stp_w, *weight_derivatives;
-    /* delay & 31 in bits 0-7, the number of later visits of the same
-     * source in the reference's serial order (capped at 32) in bits 8-15 */
-    const uint16_t *delay_rank;
-    /* time since spike per source: value after the last activation, and
-     * the buffer the next activation writes */
+    /* per-synapse delay & 31 (the bit of the source word the synapse reads) */
+    const uint8_t *delays;
+    /* time since spike per source after the last activation (read by the
+     * updater) */
     const int32_t *tss_cur;
-    int32_t *tss_next;
-    /* min(32, visits per source) */
-    const uint8_t *visits;
     float baseline_conductance;
     float learning_rate;
     float max_weight;
@@ -117,9 +122,16 @@ typedef struct sg_iz_layer {
  * extractor.cpp:8-15, aggregator.cpp:15-48). */
 int sg_hip_activate_base(const sg_conn *c, void *stream);
 
-/* time-since-spike pre-pass: tss_next = min(32, tss_cur + visits). */
-int sg_hip_tss_prepass(int32_t *tss_next, const int32_t *tss_cur,
-                       const uint8_t *visits, int n, void *stream);
+/* Time since spike of every source after one activation of an Izhikevich
+ * conductance connection (izhikevich_attributes.cpp:258-262, applied once
+ * per visit in the reference's serial order).  A source visited m times
+ * ends at min(32, m - j) if its j-th (1-based) visit is the last one that
+ * sees a spike, else at min(32, tss + m); only the last 32 visits can
+ * matter.  rev_delays[f * 32 + i] is the delay (& 31) of the i-th visit of
+ * source f counted back from its last one, visits[f] = min(32, m). */
+int sg_hip_tss_step(int32_t *tss_next, const int32_t *tss_cur,
+                    const uint8_t *visits, const uint8_t *rev_delays,
+                    const uint32_t *src_words, int n, void *stream);
 
 /* Izhikevich conductance / gap activators
  * (izhikevich_attributes.cpp:219-366). */
@@ -176,6 +188,15 @@ int sg_hip_set_f32(float *p, float val, long n, int overwrite, void *stream);
  * [cols][rows]. */
 int sg_hip_transpose32(const void *src, void *dst, int rows, int cols,
                        void *stream);
+
+/* Layout change between the reference's host layout [slots][K] and the
+ * 64-destination tiled device layout, optionally interleaved with other
+ * per-synapse arrays: element (s, k) of field f of nfields lives at
+ * (((s / 64) * K + k) * nfields + f) * 64 + s % 64 (to_tiled != 0: host ->
+ * tiled; else tiled -> host).  The tiled buffer holds
+ * ceil(slots / 64) * 64 * K * nfields elements. */
+int sg_hip_tile32(const void *src, void *dst, int slots, int K, int nfields,
+                  int field, int to_tiled, void *stream);
 
 /* Device properties: number of visible devices, -1 on error. */
 int sg_hip_device_count(void);

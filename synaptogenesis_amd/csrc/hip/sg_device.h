@@ -57,21 +57,29 @@ __device__ __forceinline__ int slot_to_index(const sg_conn &c, int slot) {
 //   CONVERGENT  synapse_kernel.h:384-421 (k_index also counts skipped
 //               out-of-bounds kernel cells)
 //   DIVERGENT   synapse_kernel.h:506-550 (C truncating division)
-// weight_index is the device layout index: k * num_dest + slot, or k for a
+// weight_index is the device layout index (see tiled_index), or k for a
 // shared convolutional kernel.
+// Per-destination synapse arrays are tiled by 64 destinations: the k-th
+// synapse of destination slot s lives at ((s / 64) * K + k) * 64 + s % 64,
+// so each wavefront-wide tile of destinations streams one contiguous block
+// of K * 64 elements per array.
+__host__ __device__ __forceinline__ long tiled_index(int slot, int k, int K) {
+    return ((long)(slot >> 6) * K + k) * 64 + (slot & 63);
+}
+
 template <int TYPE, class F>
 __device__ __forceinline__ void for_each_synapse(const sg_conn &c, int slot,
                                                  F &&f) {
-    const int nd = c.num_dest;
+    const int K = c.k_per_dest;
     if (TYPE == SG_FULLY_CONNECTED) {
-        for (int k = 0; k < c.from_size; ++k) f(k, k, k * nd + slot);
+        for (int k = 0; k < c.from_size; ++k) f(k, k, tiled_index(slot, k, K));
     } else if (TYPE == SG_ONE_TO_ONE) {
         f(0, slot, slot);
     } else if (TYPE == SG_SUBSET) {
         int k = 0;
         for (int fr = c.from_row_start; fr < c.from_row_end; ++fr)
             for (int fc = c.from_col_start; fc < c.from_col_end; ++fc, ++k)
-                f(k, fr * c.from_cols + fc, k * nd + slot);
+                f(k, fr * c.from_cols + fc, tiled_index(slot, k, K));
     } else if (TYPE == SG_CONVERGENT) {
         const int to_row = slot / c.to_cols, to_col = slot % c.to_cols;
         const int s_row = to_row * c.row_stride + c.row_spacing * c.row_offset;
@@ -94,7 +102,7 @@ __device__ __forceinline__ void for_each_synapse(const sg_conn &c, int slot,
                     if (!row_in || col < 0 || col >= c.from_cols) continue;
                 }
                 f(k, fr * c.from_cols + col,
-                  c.convolutional ? k : k * nd + slot);
+                  c.convolutional ? k : tiled_index(slot, k, K));
             }
         }
     } else if (TYPE == SG_DIVERGENT) {
@@ -118,7 +126,7 @@ __device__ __forceinline__ void for_each_synapse(const sg_conn &c, int slot,
                 if (c.wrap) fc = wrap_index(fc, c.from_cols);
                 else if (!row_in || fc < 0 || fc >= c.from_cols) continue;
                 f(k, fr * c.from_cols + fc,
-                  c.convolutional ? k : k * nd + slot);
+                  c.convolutional ? k : tiled_index(slot, k, K));
             }
         }
     }
@@ -126,4 +134,143 @@ __device__ __forceinline__ void for_each_synapse(const sg_conn &c, int slot,
 
 }  // namespace sg
 
+#endif
+
+#ifndef SG_DEVICE_RANDOM_ACCESS
+#define SG_DEVICE_RANDOM_ACCESS
+namespace sg {
+
+// Random access into one destination's synapse list: the k-th visit of the
+// reference's serial loop, so loads for a batch of k can be issued before
+// the ordered accumulation consumes them.  `count` is the loop trip count
+// (k runs over it even where a visit is skipped, as k_index does).
+struct SynapseCursor {
+    int count;        // visits in the serial loop, skipped ones included
+    int row0, col0;   // first source row/column (before wrapping)
+    int ncols;        // inner-loop trip count
+    int rstep, cstep; // source row/column step per inner/outer iteration
+};
+
+template <int TYPE>
+__device__ __forceinline__ SynapseCursor cursor(const sg_conn &c, int slot) {
+    SynapseCursor q;
+    if (TYPE == SG_FULLY_CONNECTED) {
+        q.count = c.from_size;
+        q.row0 = 0; q.col0 = 0; q.ncols = c.from_cols; q.rstep = 1; q.cstep = 1;
+    } else if (TYPE == SG_ONE_TO_ONE) {
+        q.count = 1;
+        q.row0 = slot / c.from_cols; q.col0 = slot % c.from_cols;
+        q.ncols = 1; q.rstep = 1; q.cstep = 1;
+    } else if (TYPE == SG_SUBSET) {
+        q.ncols = c.from_col_end - c.from_col_start;
+        q.count = (c.from_row_end - c.from_row_start) * q.ncols;
+        q.row0 = c.from_row_start; q.col0 = c.from_col_start;
+        q.rstep = 1; q.cstep = 1;
+    } else if (TYPE == SG_CONVERGENT) {
+        const int to_row = slot / c.to_cols, to_col = slot % c.to_cols;
+        q.row0 = to_row * c.row_stride + c.row_spacing * c.row_offset;
+        q.col0 = to_col * c.col_stride + c.col_spacing * c.col_offset;
+        q.ncols = c.col_field;
+        q.count = c.row_field * c.col_field;
+        q.rstep = c.row_spacing; q.cstep = c.col_spacing;
+    } else {  // SG_DIVERGENT
+        const int to_row = slot / c.to_cols, to_col = slot % c.to_cols;
+        q.row0 = (to_row + c.row_spacing * (-c.row_offset - c.row_field +
+                                            c.row_stride)) / c.row_stride;
+        q.col0 = (to_col + c.col_spacing * (-c.col_offset - c.col_field +
+                                            c.col_stride)) / c.col_stride;
+        const int rspan = c.row_spacing * (c.row_field - c.row_stride) /
+                          c.row_stride;
+        const int cspan = c.col_spacing * (c.col_field - c.col_stride) /
+                          c.col_stride;
+        const int nrows = rspan < 0 ? 0 : rspan / c.row_spacing + 1;
+        q.ncols = cspan < 0 ? 0 : cspan / c.col_spacing + 1;
+        q.count = nrows * q.ncols;
+        q.rstep = c.row_spacing; q.cstep = c.col_spacing;
+    }
+    return q;
+}
+
+// Source index of visit k, or -1 where the serial loop skips it.
+template <int TYPE>
+__device__ __forceinline__ int source_at(const sg_conn &c,
+                                         const SynapseCursor &q, int k) {
+    if (TYPE == SG_FULLY_CONNECTED) return k;
+    if (TYPE == SG_ONE_TO_ONE) return q.row0 * c.from_cols + q.col0;
+    const int kr = k / q.ncols, kc = k - kr * q.ncols;
+    int r = q.row0 + kr * q.rstep, col = q.col0 + kc * q.cstep;
+    if (TYPE == SG_SUBSET) return r * c.from_cols + col;
+    if (c.wrap) {
+        r = wrap_index(r, c.from_rows);
+        col = wrap_index(col, c.from_cols);
+    } else if (r < 0 || r >= c.from_rows || col < 0 || col >= c.from_cols) {
+        return -1;
+    }
+    return r * c.from_cols + col;
+}
+
+// Division-free walk over one destination's visits in serial order: the
+// source row/column advance by their steps and wrap by subtraction, the
+// device-layout weight index advances by one k-stride per visit.
+template <int TYPE>
+struct SynapseIter {
+    int k, count, kc, ncols;
+    int r, c, c0, rstep, cstep, rows, cols;
+    bool wrap;
+    long wi, wstep;
+
+    __device__ __forceinline__ void init(const sg_conn &cn, int slot) {
+        const SynapseCursor q = cursor<TYPE>(cn, slot);
+        k = 0; kc = 0;
+        count = q.count; ncols = q.ncols;
+        rows = cn.from_rows; cols = cn.from_cols;
+        rstep = q.rstep; cstep = q.cstep;
+        wrap = (TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) && cn.wrap;
+        r = q.row0; c = q.col0;
+        if (wrap) { r = wrap_index(r, rows); c = wrap_index(c, cols); }
+        c0 = c;
+        if (TYPE == SG_ONE_TO_ONE) { wi = slot; wstep = 0; }
+        else if ((TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) &&
+                 cn.convolutional) { wi = 0; wstep = 1; }
+        else { wi = tiled_index(slot, 0, cn.k_per_dest); wstep = 64; }
+    }
+    // Jump to visit k (one division; then advance() walks on).
+    __device__ __forceinline__ void seek(const sg_conn &cn, int slot,
+                                         int kk) {
+        init(cn, slot);
+        if (kk <= 0 || TYPE == SG_ONE_TO_ONE) return;
+        const SynapseCursor q = cursor<TYPE>(cn, slot);
+        k = kk;
+        kc = kk % ncols;
+        const int kr = kk / ncols;
+        r = q.row0 + kr * rstep;
+        c = q.col0 + kc * cstep;
+        if (wrap) { r = wrap_index(r, rows); c = wrap_index(c, cols); }
+        wi += (long)kk * wstep;
+    }
+    // Source index of the current visit, or -1 where the loop skips it.
+    __device__ __forceinline__ int source() const {
+        if ((TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) && !wrap &&
+            (r < 0 || r >= rows || c < 0 || c >= cols))
+            return -1;
+        return r * cols + c;
+    }
+    __device__ __forceinline__ void advance() {
+        ++k;
+        wi += wstep;
+        if (++kc == ncols) {
+            kc = 0;
+            r += rstep;
+            if (wrap) while (r >= rows) r -= rows;
+            c = c0;
+        } else {
+            c += cstep;
+            if (wrap) while (c >= cols) c -= cols;
+        }
+    }
+};
+
+
+
+}  // namespace sg
 #endif

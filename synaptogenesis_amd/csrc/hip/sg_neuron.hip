@@ -211,6 +211,28 @@ __global__ __launch_bounds__(kTile * kRowsPerPass) void transpose32_kernel(
     }
 }
 
+__global__ __launch_bounds__(kNeuronBlock) void tile32_kernel(
+        const uint32_t *src, uint32_t *dst, int slots, int K, int nfields,
+        int field, int to_tiled) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tiles = (slots + 63) / 64;
+    if (i >= tiles * 64 * K) return;
+    // i enumerates one field of the tiled buffer: tile, k, lane
+    const int lane = (int)(i & 63);
+    const long tk = i >> 6;
+    const int k = (int)(tk % K);
+    const long tile = tk / K;
+    const long slot = tile * 64 + lane;
+    const long t = ((tile * K + k) * nfields + field) * 64 + lane;
+    if (slot >= slots) {
+        if (to_tiled) dst[t] = 0u;
+        return;
+    }
+    const long h = slot * K + k;
+    if (to_tiled) dst[t] = src[h];
+    else dst[h] = src[t];
+}
+
 }  // namespace sg
 
 using namespace sg;
@@ -276,6 +298,18 @@ extern "C" int sg_hip_transpose32(const void *src, void *dst, int rows,
     hipLaunchKernelGGL(transpose32_kernel, g, dim3(kTile, kRowsPerPass), 0,
                        (hipStream_t)stream, (const uint32_t *)src,
                        (uint32_t *)dst, rows, cols);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_tile32(const void *src, void *dst, int slots, int K,
+                             int nfields, int field, int to_tiled,
+                             void *stream) {
+    if (slots <= 0 || K <= 0 || nfields <= 0 || field < 0 || field >= nfields)
+        return slots <= 0 || K <= 0 ? 0 : (int)hipErrorInvalidValue;
+    const long n = (long)((slots + 63) / 64) * 64 * K;
+    hipLaunchKernelGGL(tile32_kernel, dim3(nblocks(n)), dim3(kNeuronBlock), 0,
+                       (hipStream_t)stream, (const uint32_t *)src,
+                       (uint32_t *)dst, slots, K, nfields, field, to_tiled);
     return hipGetLastError();
 }
 

@@ -20,21 +20,57 @@ static inline int blocks_for(long n) {
     return (int)((n + kBlock - 1) / kBlock);
 }
 
+// Synapses are visited in batches of kBatch: the batch's loads are issued
+// together (one coalesced wavefront load per array and k), then the
+// synapses are consumed strictly in k order, so the FP32 sums keep the
+// reference's sequence while each wavefront keeps dozens of loads in flight.
+constexpr int kBatch = 8;
+
+
 // ---------------------------------------------------------------------------
 // Base activator (synapse_kernel.h:887-912): sum = 0; sum += extract * w in
 // serial order; input = aggregate(input, sum).
 template <int TYPE, int OUT>
 __global__ __launch_bounds__(kBlock) void activate_base_kernel(sg_conn c) {
+    constexpr int B = 16;
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= c.num_dest) return;
-    const uint32_t *src = c.src;
-    const float *w = c.weights;
+    SynapseIter<TYPE> it;
+    it.init(c, slot);
+    const int count = it.count;
     const int delay = c.delay;
     float sum = 0.0f;
-    for_each_synapse<TYPE>(c, slot, [&](int, int from, int wi) {
-        float val = extract<OUT>(src[from], delay) * w[wi];
-        sum += val;
-    });
+    float wa[B], wb[B];
+    uint32_t xa[B], xb[B];
+    int fa[B], fb[B];
+    auto load = [&](int k0, float *w, uint32_t *x, int *from) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            from[b] = it.k < count ? it.source() : -1;
+            if (from[b] >= 0) {
+                w[b] = c.weights[it.wi];
+                x[b] = c.src[from[b]];
+            }
+            if (it.k < count) it.advance();
+        }
+    };
+    auto consume = [&](const float *w, const uint32_t *x, const int *from) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (from[b] < 0) continue;
+            const float val = extract<OUT>(x[b], delay) * w[b];
+            sum += val;
+        }
+    };
+    // Two batches in flight: load the next while consuming the current.
+    load(0, wa, xa, fa);
+    for (int k0 = 0; k0 < count; k0 += 2 * B) {
+        if (k0 + B < count) load(k0 + B, wb, xb, fb);
+        consume(wa, xa, fa);
+        if (k0 + B >= count) break;
+        if (k0 + 2 * B < count) load(k0 + 2 * B, wa, xa, fa);
+        consume(wb, xb, fb);
+    }
     const int to = slot_to_index(c, slot);
     c.dst[to] = aggregate(c.opcode, c.dst[to], sum);
 }
@@ -59,103 +95,98 @@ static hipError_t launch_base_t(const sg_conn &c, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// time-since-spike of each source (izhikevich_attributes.cpp:258-262).  The
-// reference updates it per visit: spike ? 0 : min(32, tss + 1).  Over one
-// activation a source visited m times ends at min(32, m - j) if its last
-// spiking visit is the j-th (1-based), else min(32, tss + m).  The pre-pass
-// writes the second value; each spiking visit then lowers it to its own
-// min(32, m - j) with atomicMin, which leaves exactly the last one's value.
-__global__ __launch_bounds__(kBlock) void tss_prepass_kernel(
-        int32_t *next, const int32_t *cur, const uint8_t *visits, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) next[i] = min(32, cur[i] + (int)visits[i]);
+// Time since spike of each source (izhikevich_attributes.cpp:258-262).  The
+// reference updates it at every visit, in serial order: spike ? 0 :
+// min(32, tss + 1).  A source visited m times therefore ends at
+// min(32, m - j) when its j-th visit (1-based) is the last one that sees a
+// spike, and at min(32, tss + m) when none does; visits more than 32 from
+// the end cannot change the result.  One lane per source scans its last
+// visits backwards (their delays were laid out for it on the host) for the
+// first one whose delay bit is set in the source's history word.
+__global__ __launch_bounds__(kBlock) void tss_step_kernel(
+        int32_t *next, const int32_t *cur, const uint8_t *visits,
+        const uint8_t *rev_delays, const uint32_t *words, int n) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    const int m = visits[f];
+    const uint32_t word = words[f];
+    int t = min(32, cur[f] + m);
+    if (word != 0u) {
+        const uint32_t *rd = (const uint32_t *)(rev_delays + (long)f * 32);
+        for (int q = 0; q < 8 && q * 4 < m; ++q) {
+            const uint32_t four = rd[q];
+            int hit = -1;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = q * 4 + b;
+                const int d = (four >> (8 * b)) & 31;
+                if (hit < 0 && i < m && ((word << d) >> 31)) hit = i;
+            }
+            if (hit >= 0) { t = hit; break; }
+        }
+    }
+    next[f] = t;
 }
 
 // ---------------------------------------------------------------------------
-// Izhikevich conductance activator step for one synapse
-// (izhikevich_attributes.cpp:257-277):
-//   weight = w * (r * stp_w)
-//   trace' = trace + spike * weight * g ; sum += trace' ; trace = trace' * tau
+// Izhikevich per-synapse state of one batch element.
+struct IzSyn {
+    float w, r, sw, pre, dw, st, lt;
+    uint8_t dr;
+    uint32_t word;
+    int tss;
+};
+
 struct IzAct {
     float short_sum = 0.0f, long_sum = 0.0f;
 };
 
-template <int KIND>
-__device__ __forceinline__ void iz_activate_synapse(
-        const sg_iz_syn &s, const sg_iz_dest &d, IzAct &acc, float w,
-        float r, float sw, uint32_t word, int from, int wi) {
-    const uint16_t dr = s.delay_rank[wi];
-    const float spike = extract<SG_OUT_BIT>(word, dr & 0xFF);
-    if (spike > 0.0f) atomicMin(&s.tss_next[from], (int)(dr >> 8));
-    const float weight = w * (r * sw);
-    const float inc = spike * weight * s.baseline_conductance;
-    const float st = s.short_traces[wi] + inc;
-    acc.short_sum += st;
-    s.short_traces[wi] = st * d.short_tau;
-    if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB) {
-        const float lt = s.long_traces[wi] + inc;
-        acc.long_sum += lt;
-        s.long_traces[wi] = lt * d.long_tau;
-    }
-}
-
-template <int TYPE, int KIND>
-__global__ __launch_bounds__(kBlock) void iz_activate_kernel(sg_conn c,
-                                                              sg_iz_syn s,
-                                                              sg_iz_dest d) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= c.num_dest) return;
-    const int to = slot_to_index(c, slot);
-    if (KIND == SG_IZ_GAP) {
-        // izhikevich_attributes.cpp:354-366: sum += (v[from] - v[to]) * w
-        // over the destination layer's voltage array.
-        const float v = d.voltage[to];
-        float sum = 0.0f;
-        for_each_synapse<TYPE>(c, slot, [&](int, int from, int wi) {
-            sum += (d.voltage[from] - v) * s.weights[wi];
-        });
-        d.input[to] += sum;
-        return;
-    }
-    IzAct acc;
-    for_each_synapse<TYPE>(c, slot, [&](int, int from, int wi) {
-        iz_activate_synapse<KIND>(s, d, acc, s.weights[wi], s.stp_r[wi],
-                                  s.stp_w[wi], c.src[from], from, wi);
-    });
-    d.short_cond[to] += acc.short_sum;
-    if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB) d.long_cond[to] += acc.long_sum;
-}
-
-// ---------------------------------------------------------------------------
-// Izhikevich updater for one synapse (izhikevich_attributes.cpp:444-508).
-// Returns the (possibly) updated weight, r and stp_w in place.
 struct IzDest {
     float dest_spike;       // MSB of the destination's history word 0
     float dest_exc_trace;
     int dest_tss;
 };
 
-// U, D, F per opcode (izhikevich_attributes.cpp:432-438)
+// Conductance activator for one synapse (izhikevich_attributes.cpp:257-277):
+//   weight = w * (r * stp_w)
+//   trace' = trace + spike * weight * g ; sum += trace' ; trace = trace' * tau
+template <int KIND>
+__device__ __forceinline__ void iz_activate_one(const sg_iz_syn &s,
+                                                const sg_iz_dest &d,
+                                                IzAct &acc, IzSyn &y,
+                                                int from) {
+    const float spike = extract<SG_OUT_BIT>(y.word, y.dr);
+    const float weight = y.w * (y.r * y.sw);
+    const float inc = spike * weight * s.baseline_conductance;
+    y.st = y.st + inc;
+    acc.short_sum += y.st;
+    y.st = y.st * d.short_tau;
+    if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB) {
+        y.lt = y.lt + inc;
+        acc.long_sum += y.lt;
+        y.lt = y.lt * d.long_tau;
+    }
+}
+
+// Updater for one synapse (izhikevich_attributes.cpp:444-508); only called
+// when w >= MIN_WEIGHT.
 template <int SUB>
-__device__ __forceinline__ void iz_update_synapse(
-        const sg_iz_syn &s, const IzDest &dd, const float *src_table,
-        const float *dst_table, float &w, float &r, float &sw, int from,
-        int wi, float max_weight) {
-    if (!(w >= 0.0001f)) return;
-    const int src_tss = s.tss_cur[from];
+__device__ __forceinline__ void iz_update_one(const sg_iz_syn &s,
+                                              const IzDest &dd,
+                                              const float *src_table,
+                                              const float *dst_table,
+                                              IzSyn &y) {
+    const int src_tss = y.tss;
     const bool src_spike = src_tss == 0;
     float src_trace, dest_trace;
     if (!SUB) {
         // STDP_ADD: presyn = spike ? 0.004 : presyn * 0.933 (double math)
-        const float p = src_spike
-            ? (float)0.004
-            : (float)((double)s.presyn_traces[wi] * 0.933);
-        s.presyn_traces[wi] = p;
-        src_trace = p;
+        y.pre = src_spike ? (float)0.004 : (float)((double)y.pre * 0.933);
+        src_trace = y.pre;
         dest_trace = dd.dest_exc_trace;
     } else {
-        // STDP_SUB: iSTDP factors depend only on the integer times since
-        // spike, tabulated on the host with the reference's own
+        // STDP_SUB: the iSTDP factors depend only on the integer times since
+        // spike; they are tabulated on the host with the reference's own
         // powf / expf expressions.
         src_trace = (dd.dest_spike > 0.0f && src_tss > 0 && src_tss < 32)
             ? src_table[src_tss] : 0.0f;
@@ -164,85 +195,130 @@ __device__ __forceinline__ void iz_update_synapse(
     }
     // weight_derivatives -= (wd - C_A) * C_TAU + dest_spike * src_trace
     //                        - src_spike * dest_trace        (double math)
-    const float wd = s.weight_derivatives[wi];
-    const double rhs = (((double)wd - 0.000001) * 0.0001 +
+    const double rhs = (((double)y.dw - 0.000001) * 0.0001 +
                         (double)(dd.dest_spike * src_trace)) -
                        (double)((float)src_spike * dest_trace);
-    const float wd_new = (float)((double)wd - rhs);
-    s.weight_derivatives[wi] = wd_new;
-    float nw = w + s.learning_rate * wd_new;
-    w = fmaxf(0.0001f, fminf(max_weight, nw));
+    y.dw = (float)((double)y.dw - rhs);
+    const float nw = y.w + s.learning_rate * y.dw;
+    y.w = fmaxf(0.0001f, fminf(s.max_weight, nw));
     if (s.stp_flag) {
+        // U, D, F of the opcode (izhikevich_attributes.cpp:432-438)
         const double U = SUB ? 0.2 : 0.5;
         const float D = SUB ? 700.0f : 800.0f;
         const double F = SUB ? 20.0 : 1000.0;
         const float fs = (float)src_spike;
-        const float r_new = r + (((1.0f - r) / D) - (r * sw * fs));
-        const float w_new = (float)((double)sw +
-                                    (((U - (double)sw) / F) +
+        const float r = y.r, sw = y.sw;
+        y.r = r + (((1.0f - r) / D) - (r * sw * fs));
+        y.sw = (float)((double)sw + (((U - (double)sw) / F) +
                                      (U * (double)(1.0f - sw) * (double)fs)));
-        r = r_new;
-        sw = w_new;
     }
 }
 
-template <int TYPE, int SUB>
-__global__ __launch_bounds__(kBlock) void iz_update_kernel(
+// One kernel for the three Izhikevich synapse passes:
+//   UPD = 1, ACT = 0 -- updater of the current step
+//   UPD = 0, ACT = 1 -- conductance activator
+//   UPD = 1, ACT = 1 -- updater of the previous step, then the activator of
+//                       this step, per synapse, with the updated weight and
+//                       short-term plasticity kept in registers between them
+template <int TYPE, int KIND, int SUB, int UPD, int ACT>
+__global__ __launch_bounds__(kBlock) void iz_synapse_kernel(
         sg_conn c, sg_iz_syn s, sg_iz_dest d, const float *src_table,
         const float *dst_table) {
+    constexpr int B = kBatch;
     const int slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= c.num_dest) return;
     const int to = slot_to_index(c, slot);
     IzDest dd;
-    dd.dest_spike = extract<SG_OUT_BIT>(d.out_word0[to], 0);
-    dd.dest_exc_trace = d.post_trace[to];
-    dd.dest_tss = d.time_since_spike[to];
-    const float mw = s.max_weight;
-    for_each_synapse<TYPE>(c, slot, [&](int, int from, int wi) {
-        float w = s.weights[wi];
-        if (!(w >= 0.0001f)) return;
-        float r = s.stp_r[wi], sw = s.stp_w[wi];
-        iz_update_synapse<SUB>(s, dd, src_table, dst_table, w, r, sw, from,
-                               wi, mw);
-        s.weights[wi] = w;
-        if (s.stp_flag) {
-            s.stp_r[wi] = r;
-            s.stp_w[wi] = sw;
-        }
-    });
-}
-
-// Fused: updater of the previous step, then this step's activator, per
-// synapse, with the weights held in registers between the two.
-template <int TYPE, int KIND, int SUB>
-__global__ __launch_bounds__(kBlock) void iz_update_activate_kernel(
-        sg_conn c, sg_iz_syn s, sg_iz_dest d, const float *src_table,
-        const float *dst_table) {
-    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= c.num_dest) return;
-    const int to = slot_to_index(c, slot);
-    IzDest dd;
-    dd.dest_spike = extract<SG_OUT_BIT>(d.out_word0[to], 0);
-    dd.dest_exc_trace = d.post_trace[to];
-    dd.dest_tss = d.time_since_spike[to];
-    const float mw = s.max_weight;
+    if (UPD) {
+        dd.dest_spike = extract<SG_OUT_BIT>(d.out_word0[to], 0);
+        dd.dest_exc_trace = d.post_trace[to];
+        dd.dest_tss = d.time_since_spike[to];
+    }
+    SynapseIter<TYPE> it;
+    it.init(c, slot);
+    const int count = it.count;
     IzAct acc;
-    for_each_synapse<TYPE>(c, slot, [&](int, int from, int wi) {
-        float w = s.weights[wi];
-        float r = s.stp_r[wi], sw = s.stp_w[wi];
-        if (w >= 0.0001f) {
-            iz_update_synapse<SUB>(s, dd, src_table, dst_table, w, r, sw,
-                                   from, wi, mw);
-            s.weights[wi] = w;
-            if (s.stp_flag) {
-                s.stp_r[wi] = r;
-                s.stp_w[wi] = sw;
+    IzSyn ya[B], yb[B];
+    int fa[B], fb[B];
+    long ia[B], ib[B];
+    auto load = [&](int k0, IzSyn *y, int *from, long *wi) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            from[b] = it.k < count ? it.source() : -1;
+            wi[b] = it.wi;
+            if (it.k < count) it.advance();
+            if (from[b] < 0) continue;
+            y[b].w = s.weights[wi[b]];
+            y[b].r = s.stp_r[wi[b]];
+            y[b].sw = s.stp_w[wi[b]];
+            if (UPD) {
+                y[b].dw = s.weight_derivatives[wi[b]];
+                if (!SUB) y[b].pre = s.presyn_traces[wi[b]];
+                y[b].tss = s.tss_cur[from[b]];
+            }
+            if (ACT) {
+                y[b].dr = s.delays[wi[b]];
+                y[b].word = c.src[from[b]];
+                y[b].st = s.short_traces[wi[b]];
+                if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB)
+                    y[b].lt = s.long_traces[wi[b]];
             }
         }
-        iz_activate_synapse<KIND>(s, d, acc, w, r, sw, c.src[from], from, wi);
+    };
+    auto consume = [&](IzSyn *y, const int *from, const long *wi) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) {
+            if (from[b] < 0) continue;
+            const long i = wi[b];
+            if (UPD && y[b].w >= 0.0001f) {
+                iz_update_one<SUB>(s, dd, src_table, dst_table, y[b]);
+                s.weights[i] = y[b].w;
+                s.weight_derivatives[i] = y[b].dw;
+                if (!SUB) s.presyn_traces[i] = y[b].pre;
+                if (s.stp_flag) {
+                    s.stp_r[i] = y[b].r;
+                    s.stp_w[i] = y[b].sw;
+                }
+            }
+            if (ACT) {
+                iz_activate_one<KIND>(s, d, acc, y[b], from[b]);
+                s.short_traces[i] = y[b].st;
+                if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB)
+                    s.long_traces[i] = y[b].lt;
+            }
+        }
+    };
+    // Two batches in flight: load the next while consuming the current.
+    load(0, ya, fa, ia);
+    for (int k0 = 0; k0 < count; k0 += 2 * B) {
+        if (k0 + B < count) load(k0 + B, yb, fb, ib);
+        consume(ya, fa, ia);
+        if (k0 + B >= count) break;
+        if (k0 + 2 * B < count) load(k0 + 2 * B, ya, fa, ia);
+        consume(yb, fb, ib);
+    }
+    if (ACT) {
+        d.short_cond[to] += acc.short_sum;
+        if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB)
+            d.long_cond[to] += acc.long_sum;
+    }
+}
+
+// Gap junctions (izhikevich_attributes.cpp:354-366): sum += (v[from] - v[to])
+// * w over the destination layer's voltage array.
+template <int TYPE>
+__global__ __launch_bounds__(kBlock) void iz_gap_kernel(sg_conn c,
+                                                         sg_iz_syn s,
+                                                         sg_iz_dest d) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= c.num_dest) return;
+    const int to = slot_to_index(c, slot);
+    const float v = d.voltage[to];
+    float sum = 0.0f;
+    for_each_synapse<TYPE>(c, slot, [&](int, int from, int wi) {
+        sum += (d.voltage[from] - v) * s.weights[wi];
     });
-    d.short_cond[to] += acc.short_sum;
-    if (KIND == SG_IZ_ADD || KIND == SG_IZ_SUB) d.long_cond[to] += acc.long_sum;
+    d.input[to] += sum;
 }
 
 // ---------------------------------------------------------------------------
@@ -275,6 +351,17 @@ __global__ __launch_bounds__(kBlock) void iz_update_activate_kernel(
         default: return hipErrorInvalidValue;                              \
     }
 
+hipError_t launch_tile_base(const sg_conn &c, hipStream_t st);
+hipError_t launch_iz_tile(const sg_conn &c, const sg_iz_syn &s,
+                          const sg_iz_dest &d, int kind, int sub, int upd,
+                          int act, const float *st_, const float *dt_,
+                          hipStream_t st);
+
+static inline bool tiled_type(int t) {
+    return t == SG_FULLY_CONNECTED || t == SG_SUBSET || t == SG_CONVERGENT ||
+           t == SG_DIVERGENT;
+}
+
 }  // namespace sg
 
 using namespace sg;
@@ -282,6 +369,7 @@ using namespace sg;
 extern "C" int sg_hip_activate_base(const sg_conn *c, void *stream) {
     if (c->num_dest <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
+    if (tiled_type(c->type)) return launch_tile_base(*c, s);
     switch (c->type) {
         case SG_FULLY_CONNECTED: return launch_base_t<SG_FULLY_CONNECTED>(*c, s);
         case SG_SUBSET: return launch_base_t<SG_SUBSET>(*c, s);
@@ -292,12 +380,14 @@ extern "C" int sg_hip_activate_base(const sg_conn *c, void *stream) {
     return hipErrorInvalidValue;
 }
 
-extern "C" int sg_hip_tss_prepass(int32_t *next, const int32_t *cur,
-                                  const uint8_t *visits, int n,
-                                  void *stream) {
+extern "C" int sg_hip_tss_step(int32_t *next, const int32_t *cur,
+                               const uint8_t *visits,
+                               const uint8_t *rev_delays,
+                               const uint32_t *words, int n, void *stream) {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(tss_prepass_kernel, dim3(blocks_for(n)), dim3(kBlock),
-                       0, (hipStream_t)stream, next, cur, visits, n);
+    hipLaunchKernelGGL(tss_step_kernel, dim3(blocks_for(n)), dim3(kBlock), 0,
+                       (hipStream_t)stream, next, cur, visits, rev_delays,
+                       words, n);
     return hipGetLastError();
 }
 
@@ -309,13 +399,14 @@ extern "C" int sg_hip_iz_activate(const sg_conn *c, const sg_iz_syn *s,
     dim3 g(blocks_for(c->num_dest)), b(kBlock);
     if (kind == SG_IZ_GAP) {
         SG_TYPE_SWITCH(c->type,
-            hipLaunchKernelGGL((iz_activate_kernel<T, SG_IZ_GAP>), g, b, 0,
-                               st, *c, *s, *d));
+            hipLaunchKernelGGL((iz_gap_kernel<T>), g, b, 0, st, *c, *s, *d));
         return hipGetLastError();
     }
+    if (tiled_type(c->type) && s->record)
+        return launch_iz_tile(*c, *s, *d, kind, 0, 0, 1, nullptr, nullptr, st);
     SG_TYPE_SWITCH(c->type, SG_KIND_SWITCH(kind,
-        hipLaunchKernelGGL((iz_activate_kernel<T, K>), g, b, 0, st, *c, *s,
-                           *d)));
+        hipLaunchKernelGGL((iz_synapse_kernel<T, K, 0, 0, 1>), g, b, 0, st,
+                           *c, *s, *d, nullptr, nullptr)));
     return hipGetLastError();
 }
 
@@ -326,14 +417,17 @@ extern "C" int sg_hip_iz_update(const sg_conn *c, const sg_iz_syn *s,
     if (c->num_dest <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     dim3 g(blocks_for(c->num_dest)), b(kBlock);
+    if (tiled_type(c->type) && s->record)
+        return launch_iz_tile(*c, *s, *d, sub ? SG_IZ_SUB : SG_IZ_ADD, sub, 1,
+                              0, src_table, dst_table, st);
     if (sub) {
         SG_TYPE_SWITCH(c->type,
-            hipLaunchKernelGGL((iz_update_kernel<T, 1>), g, b, 0, st, *c, *s,
-                               *d, src_table, dst_table));
+            hipLaunchKernelGGL((iz_synapse_kernel<T, SG_IZ_SUB, 1, 1, 0>), g,
+                               b, 0, st, *c, *s, *d, src_table, dst_table));
     } else {
         SG_TYPE_SWITCH(c->type,
-            hipLaunchKernelGGL((iz_update_kernel<T, 0>), g, b, 0, st, *c, *s,
-                               *d, src_table, dst_table));
+            hipLaunchKernelGGL((iz_synapse_kernel<T, SG_IZ_ADD, 0, 1, 0>), g,
+                               b, 0, st, *c, *s, *d, src_table, dst_table));
     }
     return hipGetLastError();
 }
@@ -346,13 +440,16 @@ extern "C" int sg_hip_iz_update_activate(const sg_conn *c, const sg_iz_syn *s,
     if (c->num_dest <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     dim3 g(blocks_for(c->num_dest)), b(kBlock);
+    if (tiled_type(c->type) && s->record)
+        return launch_iz_tile(*c, *s, *d, kind, sub, 1, 1, src_table,
+                              dst_table, st);
     if (sub) {
         SG_TYPE_SWITCH(c->type, SG_KIND_SWITCH(kind,
-            hipLaunchKernelGGL((iz_update_activate_kernel<T, K, 1>), g, b, 0,
+            hipLaunchKernelGGL((iz_synapse_kernel<T, K, 1, 1, 1>), g, b, 0,
                                st, *c, *s, *d, src_table, dst_table)));
     } else {
         SG_TYPE_SWITCH(c->type, SG_KIND_SWITCH(kind,
-            hipLaunchKernelGGL((iz_update_activate_kernel<T, K, 0>), g, b, 0,
+            hipLaunchKernelGGL((iz_synapse_kernel<T, K, 0, 1, 1>), g, b, 0,
                                st, *c, *s, *d, src_table, dst_table)));
     }
     return hipGetLastError();

@@ -1,0 +1,310 @@
+// Tile kernels for every per-destination connection type (FULLY_CONNECTED,
+// SUBSET, CONVERGENT, DIVERGENT): the base activator and the three
+// Izhikevich synapse passes (update, activate, fused update+activate).
+//
+// A 256-lane block owns a tile of 64 destinations.  Its four wavefronts
+// split the tile's synapse list k-wise in chunks of 32 (8 visits per
+// wavefront per chunk): every synapse's independent work -- loads, STDP /
+// short-term plasticity, trace decay, stores -- runs with four wavefronts in
+// flight per tile instead of one.  The only order-dependent part, the FP32
+// sum over a destination's synapses, is taken out of the per-synapse pass:
+// each synapse writes the value it adds to a double-buffered LDS chunk, and
+// after a barrier one wavefront per sum adds the chunk's 32 values in k
+// order.  The result is the reference's sequential sum, bit for bit.
+//
+// Per-synapse arrays are in the 64-destination tiled layout (tiled_index),
+// so each wavefront load is one contiguous 256-byte line and each tile
+// streams one contiguous K * 256-byte block per array.
+#include <hip/hip_runtime.h>
+
+#include "sg_device.h"
+
+namespace sg {
+
+constexpr int kTileWaves = 4;
+constexpr int kTileThreads = 64 * kTileWaves;
+constexpr int kChunk = 32;
+constexpr int kPerWave = kChunk / kTileWaves;
+
+template <int TYPE>
+__device__ __forceinline__ int visit_count(const sg_conn &c) {
+    return cursor<TYPE>(c, 0).count;
+}
+
+// Adds this chunk's staged values in k order (wavefront `which` of the
+// block, one lane per destination).  `walk` tracks which visits exist.
+template <int TYPE>
+__device__ __forceinline__ void ordered_sum(const float (*vals)[64],
+                                            SynapseIter<TYPE> &walk, int k0,
+                                            int count, int lane,
+                                            float &sum) {
+    const int n = min(kChunk, count - k0);
+    for (int j = 0; j < n; ++j) {
+        const bool valid = walk.source() >= 0;
+        walk.advance();
+        if (valid) sum += vals[j][lane];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Base activator (synapse_kernel.h:887-912).
+template <int TYPE, int OUT>
+__global__ __launch_bounds__(kTileThreads) void tile_base_kernel(sg_conn c) {
+    __shared__ float vals[2][kChunk][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = blockIdx.x * 64 + lane;
+    const bool live = slot < c.num_dest;
+    const int count = visit_count<TYPE>(c);
+    SynapseIter<TYPE> walk;
+    walk.init(c, live ? slot : 0);
+    float sum = 0.0f;
+    for (int k0 = 0, chunk = 0; k0 < count; k0 += kChunk, ++chunk) {
+        const int buf = chunk & 1;
+        const int kw = k0 + wave * kPerWave;
+        if (live && kw < count) {
+            SynapseIter<TYPE> it;
+            it.seek(c, slot, kw);
+            float w[kPerWave];
+            uint32_t x[kPerWave];
+            int from[kPerWave];
+#pragma unroll
+            for (int b = 0; b < kPerWave; ++b) {
+                from[b] = it.k < count ? it.source() : -1;
+                if (from[b] >= 0) {
+                    w[b] = c.weights[it.wi];
+                    x[b] = c.src[from[b]];
+                }
+                if (it.k < count) it.advance();
+            }
+#pragma unroll
+            for (int b = 0; b < kPerWave; ++b)
+                if (from[b] >= 0)
+                    vals[buf][wave * kPerWave + b][lane] =
+                        extract<OUT>(x[b], c.delay) * w[b];
+        }
+        __syncthreads();
+        if (live && wave == 0) ordered_sum(vals[buf], walk, k0, count, lane, sum);
+    }
+    if (live && wave == 0) {
+        const int to = slot_to_index(c, slot);
+        c.dst[to] = aggregate(c.opcode, c.dst[to], sum);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Izhikevich synapse passes.  UPD: STDP / iSTDP + short-term plasticity
+// (izhikevich_attributes.cpp:444-508) of the previous step (fused) or of
+// this step (UPD only).  ACT: conductance activator
+// (izhikevich_attributes.cpp:257-284).
+template <int SUB>
+__device__ __forceinline__ void tile_update(const sg_iz_syn &s, float dest_spike,
+                                            float dest_exc_trace, int dest_tss,
+                                            const float *src_table,
+                                            const float *dst_table, int src_tss,
+                                            float &w, float &r, float &sw,
+                                            float &pre, float &dw) {
+    const bool src_spike = src_tss == 0;
+    float src_trace, dest_trace;
+    if (!SUB) {
+        pre = src_spike ? (float)0.004 : (float)((double)pre * 0.933);
+        src_trace = pre;
+        dest_trace = dest_exc_trace;
+    } else {
+        src_trace = (dest_spike > 0.0f && src_tss > 0 && src_tss < 32)
+            ? src_table[src_tss] : 0.0f;
+        dest_trace = (src_spike && dest_tss > 0 && dest_tss < 32)
+            ? dst_table[dest_tss] : 0.0f;
+    }
+    const double rhs = (((double)dw - 0.000001) * 0.0001 +
+                        (double)(dest_spike * src_trace)) -
+                       (double)((float)src_spike * dest_trace);
+    dw = (float)((double)dw - rhs);
+    const float nw = w + s.learning_rate * dw;
+    w = fmaxf(0.0001f, fminf(s.max_weight, nw));
+    if (s.stp_flag) {
+        const double U = SUB ? 0.2 : 0.5;
+        const float D = SUB ? 700.0f : 800.0f;
+        const double F = SUB ? 20.0 : 1000.0;
+        const float fs = (float)src_spike;
+        const float r0 = r, sw0 = sw;
+        r = r0 + (((1.0f - r0) / D) - (r0 * sw0 * fs));
+        sw = (float)((double)sw0 + (((U - (double)sw0) / F) +
+                                    (U * (double)(1.0f - sw0) * (double)fs)));
+    }
+}
+
+template <int TYPE, int KIND, int SUB, int UPD, int ACT>
+__global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
+        sg_conn c, sg_iz_syn s, sg_iz_dest d, const float *src_table,
+        const float *dst_table) {
+    constexpr bool kLong = KIND == SG_IZ_ADD || KIND == SG_IZ_SUB;
+    constexpr int F = SG_IZ_FIELDS;
+    __shared__ float vals[2][2][kChunk][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int slot = blockIdx.x * 64 + lane;
+    const bool live = slot < c.num_dest;
+    const int to = live ? slot_to_index(c, slot) : 0;
+    const int K = c.k_per_dest;
+    float dest_spike = 0.0f, dest_trace = 0.0f;
+    int dest_tss = 0;
+    if (UPD && live) {
+        dest_spike = extract<SG_OUT_BIT>(d.out_word0[to], 0);
+        dest_trace = d.post_trace[to];
+        dest_tss = d.time_since_spike[to];
+    }
+    const int count = visit_count<TYPE>(c);
+    SynapseIter<TYPE> walk;
+    walk.init(c, live ? slot : 0);
+    float sum = 0.0f;
+    for (int k0 = 0, chunk = 0; k0 < count; k0 += kChunk, ++chunk) {
+        const int buf = chunk & 1;
+        const int kw = k0 + wave * kPerWave;
+        if (live && kw < count) {
+            SynapseIter<TYPE> it;
+            it.seek(c, slot, kw);
+            // Record of visit kw of this lane's destination; visit kw + b is
+            // b * F * 64 further, its delay byte b * 64 further.
+            float *rec = s.record + (((long)blockIdx.x * K + kw) * F) * 64 +
+                         lane;
+            const uint8_t *dly = s.delays + tiled_index(slot, kw, K);
+            float w[kPerWave], r[kPerWave], sw[kPerWave], pre[kPerWave],
+                dw[kPerWave], st[kPerWave], lt[kPerWave];
+            uint32_t x[kPerWave];
+            int from[kPerWave], tss[kPerWave], dl[kPerWave];
+#pragma unroll
+            for (int b = 0; b < kPerWave; ++b) {
+                from[b] = it.k < count ? it.source() : -1;
+                if (it.k < count) it.advance();
+                if (from[b] < 0) continue;
+                const float *q = rec + b * F * 64;
+                w[b] = q[SG_IZ_W * 64];
+                r[b] = q[SG_IZ_R * 64];
+                sw[b] = q[SG_IZ_SW * 64];
+                if (UPD) {
+                    dw[b] = q[SG_IZ_DW * 64];
+                    if (!SUB) pre[b] = q[SG_IZ_PRE * 64];
+                    tss[b] = s.tss_cur[from[b]];
+                }
+                if (ACT) {
+                    dl[b] = dly[b * 64];
+                    x[b] = c.src[from[b]];
+                    st[b] = q[SG_IZ_ST * 64];
+                    if (kLong) lt[b] = q[SG_IZ_LT * 64];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kPerWave; ++b) {
+                if (from[b] < 0) continue;
+                float *q = rec + b * F * 64;
+                if (UPD && w[b] >= 0.0001f) {
+                    tile_update<SUB>(s, dest_spike, dest_trace, dest_tss,
+                                     src_table, dst_table, tss[b], w[b], r[b],
+                                     sw[b], pre[b], dw[b]);
+                    q[SG_IZ_W * 64] = w[b];
+                    q[SG_IZ_DW * 64] = dw[b];
+                    if (!SUB) q[SG_IZ_PRE * 64] = pre[b];
+                    if (s.stp_flag) {
+                        q[SG_IZ_R * 64] = r[b];
+                        q[SG_IZ_SW * 64] = sw[b];
+                    }
+                }
+                if (ACT) {
+                    const float spike = extract<SG_OUT_BIT>(x[b], dl[b]);
+                    const float weight = w[b] * (r[b] * sw[b]);
+                    const float inc = spike * weight * s.baseline_conductance;
+                    const float st1 = st[b] + inc;
+                    vals[buf][0][wave * kPerWave + b][lane] = st1;
+                    q[SG_IZ_ST * 64] = st1 * d.short_tau;
+                    if (kLong) {
+                        const float lt1 = lt[b] + inc;
+                        vals[buf][1][wave * kPerWave + b][lane] = lt1;
+                        q[SG_IZ_LT * 64] = lt1 * d.long_tau;
+                    }
+                }
+            }
+        }
+        if (ACT) {
+            __syncthreads();
+            if (live && (wave == 0 || (kLong && wave == 1)))
+                ordered_sum(vals[buf][wave], walk, k0, count, lane, sum);
+        }
+    }
+    if (ACT && live) {
+        if (wave == 0) d.short_cond[to] += sum;
+        else if (kLong && wave == 1) d.long_cond[to] += sum;
+    }
+}
+
+}  // namespace sg
+
+// Launchers used by sg_synapse.hip for the tiled connection types.
+namespace sg {
+
+hipError_t launch_tile_base(const sg_conn &c, hipStream_t st) {
+    dim3 g((c.num_dest + 63) / 64), b(kTileThreads);
+#define SG_TB(T)                                                             \
+    switch (c.out_type) {                                                    \
+        case SG_OUT_BIT:                                                     \
+            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_BIT>), g, b, 0,   \
+                               st, c); break;                                \
+        case SG_OUT_INT:                                                     \
+            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_INT>), g, b, 0,   \
+                               st, c); break;                                \
+        default:                                                             \
+            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_FLOAT>), g, b, 0, \
+                               st, c);                                       \
+    }
+    switch (c.type) {
+        case SG_FULLY_CONNECTED: SG_TB(SG_FULLY_CONNECTED); break;
+        case SG_SUBSET: SG_TB(SG_SUBSET); break;
+        case SG_CONVERGENT: SG_TB(SG_CONVERGENT); break;
+        case SG_DIVERGENT: SG_TB(SG_DIVERGENT); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef SG_TB
+    return hipGetLastError();
+}
+
+template <int UPD, int ACT>
+static hipError_t launch_iz_tile_t(const sg_conn &c, const sg_iz_syn &s,
+                                   const sg_iz_dest &d, int kind, int sub,
+                                   const float *st_, const float *dt_,
+                                   hipStream_t st) {
+    dim3 g((c.num_dest + 63) / 64), b(kTileThreads);
+#define SG_IZ(T, K, S)                                                       \
+    hipLaunchKernelGGL((iz_tile_kernel<T, K, S, UPD, ACT>), g, b, 0, st, c,  \
+                       s, d, st_, dt_)
+#define SG_IZK(T, S)                                                         \
+    switch (kind) {                                                          \
+        case SG_IZ_ADD: SG_IZ(T, SG_IZ_ADD, S); break;                       \
+        case SG_IZ_SUB: SG_IZ(T, SG_IZ_SUB, S); break;                       \
+        case SG_IZ_MULT: SG_IZ(T, SG_IZ_MULT, S); break;                     \
+        case SG_IZ_REWARD: SG_IZ(T, SG_IZ_REWARD, S); break;                 \
+        default: return hipErrorInvalidValue;                                \
+    }
+#define SG_IZT(T)                                                            \
+    if (sub) { SG_IZK(T, 1) } else { SG_IZK(T, 0) }
+    switch (c.type) {
+        case SG_FULLY_CONNECTED: SG_IZT(SG_FULLY_CONNECTED); break;
+        case SG_SUBSET: SG_IZT(SG_SUBSET); break;
+        case SG_CONVERGENT: SG_IZT(SG_CONVERGENT); break;
+        case SG_DIVERGENT: SG_IZT(SG_DIVERGENT); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef SG_IZ
+#undef SG_IZK
+#undef SG_IZT
+    return hipGetLastError();
+}
+
+hipError_t launch_iz_tile(const sg_conn &c, const sg_iz_syn &s,
+                          const sg_iz_dest &d, int kind, int sub, int upd,
+                          int act, const float *st_, const float *dt_,
+                          hipStream_t st) {
+    if (upd && act) return launch_iz_tile_t<1, 1>(c, s, d, kind, sub, st_, dt_, st);
+    if (upd) return launch_iz_tile_t<1, 0>(c, s, d, sub ? SG_IZ_SUB : SG_IZ_ADD,
+                                           sub, st_, dt_, st);
+    return launch_iz_tile_t<0, 1>(c, s, d, kind, 0, st_, dt_, st);
+}
+
+}  // namespace sg

@@ -49,6 +49,45 @@ struct Module {
     int start_delay = 0, cutoff = 0, rate = 1, curr = 0;
 };
 
+// Optional per-kernel timing with HIP events recorded on the engine's
+// stream around each launch ("kernel timing": "true" in the run args).
+struct KernelTimer {
+    bool on = false;
+    hipStream_t stream = nullptr;
+    struct Rec { std::string name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    template <class F>
+    void time(const char *name, F &&launch) {
+        if (!on) { launch(); return; }
+        Rec r{name, nullptr, nullptr};
+        check_hip(hipEventCreate(&r.a), "hipEventCreate");
+        check_hip(hipEventCreate(&r.b), "hipEventCreate");
+        check_hip(hipEventRecord(r.a, stream), "hipEventRecord");
+        launch();
+        check_hip(hipEventRecord(r.b, stream), "hipEventRecord");
+        recs.push_back(r);
+    }
+    void report(Props &out) {
+        std::map<std::string, std::pair<double, long>> acc;
+        for (auto &r : recs) {
+            float ms = 0.0f;
+            check_hip(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+            acc[r.name].first += ms;
+            acc[r.name].second += 1;
+        }
+        for (auto &kv : acc) {
+            out.set(kv.first + " ms", std::to_string(kv.second.first));
+            out.set(kv.first + " launches", std::to_string(kv.second.second));
+        }
+    }
+    ~KernelTimer() {
+        for (auto &r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+};
+
 void check_layer_model_supports(LayerState *ls, const std::string &key) {
     if (key == "input" || key == "output") return;
     if (!ls->vars.count(key))
@@ -211,6 +250,9 @@ Props *Engine::run(const Props &args) {
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg_{stream};
+    KernelTimer kt;
+    kt.on = args.get_bool("kernel timing", false);
+    kt.stream = stream;
 
     // ---- per-connection launch helpers -----------------------------------
     auto conn_desc = [&](ConnState &cs) {
@@ -252,6 +294,7 @@ Props *Engine::run(const Props &args) {
     auto iz_syn = [&](ConnState &cs) {
         sg_iz_syn s;
         std::memset(&s, 0, sizeof(s));
+        s.record = cs.packed ? cs.d_record : nullptr;
         s.weights = cs.arr("weights").df();
         s.short_traces = cs.arr("short trace").df();
         s.long_traces = cs.arr("long trace").df();
@@ -259,10 +302,8 @@ Props *Engine::run(const Props &args) {
         s.stp_r = cs.arr("short term R").df();
         s.stp_w = cs.arr("short term w").df();
         s.weight_derivatives = cs.arr("weight derivatives").df();
-        s.delay_rank = cs.d_delay_rank;
+        s.delays = cs.d_delays;
         s.tss_cur = cs.d_tss[cs.tss_cur];
-        s.tss_next = cs.d_tss[1 - cs.tss_cur];
-        s.visits = cs.d_visits;
         s.baseline_conductance = cs.baseline_conductance;
         s.learning_rate = cs.learning_rate;
         s.max_weight = cs.conn->max_weight;
@@ -306,29 +347,41 @@ Props *Engine::run(const Props &args) {
         if (cs.fused_gol) return;
         sg_conn d = conn_desc(cs);
         if (!cs.iz || cs.direct) {
-            check_hip(sg_hip_activate_base(&d, stream), "activate");
+            kt.time("activate_base", [&] {
+                check_hip(sg_hip_activate_base(&d, stream), "activate");
+            });
             return;
         }
         sg_iz_dest dd = iz_dest(cs, cs.iz_kind);
         sg_iz_syn s = iz_syn(cs);
         if (cs.iz_kind == SG_IZ_GAP) {
-            check_hip(sg_hip_iz_activate(&d, &s, &dd, SG_IZ_GAP, stream),
-                      "gap activate");
+            kt.time("iz_gap", [&] {
+                check_hip(sg_hip_iz_activate(&d, &s, &dd, SG_IZ_GAP, stream),
+                          "gap activate");
+            });
             return;
         }
-        check_hip(sg_hip_tss_prepass(s.tss_next, s.tss_cur, cs.d_visits,
-                                     cs.conn->from->size, stream),
-                  "tss prepass");
+        kt.time("tss_step", [&] {
+            check_hip(sg_hip_tss_step(cs.d_tss[1 - cs.tss_cur],
+                                      cs.d_tss[cs.tss_cur], cs.d_visits,
+                                      cs.d_rev_delays, d.src,
+                                      cs.conn->from->size, stream),
+                      "tss step");
+        });
         if (cs.pending_update) {
-            check_hip(sg_hip_iz_update_activate(
-                          &d, &s, &dd, cs.iz_kind, cs.update_kind,
-                          state_->d_src_table(), state_->d_dst_table(),
-                          stream),
-                      "update+activate");
+            kt.time("iz_update_activate", [&] {
+                check_hip(sg_hip_iz_update_activate(
+                              &d, &s, &dd, cs.iz_kind, cs.update_kind,
+                              state_->d_src_table(), state_->d_dst_table(),
+                              stream),
+                          "update+activate");
+            });
             cs.pending_update = false;
         } else {
-            check_hip(sg_hip_iz_activate(&d, &s, &dd, cs.iz_kind, stream),
-                      "iz activate");
+            kt.time("iz_activate", [&] {
+                check_hip(sg_hip_iz_activate(&d, &s, &dd, cs.iz_kind, stream),
+                          "iz activate");
+            });
         }
         cs.tss_cur = 1 - cs.tss_cur;
     };
@@ -336,10 +389,12 @@ Props *Engine::run(const Props &args) {
         sg_conn d = conn_desc(cs);
         sg_iz_syn s = iz_syn(cs);
         sg_iz_dest dd = iz_dest(cs, cs.iz_kind >= 0 ? cs.iz_kind : SG_IZ_ADD);
-        check_hip(sg_hip_iz_update(&d, &s, &dd, cs.update_kind,
-                                   state_->d_src_table(),
-                                   state_->d_dst_table(), stream),
-                  "iz update");
+        kt.time("iz_update", [&] {
+            check_hip(sg_hip_iz_update(&d, &s, &dd, cs.update_kind,
+                                       state_->d_src_table(),
+                                       state_->d_dst_table(), stream),
+                      "iz update");
+        });
     };
 
     // ---- game-of-life fast path -------------------------------------------
@@ -502,7 +557,9 @@ Props *Engine::run(const Props &args) {
                     l.time_since_spike = ls->var("time since spike").di();
                     l.a = ls->var("a").df(); l.b = ls->var("b").df();
                     l.c = ls->var("c").df(); l.d = ls->var("d").df();
-                    check_hip(sg_hip_iz_neuron(&l, stream), "iz neuron");
+                    kt.time("iz_neuron", [&] {
+                        check_hip(sg_hip_iz_neuron(&l, stream), "iz neuron");
+                    });
                     break;
                 }
                 case Model::GAME_OF_LIFE: {
@@ -511,14 +568,17 @@ Props *Engine::run(const Props &args) {
                         uint32_t *spare = gol_spare[ls];
                         float k9[9];
                         std::memcpy(k9, gf->second->arr("weights").f(), 36);
-                        check_hip(sg_hip_gol_step(
-                                      ls->output.du(), spare,
-                                      last ? ls->input.df() : nullptr,
-                                      ls->layer->rows, ls->layer->cols,
-                                      gf->second->conn->arbor.wrap, k9,
-                                      ls->survival_min, ls->survival_max,
-                                      ls->birth_min, ls->birth_max, stream),
-                                  "gol step");
+                        kt.time("gol_step", [&] {
+                            check_hip(sg_hip_gol_step(
+                                          ls->output.du(), spare,
+                                          last ? ls->input.df() : nullptr,
+                                          ls->layer->rows, ls->layer->cols,
+                                          gf->second->conn->arbor.wrap, k9,
+                                          ls->survival_min, ls->survival_max,
+                                          ls->birth_min, ls->birth_max,
+                                          stream),
+                                      "gol step");
+                        });
                         gol_spare[ls] = ls->output.du();
                         ls->output.dev = spare;
                     } else {
@@ -617,6 +677,8 @@ Props *Engine::run(const Props &args) {
         }
     }
     check_hip(hipStreamSynchronize(stream), "final sync");
+    Props timing;
+    kt.report(timing);
     const double secs = std::chrono::duration<double>(
         std::chrono::steady_clock::now() - t0).count();
 
@@ -638,6 +700,7 @@ Props *Engine::run(const Props &args) {
     mem.set("device", std::to_string(device));
     mem.set("bytes", std::to_string(state_->network_bytes()));
     r->add_to_child_array("memory usage", mem);
+    if (kt.on) r->set_child("kernel timing", timing);
     return r;
 }
 

@@ -90,7 +90,9 @@ State::~State() {
     }
     for (auto &cs : conns_) {
         for (auto &kv : cs->arrays) free_dev(kv.second.dev);
-        free_dev(cs->d_delay_rank);
+        free_dev(cs->d_record);
+        free_dev(cs->d_delays);
+        free_dev(cs->d_rev_delays);
         free_dev(cs->d_visits);
         free_dev(cs->d_tss[0]);
         free_dev(cs->d_tss[1]);
@@ -634,13 +636,22 @@ static bool needs_device(const std::string &key) {
     return true;
 }
 
+static size_t tiled_elems(int slots, int K) {
+    return (size_t)((slots + 63) / 64) * 64 * (size_t)K;
+}
+
 void State::upload(Array &a) {
     if (a.host.empty()) return;
-    if (!a.dev) {
-        check_hip(hipMalloc(&a.dev, a.bytes()), "hipMalloc");
-        a.dev_elems = a.size();
+    const size_t want = a.tiled ? tiled_elems(a.rows, a.cols) : a.size();
+    if (a.dev && a.dev_elems != want) {
+        check_hip(hipFree(a.dev), "hipFree");
+        a.dev = nullptr;
     }
-    if (!a.transpose) {
+    if (!a.dev) {
+        check_hip(hipMalloc(&a.dev, want * 4), "hipMalloc");
+        a.dev_elems = want;
+    }
+    if (!a.tiled) {
         check_hip(hipMemcpy(a.dev, a.host.data(), a.bytes(),
                             hipMemcpyHostToDevice), "hipMemcpy H2D");
         return;
@@ -648,21 +659,21 @@ void State::upload(Array &a) {
     void *tmp = scratch(a.bytes());
     check_hip(hipMemcpy(tmp, a.host.data(), a.bytes(), hipMemcpyHostToDevice),
               "hipMemcpy H2D");
-    check_hip(sg_hip_transpose32(tmp, a.dev, a.rows, a.cols, nullptr),
-              "transpose");
-    check_hip(hipDeviceSynchronize(), "transpose sync");
+    check_hip(sg_hip_tile32(tmp, a.dev, a.rows, a.cols, 1, 0, 1, nullptr),
+              "tile");
+    check_hip(hipDeviceSynchronize(), "tile sync");
 }
 
 void State::download(Array &a) {
     if (a.host.empty() || !a.dev) return;
-    if (!a.transpose) {
+    if (!a.tiled) {
         check_hip(hipMemcpy(a.host.data(), a.dev, a.bytes(),
                             hipMemcpyDeviceToHost), "hipMemcpy D2H");
         return;
     }
     void *tmp = scratch(a.bytes());
-    check_hip(sg_hip_transpose32(a.dev, tmp, a.cols, a.rows, nullptr),
-              "transpose");
+    check_hip(sg_hip_tile32(a.dev, tmp, a.rows, a.cols, 1, 0, 0, nullptr),
+              "untile");
     check_hip(hipMemcpy(a.host.data(), tmp, a.bytes(), hipMemcpyDeviceToHost),
               "hipMemcpy D2H");
 }
@@ -681,14 +692,16 @@ void State::ensure_device() {
     device_ready_ = true;
 }
 
-// Per-destination-slot arrays ([slot][k] on the host) go k-major on the
-// device; shared kernels and one-to-one arrays are copied as they are.
+// Per-destination-slot arrays ([slot][k] on the host) are tiled by 64
+// destinations on the device; shared kernels and one-to-one arrays are
+// copied as they are.
+static bool conn_tiled(const ConnState &cs) {
+    return !cs.conn->convolutional && cs.conn->type != SG_ONE_TO_ONE;
+}
+
 static void set_conn_layout(ConnState &cs, Array &a) {
-    const Connection &c = *cs.conn;
-    const bool per_slot = !c.convolutional && c.type != SG_ONE_TO_ONE &&
-                          cs.k_per_dest > 1 && cs.num_dest > 1 &&
-                          (long)cs.num_dest * cs.k_per_dest == (long)a.size();
-    a.transpose = per_slot;
+    a.tiled = conn_tiled(cs) &&
+              (long)cs.num_dest * cs.k_per_dest == (long)a.size();
     a.rows = cs.num_dest;
     a.cols = cs.k_per_dest;
 }
@@ -712,36 +725,55 @@ void State::upload_conn_derived(ConnState &cs) {
     cs.tss_cur = 0;
     if (!conductance) return;
 
-    // Visit ranks in the reference's serial order (destination-major), for
-    // the closed form of its per-visit time-since-spike update.
+    // Delays in the synapse layout, and per source the delays of its last
+    // (up to) 32 visits in the reference's serial (destination-major) order,
+    // newest first, for the time-since-spike pass.
     std::vector<int32_t> visits(from_n, 0);
     enumerate_synapses(c, cs.num_dest, [&](int, int, int, int r, int col,
                                            int, int, long) {
         ++visits[r * c.from->cols + col];
     });
-    const size_t nw = c.num_weights;
-    std::vector<uint16_t> packed(nw, 0);
+    const bool tiled = conn_tiled(cs);
+    const size_t nw = tiled ? tiled_elems(cs.num_dest, cs.k_per_dest)
+                            : (size_t)c.num_weights;
+    std::vector<uint8_t> dly(nw, 0);
+    std::vector<uint8_t> rev((size_t)from_n * 32, 0);
     std::vector<int32_t> seen(from_n, 0);
     const int32_t *dl = cs.arr("delays").i();
-    const bool per_slot = cs.arr("weights").transpose;
+    const int K = cs.k_per_dest;
     enumerate_synapses(c, cs.num_dest, [&](int slot, int, int k, int r,
                                            int col, int, int, long wi) {
         const int f = r * c.from->cols + col;
-        const int j = ++seen[f];
-        const int later = std::min(32, visits[f] - j);
-        const long di = per_slot ? (long)k * cs.num_dest + slot : wi;
-        packed[di] = (uint16_t)((later << 8) | (dl[wi] & 31));
+        const int j = ++seen[f];           // 1-based position of this visit
+        const int back = visits[f] - j;    // visits of f after this one
+        const uint8_t d = (uint8_t)(dl[wi] & 31);
+        if (back < 32) rev[(size_t)f * 32 + back] = d;
+        const long di = tiled ? ((long)(slot >> 6) * K + k) * 64 + (slot & 63)
+                              : wi;
+        dly[di] = d;
     });
     std::vector<uint8_t> vcap(from_n);
     for (int f = 0; f < from_n; ++f) vcap[f] = (uint8_t)std::min(32, visits[f]);
-    if (!cs.d_delay_rank)
-        check_hip(hipMalloc((void **)&cs.d_delay_rank, nw * 2), "hipMalloc");
-    if (!cs.d_visits)
-        check_hip(hipMalloc((void **)&cs.d_visits, from_n), "hipMalloc");
-    check_hip(hipMemcpy(cs.d_delay_rank, packed.data(), nw * 2,
-                        hipMemcpyHostToDevice), "hipMemcpy");
-    check_hip(hipMemcpy(cs.d_visits, vcap.data(), from_n,
-                        hipMemcpyHostToDevice), "hipMemcpy");
+    auto put = [&](uint8_t *&dst, const std::vector<uint8_t> &v) {
+        if (dst) check_hip(hipFree(dst), "hipFree");
+        check_hip(hipMalloc((void **)&dst, v.size()), "hipMalloc");
+        check_hip(hipMemcpy(dst, v.data(), v.size(), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+    };
+    put(cs.d_delays, dly);
+    put(cs.d_rev_delays, rev);
+    put(cs.d_visits, vcap);
+}
+
+// The seven per-synapse float arrays of a tiled Izhikevich conductance
+// connection, in the field order of its interleaved device record.
+static const char *kRecordFields[SG_IZ_FIELDS] = {
+    "weights", "short term R", "short term w", "presyn trace",
+    "weight derivatives", "short trace", "long trace"};
+
+static bool record_packed(const ConnState &cs) {
+    return cs.iz && !cs.direct && cs.iz_kind >= 0 && cs.iz_kind != SG_IZ_GAP &&
+           conn_tiled(cs);
 }
 
 void State::to_device() {
@@ -753,9 +785,32 @@ void State::to_device() {
         for (auto &kv : ls->vars) upload(kv.second);
     }
     for (auto &cs : conns_) {
+        cs->packed = record_packed(*cs);
+        if (cs->packed) {
+            const size_t n = tiled_elems(cs->num_dest, cs->k_per_dest) *
+                             SG_IZ_FIELDS;
+            if (!cs->d_record)
+                check_hip(hipMalloc((void **)&cs->d_record, n * 4),
+                          "hipMalloc record");
+            for (int f = 0; f < SG_IZ_FIELDS; ++f) {
+                Array &a = cs->arr(kRecordFields[f]);
+                void *tmp = scratch(a.bytes());
+                check_hip(hipMemcpy(tmp, a.host.data(), a.bytes(),
+                                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+                check_hip(sg_hip_tile32(tmp, cs->d_record, cs->num_dest,
+                                        cs->k_per_dest, SG_IZ_FIELDS, f, 1,
+                                        nullptr), "tile record");
+                check_hip(hipDeviceSynchronize(), "tile sync");
+            }
+        }
         for (auto &kv : cs->arrays) {
             if (!needs_device(kv.first)) continue;
+            if (cs->packed && kv.first != "weights" &&
+                std::find(std::begin(kRecordFields), std::end(kRecordFields),
+                          kv.first) != std::end(kRecordFields))
+                continue;
             set_conn_layout(*cs, kv.second);
+            if (cs->packed && kv.first == "weights") continue;
             upload(kv.second);
         }
         upload_conn_derived(*cs);
@@ -773,8 +828,20 @@ void State::to_host() {
         for (auto &kv : ls->vars) download(kv.second);
     }
     for (auto &cs : conns_) {
-        for (auto &kv : cs->arrays)
-            if (needs_device(kv.first)) download(kv.second);
+        if (cs->packed) {
+            for (int f = 0; f < SG_IZ_FIELDS; ++f) {
+                Array &a = cs->arr(kRecordFields[f]);
+                void *tmp = scratch(a.bytes());
+                check_hip(sg_hip_tile32(cs->d_record, tmp, cs->num_dest,
+                                        cs->k_per_dest, SG_IZ_FIELDS, f, 0,
+                                        nullptr), "untile record");
+                check_hip(hipMemcpy(a.host.data(), tmp, a.bytes(),
+                                    hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            }
+        } else {
+            for (auto &kv : cs->arrays)
+                if (needs_device(kv.first)) download(kv.second);
+        }
         if (cs->iz) {
             Array &tss = cs->arr("time since spike");
             const size_t nn = std::min((size_t)cs->conn->from->size,
@@ -880,12 +947,6 @@ void State::load(const std::string &path) {
         if (!in.read((char *)a->host.data(), k.bytes))
             throw Error("Error reading data from file!");
     }
-    for (auto &cs : conns_)
-        for (auto &kv : cs->arrays)
-            if (kv.second.dev && kv.second.dev_elems != kv.second.size()) {
-                (void)hipFree(kv.second.dev);
-                kv.second.dev = nullptr;
-            }
 }
 
 }  // namespace sg

@@ -26,8 +26,9 @@ enum class Elem { F32, I32, U32 };
 enum class Model { IZHIKEVICH, GAME_OF_LIFE, RATE_ENCODING, RELAY };
 
 // One named array.  `host` is the reference's host layout.  On the device
-// the array is either a copy (no transpose) or the [cols][rows] transpose of
-// the host [rows][cols] matrix.
+// the array is either a copy or, for per-destination synapse arrays
+// ([rows = destination slots][cols = K] on the host), the 64-destination
+// tiled layout the synapse kernels stream.
 struct Array {
     std::string key;
     Elem elem = Elem::F32;
@@ -35,7 +36,7 @@ struct Array {
     void *dev = nullptr;
     size_t dev_elems = 0;   // elements in the device twin (may be < host)
     int rows = 0, cols = 0;
-    bool transpose = false;
+    bool tiled = false;
 
     float *f() { return reinterpret_cast<float *>(host.data()); }
     int32_t *i() { return reinterpret_cast<int32_t *>(host.data()); }
@@ -73,8 +74,11 @@ struct ConnState {
     bool stp_flag = true;
     int num_dest = 0, k_per_dest = 0;
     // device-only derived arrays
-    uint16_t *d_delay_rank = nullptr;
-    uint8_t *d_visits = nullptr;
+    float *d_record = nullptr;        // interleaved per-synapse record
+    bool packed = false;              // iz arrays live in d_record
+    uint8_t *d_delays = nullptr;      // delay & 31, synapse layout
+    uint8_t *d_rev_delays = nullptr;  // [source][32] last visits, reversed
+    uint8_t *d_visits = nullptr;      // min(32, visits per source)
     int32_t *d_tss[2] = {nullptr, nullptr};
     int tss_cur = 0;
     bool pending_update = false;

@@ -1,0 +1,102 @@
+"""Benchmark workloads, as reference-format network descriptions plus the
+seeded weights and delays injected through the C-ABI callbacks.
+
+* ``sheet``  -- Izhikevich excitatory/inhibitory sheet: E 256x320
+  (regular), I 128x160 (fast); E->E convergent 15x15 stride 1 wrap on the
+  conductance path with STDP + short-term plasticity (learning rate 0.01) and
+  per-synapse delays U{0..31}; E->I convergent 15x15 stride 2 wrap and
+  I->E divergent 9x9 stride 2 wrap (sub), both direct; weights U[0,0.5);
+  thalamic noise N(0, 3) into E and I every step.
+* ``gol``    -- game of life on a square board, one 3x3 convolutional
+  self-connection with wrap, weight 1 with the centre cleared.
+* ``izfc``   -- two Izhikevich layers of 1000, FULLY_CONNECTED direct,
+  weights U[0,0.5), input N(5, 2) into layer 1.
+
+Per-step noise is drawn on the device (layer "init config", normal) so a
+throughput run does not stop at a host callback every step; parity runs
+inject host-generated noise instead (tests/netdefs.py).
+"""
+
+import numpy as np
+
+from .inputs import Stream
+
+
+def _s(x):
+    if isinstance(x, bool):
+        return "true" if x else "false"
+    return str(x)
+
+
+def _layer(name, model, rows, cols, **kw):
+    d = {"name": name, "neural model": model, "rows": _s(rows),
+         "columns": _s(cols)}
+    d.update({k: (v if isinstance(v, dict) else _s(v)) for k, v in kw.items()})
+    return d
+
+
+def sheet(e_rows=256, e_cols=320, noise_std=3.0):
+    i_rows, i_cols = e_rows // 2, e_cols // 2
+    noise = {"type": "normal", "mean": "0", "std dev": _s(noise_std)}
+    layers = [_layer("E", "izhikevich", e_rows, e_cols, params="regular",
+                     **{"init config": noise}),
+              _layer("I", "izhikevich", i_rows, i_cols, params="fast",
+                     **{"init config": noise})]
+    conns = [
+        {"name": "EE", "from layer": "E", "to layer": "E",
+         "type": "convergent", "opcode": "add", "plastic": "true",
+         "learning rate": "0.01", "delay": "0",
+         "arborized config": {"field size": "15", "stride": "1",
+                              "wrap": "true"}},
+        {"name": "EI", "from layer": "E", "to layer": "I",
+         "type": "convergent", "opcode": "add", "plastic": "false",
+         "direct": "true", "myelinated": "true", "delay": "0",
+         "arborized config": {"field size": "15", "stride": "2",
+                              "wrap": "true"}},
+        {"name": "IE", "from layer": "I", "to layer": "E",
+         "type": "divergent", "opcode": "sub", "plastic": "false",
+         "direct": "true", "myelinated": "true", "delay": "0",
+         "arborized config": {"field size": "9", "stride": "2",
+                              "wrap": "true"}},
+    ]
+    ne, ni = e_rows * e_cols, i_rows * i_cols
+    nw = {"EE": ne * 225, "EI": ni * 225, "IE": ne * 81}
+    st = Stream(3)
+    weights = {k: (lambda n=n, i=i: st.uniform(n, stream=10 + i, low=0.0,
+                                               high=0.5))
+               for i, (k, n) in enumerate(nw.items())}
+    delays = {"EE": lambda: st.integers(nw["EE"], 0, 31, stream=20)}
+    # Synapse visits per step: E->E every kernel cell (wrap), E->I every
+    # cell, I->E 4x4 of the 9x9 kernel at stride 2.
+    visits = ne * 225 + ni * 225 + ne * 16
+    net = {"structures": [{"name": "sheet", "type": "parallel",
+                           "layers": layers}],
+           "connections": conns}
+    return {"net": net, "weights": weights, "delays": delays,
+            "visits_per_step": visits, "neurons": ne + ni,
+            "init": {("sheet", "E", "voltage"): np.full(ne, -65.0, np.float32),
+                     ("sheet", "E", "recovery"): np.full(
+                         ne, np.float32(0.2) * np.float32(-65.0), np.float32),
+                     ("sheet", "I", "voltage"): np.full(ni, -65.0, np.float32),
+                     ("sheet", "I", "recovery"): np.full(
+                         ni, np.float32(0.2) * np.float32(-65.0), np.float32)},
+            "name": "izhikevich E/I sheet E%dx%d I%dx%d" % (
+                e_rows, e_cols, i_rows, i_cols)}
+
+
+def gol(n=1024):
+    st = Stream(2)
+    layers = [_layer("board", "game_of_life", n, n)]
+    conns = [{"name": "life", "from layer": "board", "to layer": "board",
+              "type": "convergent", "opcode": "add", "plastic": "false",
+              "convolutional": "true", "delay": "0",
+              "arborized config": {"field size": "3", "wrap": "true"},
+              "weight config": {"type": "flat", "weight": "1",
+                                "diagonal": "false"}}]
+    words = st.bernoulli(n * n, 0.5, stream=1).astype(np.uint32) << 31
+    net = {"structures": [{"name": "gol", "type": "parallel",
+                           "layers": layers}], "connections": conns}
+    return {"net": net, "weights": {}, "delays": {},
+            "visits_per_step": 9 * n * n, "neurons": n * n,
+            "init": {("gol", "board", "output"): words},
+            "name": "game of life %dx%d" % (n, n)}
