@@ -126,6 +126,38 @@ def cpu_baseline(wl, steps):
                       "%.2f s/step" % (steps, wl["name"], dt / steps)}
 
 
+def copy_calibration(mb):
+    """Copies mb MiB device-to-device with sg_hip_copy32 three times."""
+    hip = ctypes.CDLL(os.path.join(ROOT, "synaptogenesis_amd",
+                                   "libsg_hip.so"))
+    amd = ctypes.CDLL("libamdhip64.so")
+    n = mb * 1024 * 1024 // 4
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert amd.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n * 4)) == 0
+    assert amd.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n * 4)) == 0
+    hip.sg_hip_copy32.argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_long, ctypes.c_void_p]
+    ev = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for e in ev:
+        assert amd.hipEventCreate(ctypes.byref(e)) == 0
+    times = []
+    for _ in range(5):
+        amd.hipEventRecord(ev[0], None)
+        assert hip.sg_hip_copy32(src, dst, n, None) == 0
+        amd.hipEventRecord(ev[1], None)
+        amd.hipEventSynchronize(ev[1])
+        ms = ctypes.c_float()
+        amd.hipEventElapsedTime(ctypes.byref(ms), ev[0], ev[1])
+        times.append(ms.value)
+    assert amd.hipDeviceSynchronize() == 0
+    amd.hipFree(src)
+    amd.hipFree(dst)
+    best = min(times[1:])
+    print(json.dumps({"copy_calibration_bytes_each_way": n * 4,
+                      "copy_ms": best,
+                      "copy_GBps_read_plus_write": 2 * n * 4 / best / 1e6}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,12 +165,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="also run a plain device copy of MB MiB (reference "
+                         "for hardware byte counters) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: this build runs a network on a "
               "single GPU (multi-GPU sharding is a design note only); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
 
     wl = workloads.sheet()
     lib = syngen.load_library()

@@ -55,6 +55,10 @@ typedef struct sg_conn {
     int k_per_dest;
     int num_dest;
     int num_weights;
+    /* Base activator with a BIT source and only finite weights: a visit
+     * whose bit is 0 adds +0 or -0, which leaves the sum unchanged (it
+     * starts at +0 and can never become -0), so its weight is not loaded. */
+    int skip_silent;
     /* source output at word delay/32 (BIT) or register delay (FLOAT/INT) */
     const uint32_t *src;
     /* destination input register */
@@ -197,6 +201,10 @@ int sg_hip_transpose32(const void *src, void *dst, int rows, int cols,
  * ceil(slots / 64) * 64 * K * nfields elements. */
 int sg_hip_tile32(const void *src, void *dst, int slots, int K, int nfields,
                   int field, int to_tiled, void *stream);
+
+/* Plain copy of n 32-bit words (a known-size reference for the hardware
+ * byte counters). */
+int sg_hip_copy32(const void *src, void *dst, long n, void *stream);
 
 /* Device properties: number of visible devices, -1 on error. */
 int sg_hip_device_count(void);

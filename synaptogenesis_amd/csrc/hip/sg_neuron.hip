@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "sg_hip.h"
 
 namespace sg {
@@ -233,6 +235,13 @@ __global__ __launch_bounds__(kNeuronBlock) void tile32_kernel(
     else dst[h] = src[t];
 }
 
+__global__ __launch_bounds__(kNeuronBlock) void copy32_kernel(
+        const uint4 *src, uint4 *dst, long n4) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace sg
 
 using namespace sg;
@@ -310,6 +319,20 @@ extern "C" int sg_hip_tile32(const void *src, void *dst, int slots, int K,
     hipLaunchKernelGGL(tile32_kernel, dim3(nblocks(n)), dim3(kNeuronBlock), 0,
                        (hipStream_t)stream, (const uint32_t *)src,
                        (uint32_t *)dst, slots, K, nfields, field, to_tiled);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_copy32(const void *src, void *dst, long n,
+                             void *stream) {
+    if (n <= 0) return 0;
+    if (n % 4 != 0 || ((uintptr_t)src | (uintptr_t)dst) % 16 != 0)
+        return (int)hipErrorInvalidValue;
+    const long n4 = n / 4;
+    const int blocks = (int)std::min<long>((n4 + kNeuronBlock - 1) /
+                                           kNeuronBlock, 256L * 64);
+    hipLaunchKernelGGL(copy32_kernel, dim3(blocks), dim3(kNeuronBlock), 0,
+                       (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst,
+                       n4);
     return hipGetLastError();
 }
 

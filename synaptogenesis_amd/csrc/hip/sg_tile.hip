@@ -25,6 +25,7 @@ constexpr int kTileWaves = 4;
 constexpr int kTileThreads = 64 * kTileWaves;
 constexpr int kChunk = 32;
 constexpr int kPerWave = kChunk / kTileWaves;
+constexpr int kSub = 8;
 
 template <int TYPE>
 __device__ __forceinline__ int visit_count(const sg_conn &c) {
@@ -47,43 +48,74 @@ __device__ __forceinline__ void ordered_sum(const float (*vals)[64],
 }
 
 // ---------------------------------------------------------------------------
-// Base activator (synapse_kernel.h:887-912).
-template <int TYPE, int OUT>
-__global__ __launch_bounds__(kTileThreads) void tile_base_kernel(sg_conn c) {
-    __shared__ float vals[2][kChunk][64];
+// Base activator (synapse_kernel.h:887-912).  WAVES wavefronts share a tile
+// and CHUNK visits per barrier; small destination layers use more
+// wavefronts per tile to keep the GPU busy.
+template <int TYPE>
+__device__ __forceinline__ void ordered_sum_n(const float (*vals)[64],
+                                              SynapseIter<TYPE> &walk, int n,
+                                              int lane, float &sum) {
+    for (int j = 0; j < n; ++j) {
+        const bool valid = walk.source() >= 0;
+        walk.advance();
+        if (valid) sum += vals[j][lane];
+    }
+}
+
+template <int TYPE, int OUT, int WAVES, int CHUNK>
+__global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
+    constexpr int PER = CHUNK / WAVES;
+    __shared__ float vals[2][CHUNK][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = blockIdx.x * 64 + lane;
     const bool live = slot < c.num_dest;
     const int count = visit_count<TYPE>(c);
+    const int K = c.k_per_dest;
+    const bool skip = OUT == SG_OUT_BIT && c.skip_silent;
     SynapseIter<TYPE> walk;
     walk.init(c, live ? slot : 0);
     float sum = 0.0f;
-    for (int k0 = 0, chunk = 0; k0 < count; k0 += kChunk, ++chunk) {
+    for (int k0 = 0, chunk = 0; k0 < count; k0 += CHUNK, ++chunk) {
         const int buf = chunk & 1;
-        const int kw = k0 + wave * kPerWave;
+        const int kw = k0 + wave * PER;
         if (live && kw < count) {
             SynapseIter<TYPE> it;
             it.seek(c, slot, kw);
-            float w[kPerWave];
-            uint32_t x[kPerWave];
-            int from[kPerWave];
+            const float *wrow = c.convolutional ? c.weights + kw
+                                                : c.weights +
+                                                      tiled_index(slot, kw, K);
+            const int wstep = c.convolutional ? 1 : 64;
+            uint32_t x[PER];
+            int from[PER];
 #pragma unroll
-            for (int b = 0; b < kPerWave; ++b) {
+            for (int b = 0; b < PER; ++b) {
                 from[b] = it.k < count ? it.source() : -1;
-                if (from[b] >= 0) {
-                    w[b] = c.weights[it.wi];
-                    x[b] = c.src[from[b]];
-                }
                 if (it.k < count) it.advance();
+                if (from[b] >= 0) x[b] = c.src[from[b]];
             }
+            if (skip) {
 #pragma unroll
-            for (int b = 0; b < kPerWave; ++b)
-                if (from[b] >= 0)
-                    vals[buf][wave * kPerWave + b][lane] =
-                        extract<OUT>(x[b], c.delay) * w[b];
+                for (int b = 0; b < PER; ++b)
+                    if (from[b] >= 0) {
+                        const bool on = (x[b] << (c.delay & 31)) >> 31;
+                        vals[buf][wave * PER + b][lane] =
+                            on ? wrow[b * wstep] : 0.0f;
+                    }
+            } else {
+                float w[PER];
+#pragma unroll
+                for (int b = 0; b < PER; ++b)
+                    if (from[b] >= 0) w[b] = wrow[b * wstep];
+#pragma unroll
+                for (int b = 0; b < PER; ++b)
+                    if (from[b] >= 0)
+                        vals[buf][wave * PER + b][lane] =
+                            extract<OUT>(x[b], c.delay) * w[b];
+            }
         }
         __syncthreads();
-        if (live && wave == 0) ordered_sum(vals[buf], walk, k0, count, lane, sum);
+        if (live && wave == 0)
+            ordered_sum_n(vals[buf], walk, min(CHUNK, count - k0), lane, sum);
     }
     if (live && wave == 0) {
         const int to = slot_to_index(c, slot);
@@ -167,16 +199,17 @@ __global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
             float *rec = s.record + (((long)blockIdx.x * K + kw) * F) * 64 +
                          lane;
             const uint8_t *dly = s.delays + tiled_index(slot, kw, K);
-            float w[kPerWave], r[kPerWave], sw[kPerWave], pre[kPerWave],
-                dw[kPerWave], st[kPerWave], lt[kPerWave];
-            uint32_t x[kPerWave];
-            int from[kPerWave], tss[kPerWave], dl[kPerWave];
+            for (int h = 0; h < kPerWave; h += kSub) {
+            float w[kSub], r[kSub], sw[kSub], pre[kSub], dw[kSub], st[kSub],
+                lt[kSub];
+            uint32_t x[kSub];
+            int from[kSub], tss[kSub], dl[kSub];
 #pragma unroll
-            for (int b = 0; b < kPerWave; ++b) {
+            for (int b = 0; b < kSub; ++b) {
                 from[b] = it.k < count ? it.source() : -1;
                 if (it.k < count) it.advance();
                 if (from[b] < 0) continue;
-                const float *q = rec + b * F * 64;
+                const float *q = rec + (h + b) * F * 64;
                 w[b] = q[SG_IZ_W * 64];
                 r[b] = q[SG_IZ_R * 64];
                 sw[b] = q[SG_IZ_SW * 64];
@@ -186,16 +219,16 @@ __global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
                     tss[b] = s.tss_cur[from[b]];
                 }
                 if (ACT) {
-                    dl[b] = dly[b * 64];
+                    dl[b] = dly[(h + b) * 64];
                     x[b] = c.src[from[b]];
                     st[b] = q[SG_IZ_ST * 64];
                     if (kLong) lt[b] = q[SG_IZ_LT * 64];
                 }
             }
 #pragma unroll
-            for (int b = 0; b < kPerWave; ++b) {
+            for (int b = 0; b < kSub; ++b) {
                 if (from[b] < 0) continue;
-                float *q = rec + b * F * 64;
+                float *q = rec + (h + b) * F * 64;
                 if (UPD && w[b] >= 0.0001f) {
                     tile_update<SUB>(s, dest_spike, dest_trace, dest_tss,
                                      src_table, dst_table, tss[b], w[b], r[b],
@@ -213,14 +246,15 @@ __global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
                     const float weight = w[b] * (r[b] * sw[b]);
                     const float inc = spike * weight * s.baseline_conductance;
                     const float st1 = st[b] + inc;
-                    vals[buf][0][wave * kPerWave + b][lane] = st1;
+                    vals[buf][0][wave * kPerWave + h + b][lane] = st1;
                     q[SG_IZ_ST * 64] = st1 * d.short_tau;
                     if (kLong) {
                         const float lt1 = lt[b] + inc;
-                        vals[buf][1][wave * kPerWave + b][lane] = lt1;
+                        vals[buf][1][wave * kPerWave + h + b][lane] = lt1;
                         q[SG_IZ_LT * 64] = lt1 * d.long_tau;
                     }
                 }
+            }
             }
         }
         if (ACT) {
@@ -240,19 +274,20 @@ __global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
 // Launchers used by sg_synapse.hip for the tiled connection types.
 namespace sg {
 
-hipError_t launch_tile_base(const sg_conn &c, hipStream_t st) {
-    dim3 g((c.num_dest + 63) / 64), b(kTileThreads);
+template <int WAVES, int CHUNK>
+static hipError_t launch_tile_base_t(const sg_conn &c, hipStream_t st) {
+    dim3 g((c.num_dest + 63) / 64), b(64 * WAVES);
 #define SG_TB(T)                                                             \
     switch (c.out_type) {                                                    \
         case SG_OUT_BIT:                                                     \
-            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_BIT>), g, b, 0,   \
-                               st, c); break;                                \
+            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_BIT, WAVES,       \
+                                CHUNK>), g, b, 0, st, c); break;             \
         case SG_OUT_INT:                                                     \
-            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_INT>), g, b, 0,   \
-                               st, c); break;                                \
+            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_INT, WAVES,       \
+                                CHUNK>), g, b, 0, st, c); break;             \
         default:                                                             \
-            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_FLOAT>), g, b, 0, \
-                               st, c);                                       \
+            hipLaunchKernelGGL((tile_base_kernel<T, SG_OUT_FLOAT, WAVES,     \
+                                CHUNK>), g, b, 0, st, c);                    \
     }
     switch (c.type) {
         case SG_FULLY_CONNECTED: SG_TB(SG_FULLY_CONNECTED); break;
@@ -263,6 +298,13 @@ hipError_t launch_tile_base(const sg_conn &c, hipStream_t st) {
     }
 #undef SG_TB
     return hipGetLastError();
+}
+
+// Fewer than two tiles per CU: sixteen wavefronts per tile.
+hipError_t launch_tile_base(const sg_conn &c, hipStream_t st) {
+    const int tiles = (c.num_dest + 63) / 64;
+    if (tiles < 512) return launch_tile_base_t<16, 64>(c, st);
+    return launch_tile_base_t<4, 32>(c, st);
 }
 
 template <int UPD, int ACT>

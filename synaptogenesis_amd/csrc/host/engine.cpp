@@ -289,6 +289,9 @@ Props *Engine::run(const Props &args) {
                 (size_t)(c.delay / per_word) * c.from->size;
         d.dst = cs.to->input.df();
         d.weights = cs.arr("weights").df();
+        // Plastic base-activator weights stay finite: the updater clamps
+        // them to [1e-4, max weight].
+        d.skip_silent = d.out_type == SG_OUT_BIT && cs.finite_weights;
         return d;
     };
     auto iz_syn = [&](ConnState &cs) {

@@ -814,6 +814,11 @@ void State::to_device() {
             upload(kv.second);
         }
         upload_conn_derived(*cs);
+        const float *w = cs->arr("weights").f();
+        bool fin = true;
+        for (size_t i = 0, n = cs->arr("weights").size(); i < n && fin; ++i)
+            fin = std::isfinite(w[i]);
+        cs->finite_weights = fin;
     }
     check_hip(hipDeviceSynchronize(), "upload sync");
     on_host_ = false;

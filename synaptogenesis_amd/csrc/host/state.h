@@ -83,6 +83,7 @@ struct ConnState {
     int tss_cur = 0;
     bool pending_update = false;
     bool fused_gol = false;   // handled by sg_hip_gol_step
+    bool finite_weights = false;  // every weight finite at upload
 
     Array &arr(const std::string &k) { return arrays.at(k); }
     bool has(const std::string &k) const { return arrays.count(k) > 0; }
