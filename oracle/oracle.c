@@ -1,0 +1,554 @@
+/*
+ * CPU restatement of the reference's serial simulation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU path is
+ * compared with: only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it.  The product (synaptogenesis_amd/) never links,
+ * loads or calls it.
+ *
+ * Structure.  The reference expands one loop nest per connection type and
+ * kernel body (engine/kernel/synapse_kernel.h:111-606, serial variants).
+ * Here the loop nest is run once, by orc_enumerate(), into an explicit list:
+ * destinations in the serial loop's order, and for each destination its
+ * visits (source index, weight index) in the order the serial WEIGHT_OP runs.
+ * Every synapse kernel then walks that list.  Because the visit order is the
+ * serial order, every floating-point sum and every order-dependent update
+ * (the per-source time-since-spike of the conductance activators) follows the
+ * reference's serial build step for step.
+ *
+ * Floating point.  Expressions keep the reference's operand types: its
+ * double literals promote parts of the Izhikevich arithmetic to FP64 and the
+ * results are rounded back to float on assignment.  Build with
+ * -ffp-contract=off and without fast-math so that nothing is fused or
+ * reassociated.  MIN/MAX of the serial build are std::fmin/std::fmax
+ * (util/parallel.h:66-67); fminf/fmaxf are used here with the same meaning.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Connection types and opcodes of this library (names as in the reference's
+ * JSON vocabulary; numbering is this file's own). */
+enum { ORC_FC = 0, ORC_SUBSET, ORC_ONE_TO_ONE, ORC_CONVERGENT, ORC_DIVERGENT };
+enum {
+    ORC_ADD = 0, ORC_SUB, ORC_MULT, ORC_DIV, ORC_POOL, ORC_REWARD, ORC_GAP,
+    ORC_ADD_HEAVISIDE, ORC_SUB_HEAVISIDE, ORC_MULT_HEAVISIDE
+};
+enum { ORC_OUT_FLOAT = 0, ORC_OUT_BIT = 1, ORC_OUT_INT = 2 };
+
+/* Geometry of one connection: what SynapseData carries into the loop nests
+ * (engine/kernel/synapse_data.cpp:9-31, synapse_kernel.h:47-69,168-180,
+ * 370-381).  Row/column pairs are [0] = row, [1] = column.  sub[] is the
+ * subset rectangle: from row start/end, from column start/end, then the same
+ * four for the destination. */
+typedef struct {
+    int type;
+    int from_rows, from_cols, to_rows, to_cols;
+    int convolutional, wrap;
+    int field[2], stride[2], spacing[2], offset[2];
+    int sub[8];
+    long num_weights;
+} orc_geom;
+
+static int wrap_index(int x, int n) { return (x % n + n) % n; }
+
+/* Emits one visit; returns the new count.  Arrays may be NULL (count only). */
+static long emit(long nv, int *src, long *widx, int from, long w) {
+    if (src) src[nv] = from;
+    if (widx) widx[nv] = w;
+    return nv + 1;
+}
+
+/* Opens a destination: records its index and where its visits start. */
+static long open_dest(long nd, int *dest, long *first, int to, long nv) {
+    if (dest) dest[nd] = to;
+    if (first) first[nd] = nv;
+    return nd + 1;
+}
+
+/*
+ * Enumerates the serial loop nest of one connection.
+ *   FULLY_CONNECTED  synapse_kernel.h:115-149 (to-major, from ascending,
+ *                    weight to*from_size + from)
+ *   SUBSET           synapse_kernel.h:182-228 (destination rectangle
+ *                    row-major; weight to_k*from_kernel_size + from_k)
+ *   ONE_TO_ONE       synapse_kernel.h:257-286 (weight = index)
+ *   CONVERGENT       synapse_kernel.h:384-461 (source start
+ *                    to*stride + spacing*offset; wrap or skip; k advances
+ *                    on skipped visits; weight offset 0 if convolutional)
+ *   DIVERGENT        synapse_kernel.h:506-581 (start by C truncating
+ *                    division, end = start + spacing*(field-stride)/stride;
+ *                    weight offset to*(num_weights/to_size))
+ * Returns the number of visits; *ndest receives the number of destinations.
+ * first[] gets ndest+1 entries when non-NULL.  Returns -1 for a geometry the
+ * reference's loops would divide by zero on.
+ */
+long orc_enumerate(const orc_geom *g, int *dest, long *first, int *src,
+                   long *widx, long *ndest) {
+    const int fr = g->from_rows, fc = g->from_cols;
+    const int tr = g->to_rows, tc = g->to_cols;
+    const long from_size = (long)fr * fc, to_size = (long)tr * tc;
+    long nv = 0, nd = 0;
+
+    switch (g->type) {
+    case ORC_FC:
+        for (int to = 0; to < to_size; ++to) {
+            nd = open_dest(nd, dest, first, to, nv);
+            for (int from = 0; from < from_size; ++from)
+                nv = emit(nv, src, widx, from, (long)to * from_size + from);
+        }
+        break;
+    case ORC_SUBSET: {
+        const int frs = g->sub[0], fre = g->sub[1];
+        const int fcs = g->sub[2], fce = g->sub[3];
+        const int trs = g->sub[4], tre = g->sub[5];
+        const int tcs = g->sub[6], tce = g->sub[7];
+        const long from_k_size = (long)(fre - frs) * (fce - fcs);
+        const int k_rows = tre - trs, k_cols = tce - tcs;
+        for (int i = 0; i < k_rows; ++i)
+            for (int j = 0; j < k_cols; ++j) {
+                int to = (i + trs) * tc + (j + tcs);
+                long to_k = (long)i * k_cols + j;
+                long from_k = 0;
+                nd = open_dest(nd, dest, first, to, nv);
+                for (int r = frs; r < fre; ++r)
+                    for (int c = fcs; c < fce; ++c, ++from_k)
+                        nv = emit(nv, src, widx, r * fc + c,
+                                  to_k * from_k_size + from_k);
+            }
+        break;
+    }
+    case ORC_ONE_TO_ONE:
+        for (int i = 0; i < from_size; ++i) {
+            nd = open_dest(nd, dest, first, i, nv);
+            nv = emit(nv, src, widx, i, i);
+        }
+        break;
+    case ORC_CONVERGENT: {
+        const long ksize = (long)g->field[0] * g->field[1];
+        for (int to_row = 0; to_row < tr; ++to_row)
+            for (int to_col = 0; to_col < tc; ++to_col) {
+                int to = to_row * tc + to_col;
+                long base = g->convolutional ? 0 : to * ksize;
+                int s_row = to_row * g->stride[0] + g->spacing[0] * g->offset[0];
+                int s_col = to_col * g->stride[1] + g->spacing[1] * g->offset[1];
+                long k = 0;
+                nd = open_dest(nd, dest, first, to, nv);
+                for (int kr = 0; kr < g->field[0]; ++kr)
+                    for (int kc = 0; kc < g->field[1]; ++kc, ++k) {
+                        int r = s_row + kr * g->spacing[0];
+                        int c = s_col + kc * g->spacing[1];
+                        if (g->wrap) {
+                            r = wrap_index(r, fr);
+                            c = wrap_index(c, fc);
+                        } else if (r < 0 || r >= fr || c < 0 || c >= fc) {
+                            continue;
+                        }
+                        nv = emit(nv, src, widx, r * fc + c, base + k);
+                    }
+            }
+        break;
+    }
+    case ORC_DIVERGENT: {
+        if (g->stride[0] == 0 || g->stride[1] == 0) return -1;
+        const long per_dest = to_size ? g->num_weights / to_size : 0;
+        for (int to_row = 0; to_row < tr; ++to_row)
+            for (int to_col = 0; to_col < tc; ++to_col) {
+                int to = to_row * tc + to_col;
+                long base = g->convolutional ? 0 : to * per_dest;
+                /* C division truncates toward zero, as in the reference */
+                int r0 = (to_row + g->spacing[0] *
+                          (-g->offset[0] - g->field[0] + g->stride[0])) /
+                         g->stride[0];
+                int c0 = (to_col + g->spacing[1] *
+                          (-g->offset[1] - g->field[1] + g->stride[1])) /
+                         g->stride[1];
+                int r1 = r0 + (g->spacing[0] * (g->field[0] - g->stride[0]) /
+                               g->stride[0]);
+                int c1 = c0 + (g->spacing[1] * (g->field[1] - g->stride[1]) /
+                               g->stride[1]);
+                long k = 0;
+                nd = open_dest(nd, dest, first, to, nv);
+                for (int sr = r0; sr <= r1; sr += g->spacing[0])
+                    for (int sc = c0; sc <= c1; sc += g->spacing[1], ++k) {
+                        int r = sr, c = sc;
+                        if (g->wrap) {
+                            r = wrap_index(r, fr);
+                            c = wrap_index(c, fc);
+                        } else if (r < 0 || r >= fr || c < 0 || c >= fc) {
+                            continue;
+                        }
+                        nv = emit(nv, src, widx, r * fc + c, base + k);
+                    }
+            }
+        break;
+    }
+    default:
+        return -1;
+    }
+    if (first) first[nd] = nv;
+    *ndest = nd;
+    return nv;
+}
+
+/* EXTRACTOR (engine/kernel/extractor.cpp:6-14): FLOAT reads the float, INT
+ * converts the unsigned word, BIT takes bit (31 - delay%32) of the word. */
+static float extract(uint32_t word, int out_type, int delay) {
+    if (out_type == ORC_OUT_BIT)
+        return (float)((word << (delay & 0x1F)) >> 31);
+    if (out_type == ORC_OUT_INT)
+        return (float)word;
+    float f;
+    memcpy(&f, &word, 4);
+    return f;
+}
+
+/* AGGREGATOR (engine/kernel/aggregator.cpp:15-31). */
+static float aggregate(int op, float prior, float input) {
+    switch (op) {
+    case ORC_ADD: return prior + input;
+    case ORC_SUB: return prior - input;
+    case ORC_MULT: return prior * input;
+    case ORC_DIV: return prior / input;
+    case ORC_POOL: return fmaxf(prior, input);
+    case ORC_ADD_HEAVISIDE: return prior + (input > 0);
+    case ORC_SUB_HEAVISIDE: return prior - (input > 0);
+    case ORC_MULT_HEAVISIDE: return prior * (input > 0);
+    }
+    return prior;
+}
+
+/*
+ * activate_base (engine/kernel/kernel.cpp:217, synapse_kernel.h:887-912):
+ * per destination sum = 0.0f; sum += extract(src) * w in visit order; then
+ * input[to] = aggregate(input[to], sum).  src_out points at the source's
+ * output register that holds the connection's delay (word delay/32 for BIT,
+ * register `delay` for FLOAT; synapse_data.cpp:43-51).
+ */
+void orc_activate_base(long ndest, const int *dest, const long *first,
+                       const int *src, const long *widx, const float *weights,
+                       const uint32_t *src_out, int out_type, int delay,
+                       int opcode, float *inputs) {
+    for (long d = 0; d < ndest; ++d) {
+        float sum = 0.0f;
+        for (long v = first[d]; v < first[d + 1]; ++v) {
+            float val = extract(src_out[src[v]], out_type, delay) *
+                        weights[widx[v]];
+            sum += val;
+        }
+        inputs[dest[d]] = aggregate(opcode, inputs[dest[d]], sum);
+    }
+}
+
+/*
+ * Izhikevich conductance activators activate_iz_{add,sub,mult,reward}
+ * (state/impl/izhikevich_attributes.cpp:218-347).  Per visit: the spike bit
+ * at the synapse's own delay; the per-source time since spike
+ * read-modify-written in visit order; effective weight
+ * w * (stp_r * stp_w); the short (and, for add/sub, long) trace gains
+ * spike*weight*conductance, is added to the destination's sum and decays.
+ * After the visits the sums are added to the destination's conductances.
+ * kind: ORC_ADD (AMPA .8 / NMDA .993), ORC_SUB (GABA_A .833 / GABA_B .993),
+ * ORC_MULT (multiplicative factor .95), ORC_REWARD (dopamine .95).
+ */
+void orc_iz_activate(int kind, long ndest, const int *dest, const long *first,
+                     const int *src, const long *widx, const float *weights,
+                     const uint32_t *src_out, int out_type,
+                     const int32_t *delays, const float *stp_r,
+                     const float *stp_w, float *short_traces,
+                     float *long_traces, int32_t *from_tss,
+                     float baseline_conductance, float *short_cond,
+                     float *long_cond) {
+    float short_tau, long_tau = 0.0f;
+    int has_long = 0;
+    switch (kind) {
+    case ORC_ADD: short_tau = 0.8; long_tau = 0.993; has_long = 1; break;
+    case ORC_SUB: short_tau = 0.833; long_tau = 0.993; has_long = 1; break;
+    default: short_tau = 0.95; break;
+    }
+    for (long d = 0; d < ndest; ++d) {
+        const int to = dest[d];
+        float short_sum = 0.0, long_sum = 0.0;
+        for (long v = first[d]; v < first[d + 1]; ++v) {
+            const int from = src[v];
+            const long w = widx[v];
+            float spike = extract(src_out[from], out_type, delays[w]);
+            from_tss[from] = (spike > 0.0)
+                ? 0 : (int32_t)fmin(32, from_tss[from] + 1);
+            float weight = weights[w] * (stp_r[w] * stp_w[w]);
+            float st = short_traces[w] + (spike * weight * baseline_conductance);
+            short_sum += st;
+            short_traces[w] = st * short_tau;
+            if (has_long) {
+                float lt = long_traces[w] +
+                           (spike * weight * baseline_conductance);
+                long_sum += lt;
+                long_traces[w] = lt * long_tau;
+            }
+        }
+        short_cond[to] += short_sum;
+        if (has_long) long_cond[to] += long_sum;
+    }
+}
+
+/* activate_iz_gap (izhikevich_attributes.cpp:349-360): gap-junction current
+ * sum of (v_from - v_to) * w added to the destination's input. */
+void orc_iz_gap(long ndest, const int *dest, const long *first,
+                const int *src, const long *widx, const float *weights,
+                const float *voltage, float *inputs) {
+    for (long d = 0; d < ndest; ++d) {
+        const int to = dest[d];
+        float v = voltage[to];
+        float sum = 0.0;
+        for (long e = first[d]; e < first[d + 1]; ++e) {
+            float weight = weights[widx[e]];
+            sum += (voltage[src[e]] - v) * weight;
+        }
+        inputs[to] += sum;
+    }
+}
+
+/*
+ * update_iz_add / update_iz_sub (izhikevich_attributes.cpp:404-520).  Per
+ * destination: its post trace, time since spike and current spike bit
+ * (extract of output word 0 at delay 0).  Per visit with w >= 1e-4f: the
+ * source spiked this step iff its per-connection time since spike is 0;
+ * STDP (add: presynaptic trace .004 / decay .933; sub: the iSTDP curves of
+ * the two times since spike); weight derivative
+ * dw -= (dw - 1e-6)*1e-4 + dest_spike*src_trace - src_spike*dest_trace;
+ * w = clamp(w + lr*dw, 1e-4f, max_weight); then short-term plasticity
+ * (add: U .5, F 1000, D 800; sub: U .2, F 20, D 700) when enabled.
+ */
+void orc_iz_update(int sub, long ndest, const int *dest, const long *first,
+                   const int *src, const long *widx, float *weights,
+                   float *presyn_traces, float *stp_rs, float *stp_ws,
+                   float *weight_derivatives, const int32_t *from_tss,
+                   const float *post_traces, const int32_t *dest_tss_arr,
+                   const uint32_t *dest_out, int out_type,
+                   float learning_rate, float max_weight, int stp_flag) {
+    const double U = sub ? 0.2 : 0.5;
+    const int F = sub ? 20 : 1000;
+    const int D = sub ? 700 : 800;
+    const float min_weight = 0.0001f;
+    for (long d = 0; d < ndest; ++d) {
+        const int to = dest[d];
+        float dest_exc_trace = post_traces[to];
+        int dest_tss = dest_tss_arr[to];
+        float dest_spike = extract(dest_out[to], out_type, 0);
+        for (long v = first[d]; v < first[d + 1]; ++v) {
+            const long w = widx[v];
+            float weight = weights[w];
+            if (!(weight >= min_weight)) continue;
+            int src_tss = from_tss[src[v]];
+            int src_spike = src_tss == 0;
+            float src_trace, dest_trace;
+            if (!sub) {
+                src_trace = presyn_traces[w] =
+                    src_spike ? 0.004 : presyn_traces[w] * 0.933;
+                dest_trace = dest_exc_trace;
+            } else {
+                src_trace = (dest_spike > 0.0 && src_tss > 0 && src_tss < 32)
+                    ? powf(src_tss, 10) * 0.000001 * 2.29 *
+                      expf(-1.1 * src_tss)
+                    : 0.0;
+                dest_trace = (src_spike && dest_tss > 0 && dest_tss < 32)
+                    ? powf(dest_tss, 10) * 0.0000001 * 2.6 *
+                      expf(-0.94 * dest_tss)
+                    : 0.0;
+            }
+            float weight_delta = weight_derivatives[w];
+            weight_delta = weight_derivatives[w] -=
+                (weight_delta - 0.000001) * 0.0001 +
+                (dest_spike * src_trace) - (src_spike * dest_trace);
+            weight += learning_rate * weight_delta;
+            weights[w] = fmaxf(min_weight, fminf(max_weight, weight));
+            if (stp_flag) {
+                float r = stp_rs[w];
+                float sw = stp_ws[w];
+                stp_rs[w] += ((1 - r) / D) - (r * sw * src_spike);
+                stp_ws[w] += ((U - sw) / F) + (U * (1 - sw) * src_spike);
+            }
+        }
+    }
+}
+
+/* SHIFT_BIT_OUTPUTS (state/attributes.h:321-333) for neuron i of a layer of
+ * n neurons with `history` words per neuron ([word][neuron] layout): every
+ * word moves one bit towards the LSB, taking the next newer word's LSB into
+ * its MSB; word 0 takes the new bit.  Returns the MSB word 0 had before,
+ * i.e. the previous step's bit. */
+static unsigned shift_bits(uint32_t *out, long n, int history, long i,
+                           unsigned bit) {
+    unsigned prev = out[i] >> 31;
+    for (int h = history - 1; h > 0; --h)
+        out[h * n + i] = (out[h * n + i] >> 1) | (out[(h - 1) * n + i] << 31);
+    out[i] = (out[i] >> 1) | (bit << 31);
+    return prev;
+}
+
+/* SHIFT_FLOAT_OUTPUTS (state/attributes.h:316-319). */
+static void shift_floats(uint32_t *out, long n, int history, long i,
+                         float value) {
+    for (int h = history - 1; h > 0; --h) out[h * n + i] = out[(h - 1) * n + i];
+    memcpy(&out[i], &value, 4);
+}
+
+/*
+ * iz_attribute_kernel (state/impl/izhikevich_attributes.cpp:102-213): up to
+ * ten 0.1 ms hybrid-Euler sub-steps while v < 30; conductances and the
+ * multiplicative factor are cleared, dopamine decays by .95; the spike bit
+ * is shifted in and the post trace / time since spike follow the PREVIOUS
+ * step's bit; spiking neurons reset v = c, u += d.
+ */
+void orc_iz_neuron(long n, int history, float *ampa, float *nmda,
+                   float *gabaa, float *gabab, float *mult, float *dopamine,
+                   float *voltages, float *recoveries, float *post_traces,
+                   int32_t *tss, uint32_t *out, const float *as,
+                   const float *bs, const float *cs, const float *ds,
+                   const float *inputs) {
+    for (long i = 0; i < n; ++i) {
+        float g_ampa = ampa[i], g_nmda = nmda[i];
+        float g_gabaa = gabaa[i], g_gabab = gabab[i];
+        float mfac = mult[i];
+        float voltage = voltages[i], recovery = recoveries[i];
+        float base_current = inputs[i];
+        float a = as[i], b = bs[i];
+        for (int it = 0; it < 10 && voltage < 30.0; ++it) {
+            float current = -g_ampa * voltage;
+            float temp = powf((voltage + 80) / 60, 2);
+            current -= g_nmda * (temp / (1 + temp)) * voltage;
+            current -= g_gabaa * (voltage + 70);
+            current -= g_gabab * (voltage + 90);
+            current *= 1 + mfac;
+            current += base_current;
+            float delta_v = (0.04 * voltage * voltage) + (5 * voltage) + 140 -
+                            recovery + current;
+            float sum_g = g_ampa + g_nmda + g_gabaa + g_gabab;
+            voltage = (voltage + 0.1 * (delta_v + (g_gabaa * -70) +
+                                        (g_gabab * -90))) /
+                      (1 + (0.1 * sum_g));
+            voltage = (voltage != voltage) ? 30.0 : voltage;
+            float adjusted_tau = (voltage > 30.0)
+                ? delta_v / (30.0 - voltage + delta_v) * 0.1
+                : 0.1;
+            recovery += a * adjusted_tau * ((b * voltage) - recovery);
+        }
+        ampa[i] = 0.0;
+        nmda[i] = 0.0;
+        gabaa[i] = 0.0;
+        gabab[i] = 0.0;
+        mult[i] = 0.0;
+        dopamine[i] *= 0.95;
+
+        unsigned spike = voltage >= 30.0;
+        unsigned prev_bit = shift_bits(out, n, history, i, spike);
+        post_traces[i] = prev_bit ? 0.004 : (post_traces[i] * 0.95);
+        tss[i] = prev_bit ? 0 : (int32_t)fmin(32, tss[i] + 1);
+        voltages[i] = spike ? cs[i] : voltage;
+        recoveries[i] = recovery + (spike * ds[i]);
+    }
+}
+
+/* gol_attribute_kernel (state/impl/game_of_life_attributes.cpp:14-33): the
+ * live count is the integer part of the input; the rule is chosen by BIT 0
+ * of word 0 (the state 32 steps back). */
+void orc_gol_neuron(long n, int history, const float *inputs, uint32_t *out,
+                    int survival_min, int survival_max, int birth_min,
+                    int birth_max) {
+    for (long i = 0; i < n; ++i) {
+        int input = (int)inputs[i];
+        int prev_live = out[i] & 1;
+        unsigned live = prev_live
+            ? (input >= survival_min && input <= survival_max)
+            : (input >= birth_min && input <= birth_max);
+        shift_bits(out, n, history, i, live);
+    }
+}
+
+/* re_attribute_kernel (state/impl/rate_encoding_attributes.cpp:14-23):
+ * positive inputs map to tanh(input), the rest to 0. */
+void orc_rate_neuron(long n, int history, const float *inputs, uint32_t *out) {
+    for (long i = 0; i < n; ++i) {
+        float input = inputs[i];
+        float value = (input > 0.0) ? tanhf(input) : 0.0;
+        shift_floats(out, n, history, i, value);
+    }
+}
+
+/* relay_attribute_kernel (state/impl/relay_attributes.cpp:10-18). */
+void orc_relay_neuron(long n, int history, const float *inputs, uint32_t *out,
+                      int ramp) {
+    for (long i = 0; i < n; ++i) {
+        float input = inputs[i];
+        float value = ramp ? fmax(0.0, input) : input;
+        shift_floats(out, n, history, i, value);
+    }
+}
+
+/*
+ * Izhikevich parameter table and initial state (izhikevich_attributes.cpp:
+ * 28-35 table, 37-79 lookup, 626-650 constructor): a, b, c, d from the named
+ * set; post trace 0; time since spike 32; v, u from iterating the
+ * membrane equation from (c, b*c) until |dv| or |du| (float magnitudes)
+ * drops to 0.001.  Returns -1 for an unknown name (the random parameter sets
+ * use the reference's unseeded generator and are not restated).
+ */
+int orc_iz_init(const char *params, long n, float *as, float *bs, float *cs,
+                float *ds, float *v, float *u, float *post, int32_t *tss) {
+    static const struct { const char *name; float a, b, c, d; } table[] = {
+        {"default", 0.02, 0.2, -70.0, 2},
+        {"regular", 0.02, 0.2, -65.0, 8},
+        {"bursting", 0.02, 0.2, -55.0, 4},
+        {"chattering", 0.02, 0.2, -50.0, 2},
+        {"fast", 0.1, 0.2, -65.0, 2},
+        {"low_threshold", 0.02, 0.25, -65.0, 2},
+        {"thalamo_cortical", 0.02, 0.25, -65.0, 0.05},
+        {"resonator", 0.1, 0.26, -65.0, 2},
+    };
+    int row = -1;
+    for (int i = 0; i < (int)(sizeof(table) / sizeof(table[0])); ++i)
+        if (strcmp(table[i].name, params) == 0) row = i;
+    if (row < 0) return -1;
+    for (long j = 0; j < n; ++j) {
+        as[j] = table[row].a;
+        bs[j] = table[row].b;
+        cs[j] = table[row].c;
+        ds[j] = table[row].d;
+        post[j] = 0.0;
+        tss[j] = 32;
+        float vv = cs[j];
+        float r = bs[j] * cs[j];
+        float delta_v, delta_r;
+        float a = as[j], b = bs[j];
+        do {
+            delta_v = (0.04 * vv * vv) + (5 * vv) + 140 - r;
+            vv += delta_v;
+            delta_r = a * ((b * vv) - r);
+            r += delta_r;
+        } while (fabsf(delta_v) > 0.001 && fabsf(delta_r) > 0.001);
+        v[j] = vv;
+        u[j] = r;
+    }
+    return 0;
+}
+
+/* Inverted circular mask of radius 0.5 used by "diagonal": false on a
+ * convergent connection (state/weight_matrix.cpp:749-823, 951-977): weights
+ * whose kernel position lies within the ellipse are set to 0.  `kernels` is
+ * 1 for a convolutional connection, else the destination count. */
+void orc_clear_diagonal_convergent(float *weights, long kernels, int rf,
+                                   int cf) {
+    float row_radius = 0.5f, col_radius = 0.5f;
+    float row_center = (float)rf / 2, col_center = (float)cf / 2;
+    float rr_sq = powf(row_radius, 2), cr_sq = powf(col_radius, 2);
+    float sq_mult = rr_sq * cr_sq;
+    for (long k = 0; k < kernels; ++k)
+        for (int r = 0; r < rf; ++r)
+            for (int c = 0; c < cf; ++c) {
+                float term = (powf(r + 0.5 - row_center, 2) * cr_sq) +
+                             (powf(c + 0.5 - col_center, 2) * rr_sq);
+                if (term <= sq_mult)
+                    weights[k * rf * cf + r * cf + c] = 0.0f;
+            }
+}
