@@ -75,11 +75,11 @@ def num_weights(c, layers):
 
 # -- cases -------------------------------------------------------------------
 
-def conn_zoo(steps=12):
+def conn_zoo(steps=12, workload=101, name="conn_zoo"):
     """Base activator over every connection type, opcode, FLOAT and BIT
     sources, word-0 and word-1 delays, wrap/no-wrap, stride, spacing,
     offset and convolutional kernels (engine/kernel/synapse_kernel.h)."""
-    st = Stream(101)
+    st = Stream(workload)
     L = [layer("src", "relay", 12, 10),
          layer("bsrc", "game_of_life", 12, 10),
          layer("d_fc", "relay", 7, 9),
@@ -145,7 +145,7 @@ def conn_zoo(steps=12):
     inputs = {("s", "src"): lambda t: st.normal(120, step=t + 1, stream=2)}
     dests = [l["name"] for l in L if l["name"].startswith("d_")]
     capture = [("s", d, "input") for d in dests] + [("s", "bsrc", "output")]
-    return Case("conn_zoo", n, steps, weights=weights, init=init,
+    return Case(name, n, steps, weights=weights, init=init,
                 inputs=inputs, capture=capture,
                 final_neuron=[("s", "bsrc", "output")])
 
@@ -170,10 +170,10 @@ def gol(name, rows, cols, steps, workload, wrap=True, conway_init=False):
                               ("gol", "board", "input")])
 
 
-def iz_fc(size=1000, steps=200, name="iz_fc"):
+def iz_fc(size=1000, steps=200, name="iz_fc", workload=1):
     """Two Izhikevich layers, FULLY_CONNECTED size->size direct, weights
     U[0,0.5), input N(5,2) into layer 1, v0=-65, u0=b*v0."""
-    st = Stream(1)
+    st = Stream(workload)
     L = [layer("L1", "izhikevich", 1, size, params="regular"),
          layer("L2", "izhikevich", 1, size, params="regular")]
     C = [conn("fc", "L1", "L2", "fully connected", direct=True,
@@ -245,11 +245,11 @@ def iz_sheet(e_rows=32, e_cols=40, steps=60, name="iz_sheet",
                 final_conn=[("EE", k) for k in ee_keys])
 
 
-def iz_ops(steps=80):
+def iz_ops(steps=80, workload=104, name="iz_ops"):
     """Izhikevich conductance activators for every opcode and connection
     type, both plastic updaters (STDP add, iSTDP sub), gap junctions, a
     direct connection and a myelinated delay."""
-    st = Stream(104)
+    st = Stream(workload)
     L = [layer("A", "izhikevich", 8, 10, params="regular"),
          layer("B", "izhikevich", 8, 10, params="fast"),
          layer("C", "izhikevich", 6, 6, params="bursting")]
@@ -289,7 +289,7 @@ def iz_ops(steps=80):
     keys = ["weights", "presyn trace", "short term R", "short term w",
             "weight derivatives", "time since spike", "short trace",
             "long trace"]
-    return Case("iz_ops", net("ops", L, C), steps, weights=weights,
+    return Case(name, net("ops", L, C), steps, weights=weights,
                 delays=delays, init=init, inputs=inputs,
                 capture=[("ops", l, k) for l in ("A", "B", "C")
                          for k in ("output", "voltage")],
@@ -301,10 +301,10 @@ def iz_ops(steps=80):
                 final_conn=[(c, k) for c in plastic for k in keys])
 
 
-def rate(steps=30):
+def rate(steps=30, workload=105, name="rate"):
     """Rate-encoding layers (rectified tanh) behind FULLY_CONNECTED and
     CONVERGENT connections from a relay input layer."""
-    st = Stream(105)
+    st = Stream(workload)
     L = [layer("in", "relay", 8, 8),
          layer("h", "rate_encoding", 1, 48),
          layer("c", "rate_encoding", 4, 4),
@@ -319,7 +319,7 @@ def rate(steps=30):
         nw = num_weights(c, L)
         weights[c["name"]] = st.normal(nw, stream=10 + i, std=0.5)
     inputs = {("rate", "in"): lambda t: st.uniform(64, step=t + 1, stream=2)}
-    return Case("rate", net("rate", L, C), steps, weights=weights,
+    return Case(name, net("rate", L, C), steps, weights=weights,
                 inputs=inputs,
                 capture=[("rate", l, "output") for l in ("h", "c", "o")],
                 final_neuron=[("rate", l, "output") for l in ("h", "c", "o")])

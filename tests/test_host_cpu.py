@@ -1,0 +1,193 @@
+"""Host side of the MI355X library, on the CPU (no HIP device needed).
+
+* The C-ABI libraries export every function include/*.h declares.
+* The state the host builds -- Izhikevich parameters and settle loop,
+  synapse arrays, injected weights and delays, history words -- equals the
+  oracle's initial state for every golden case, array by array.
+* Network and state files round-trip through save/load.
+* The property tree round-trips through the C-ABI.
+* Without a HIP device ``run`` fails loudly (null report), never on a CPU
+  fallback.
+"""
+
+import copy
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import harness
+import netdefs
+from oracle import stepper
+from synaptogenesis_amd import syngen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "synaptogenesis_amd")
+HEADERS = {"syngen.h": "libsyngen.so", "sg_hip.h": "libsg_hip.so"}
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set()
+    for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+        name = m.group(1)
+        before = text[:m.start()].rstrip()
+        if name in ("if", "sizeof", "defined", "__attribute__"):
+            continue
+        # a declaration: the name follows a return type, not an operator
+        if before and (before[-1].isalnum() or before[-1] in "*_"):
+            names.add(name)
+    return names
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_library_exports_every_declared_function(header):
+    names = declared_functions(header)
+    assert len(names) > 10
+    lib = ctypes.CDLL(os.path.join(PKG, HEADERS[header]))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_syngen_exports_the_reference_symbol_set():
+    """The 51 functions the reference defines behind lib/syngen.py
+    (extern.h:24-97), plus get_num_gpus, which it declares (extern.h:77)."""
+    names = declared_functions("syngen.h")
+    assert len(names) == 52, sorted(names)
+
+
+def neuron_keys(layer):
+    if layer.model == "izhikevich":
+        return stepper.IZ_VARS + ["time since spike", "output", "input"]
+    return ["output", "input"]
+
+
+def conn_keys(conn):
+    keys = ["weights"]
+    if conn.iz:
+        keys += ["short trace", "long trace", "presyn trace", "short term R",
+                 "short term w", "weight derivatives", "time since spike"]
+        if conn.arrays.get("delays") is not None:
+            keys.append("delays")
+    return keys
+
+
+@pytest.mark.parametrize("name", [c.name for c in netdefs.all_cases()])
+def test_initial_state_matches_oracle(lib, name):
+    case = copy.copy([c for c in netdefs.all_cases() if c.name == name][0])
+    net = stepper.Network(case)
+    case.steps = 0
+    case.capture = []
+    case.final_neuron = [(l.structure, l.name, k) for l in net.order
+                         for k in neuron_keys(l)]
+    case.final_conn = [(c.name, k) for c in net.conns for k in conn_keys(c)]
+    _, finals, _ = harness.run_case(case, lib)
+    bad = []
+    for (s, l, key) in case.final_neuron:
+        want = net.neuron_data(s, l, key)
+        got = finals[("neuron", s, l, key)]
+        if got.view(np.uint32).tobytes() != want.view(np.uint32).tobytes():
+            bad.append((l, key))
+    for (c, key) in case.final_conn:
+        want = net.conn_data(c, key)
+        got = finals[("conn", c, key)]
+        if got.view(np.uint32).tobytes() != want.view(np.uint32).tobytes():
+            bad.append((c, key))
+    assert not bad, bad
+
+
+def _snapshot(network, case, net):
+    out = {}
+    for layer in net.order:
+        for k in neuron_keys(layer):
+            out[(layer.name, k)] = harness._view(
+                network.get_neuron_data(layer.structure, layer.name, k), k)
+    for c in net.conns:
+        for k in conn_keys(c):
+            out[(c.name, k)] = harness._view(
+                network.get_weight_matrix(c.name, k), k)
+    return out
+
+
+def test_state_save_load_round_trip(lib, tmp_path):
+    case = netdefs.iz_ops()
+    net = stepper.Network(case)
+    network = syngen.Network(case.net, lib)
+    network.build_state()
+    # write distinct values through the views, then save
+    rng = np.random.default_rng(7)
+    for c in net.conns:
+        view = harness._writable(network.get_weight_matrix(c.name), "w")
+        view[:] = rng.random(view.size, dtype=np.float32)
+    v = harness._writable(network.get_neuron_data("ops", "A", "voltage"),
+                          "voltage")
+    v[:] = rng.random(v.size, dtype=np.float32) - 70.0
+    before = _snapshot(network, case, net)
+    path = str(tmp_path / "ops.bin")
+    assert network.save_state(path)
+    # a fresh state of the same network, loaded from the file
+    fresh = syngen.Network(case.net, lib)
+    assert fresh.build_state(path)
+    after = _snapshot(fresh, case, net)
+    for k in before:
+        assert before[k].tobytes() == after[k].tobytes(), k
+    network.free()
+    fresh.free()
+
+
+def test_network_file_round_trip(lib, tmp_path):
+    """A saved network loads back to the same content.  Key order follows
+    the reference: insertion order for trees built through the C-ABI, the
+    JSON reader's (sorted) order after a load, so a second save of a loaded
+    file reproduces it byte for byte."""
+    import json
+    case = netdefs.iz_sheet()
+    network = syngen.Network(case.net, lib)
+    first = tmp_path / "first.json"
+    assert network.save(str(first))
+    loaded = syngen.Network(str(first), lib)
+    assert loaded.obj
+    second = tmp_path / "second.json"
+    assert loaded.save(str(second))
+    assert json.loads(first.read_text()) == json.loads(second.read_text())
+    again = syngen.Network(str(second), lib)
+    third = tmp_path / "third.json"
+    assert again.save(str(third))
+    assert third.read_text() == second.read_text()
+    for n in (network, loaded, again):
+        n.free()
+
+
+def test_property_tree_round_trip(lib):
+    src = {"name": "x", "rows": "3", "child": {"k": "v", "n": "1"},
+           "list": ["a", "b"], "kids": [{"p": "1"}, {"q": "2"}]}
+    p = syngen.Properties(src, lib)
+    back = syngen.Properties(p.obj, lib).to_dict()
+    assert back["name"] == "x" and back["rows"] == "3"
+    assert back["child"] == {"k": "v", "n": "1"}
+    assert back["list"] == ["a", "b"]
+    assert back["kids"] == [{"p": "1"}, {"q": "2"}]
+    p.free()
+
+
+def test_invalid_networks_are_rejected(lib):
+    lib.set_suppress_output(True)
+    bad = netdefs.net("s", [netdefs.layer("A", "izhikevich", 2, 2)],
+                      [netdefs.conn("c", "A", "Z", "fully connected")])
+    assert not syngen.Network(bad, lib).obj
+    bad = netdefs.net("s", [netdefs.layer("A", "no_such_model", 2, 2)], [])
+    net = syngen.Network(bad, lib)
+    assert not net.obj or not net.build_state()
+
+
+def test_run_without_device_fails_loudly(lib):
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a HIP device is present")
+    case = netdefs.gol("g", 6, 6, 2, 1)
+    _, _, report = harness.run_case(case, lib)
+    assert report is not None and report.get("iterations") is None
