@@ -20,29 +20,50 @@ _W1 = np.uint32(0xBB67AE85)
 _MASK = np.uint64(0xFFFFFFFF)
 
 
+_CHUNK = 1 << 18
+
+
+def _rounds(c0, c1, c2, c3, k0, k1, rounds):
+    """Philox rounds over equal-length uint32 arrays, in place."""
+    p0 = np.empty(c0.shape, np.uint64)
+    p1 = np.empty(c0.shape, np.uint64)
+    t = np.empty(c0.shape, np.uint64)
+    for r in range(rounds):
+        np.multiply(c0, _M0, out=p0, dtype=np.uint64)
+        np.multiply(c2, _M1, out=p1, dtype=np.uint64)
+        # c0' = hi(p1) ^ c1 ^ k0 ; c2' = hi(p0) ^ c3 ^ k1
+        np.right_shift(p1, np.uint64(32), out=t)
+        np.bitwise_xor(t.astype(np.uint32), c1, out=c0)
+        np.bitwise_xor(c0, k0, out=c0)
+        np.right_shift(p0, np.uint64(32), out=t)
+        np.bitwise_xor(t.astype(np.uint32), c3, out=c2)
+        np.bitwise_xor(c2, k1, out=c2)
+        # c1' = lo(p1) ; c3' = lo(p0)
+        c1[:] = p1            # uint64 -> uint32 keeps the low word
+        c3[:] = p0
+        if r != rounds - 1:
+            k0 = np.uint32((int(k0) + int(_W0)) & 0xFFFFFFFF)
+            k1 = np.uint32((int(k1) + int(_W1)) & 0xFFFFFFFF)
+
+
 def philox4x32(c0, c1, c2, c3, k0, k1, rounds=10):
     """Vectorised Philox4x32 over uint32 arrays (broadcast together).
 
-    Returns the four output words as uint32 arrays.
+    Returns the four output words as uint32 arrays.  Works through the
+    arrays in cache-sized chunks.
     """
     c0, c1, c2, c3 = (np.asarray(c, dtype=np.uint32) for c in (c0, c1, c2, c3))
     c0, c1, c2, c3 = np.broadcast_arrays(c0, c1, c2, c3)
-    c0, c1, c2, c3 = (c.astype(np.uint32).copy() for c in (c0, c1, c2, c3))
+    shape = c0.shape
+    out = [np.ascontiguousarray(c, dtype=np.uint32).reshape(-1).copy()
+           for c in (c0, c1, c2, c3)]
     k0 = np.uint32(k0)
     k1 = np.uint32(k1)
+    n = out[0].size
     with np.errstate(over="ignore"):
-        for r in range(rounds):
-            p0 = _M0 * c0.astype(np.uint64)
-            p1 = _M1 * c2.astype(np.uint64)
-            hi0 = (p0 >> np.uint64(32)).astype(np.uint32)
-            lo0 = (p0 & _MASK).astype(np.uint32)
-            hi1 = (p1 >> np.uint64(32)).astype(np.uint32)
-            lo1 = (p1 & _MASK).astype(np.uint32)
-            c0, c1, c2, c3 = hi1 ^ c1 ^ k0, lo1, hi0 ^ c3 ^ k1, lo0
-            if r != rounds - 1:
-                k0 = np.uint32(k0 + _W0)
-                k1 = np.uint32(k1 + _W1)
-    return c0, c1, c2, c3
+        for i in range(0, n, _CHUNK):
+            _rounds(*(o[i:i + _CHUNK] for o in out), k0, k1, rounds)
+    return tuple(o.reshape(shape) for o in out)
 
 
 class Stream:

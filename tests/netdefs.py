@@ -338,6 +338,63 @@ def iz_init():
                 final_neuron=[("init", p, k) for p in names for k in keys])
 
 
+HISTORY_DELAYS = (0, 31, 32, 63, 64, 95)
+
+
+def history(steps=110, workload=108, name="history"):
+    """Delays at and across history-word boundaries (0, 31, 32, 63, 64, 95
+    steps) from a FLOAT source (one register per step) and a BIT source
+    (32 steps per word), and an Izhikevich conductance connection whose
+    connection delay (40) puts its per-synapse delays U{0..31} in word 1
+    (constants.h:24-37, synapse_data.cpp:43-51)."""
+    st = Stream(workload)
+    L = [layer("f", "relay", 5, 7), layer("b", "game_of_life", 5, 7),
+         layer("E", "izhikevich", 5, 7, params="regular")]
+    C = [conn("life", "b", "b", "convergent", convolutional=True,
+              arbor={"field size": 3, "wrap": True},
+              weight={"type": "flat", "weight": 1, "diagonal": False}),
+         conn("EE", "E", "E", "fully connected", plastic=True, delay=40,
+              learning_rate=0.05)]
+    for d in HISTORY_DELAYS:
+        L += [layer("df%d" % d, "relay", 5, 7),
+              layer("db%d" % d, "relay", 5, 7)]
+        C += [conn("f%d" % d, "f", "df%d" % d, "one to one", delay=d),
+              conn("b%d" % d, "b", "db%d" % d, "convergent", "add",
+                   delay=d, arbor={"field size": 3, "wrap": True})]
+    weights = {c["name"]: st.uniform(num_weights(c, L), stream=10 + i,
+                                     low=0.0, high=1.0)
+               for i, c in enumerate(C) if c["name"] != "life"}
+    delays = {"EE": st.integers(35 * 35, 0, 31, stream=5)}
+    init = {("h", "b", "output"): st.bits(3 * 35, stream=1),
+            ("h", "E", "voltage"): np.full(35, -65.0, np.float32),
+            ("h", "E", "recovery"): np.full(
+                35, np.float32(0.2) * np.float32(-65.0), np.float32)}
+    inputs = {("h", "f"): lambda t: st.normal(35, step=t + 1, stream=2),
+              ("h", "E"): lambda t: st.normal(35, step=t + 1, stream=3,
+                                              mean=6.0, std=4.0)}
+    capture = [("h", "df%d" % d, "input") for d in HISTORY_DELAYS] + \
+        [("h", "db%d" % d, "input") for d in HISTORY_DELAYS] + \
+        [("h", "E", "output"), ("h", "b", "output")]
+    return Case(name, net("h", L, C), steps, weights=weights, delays=delays,
+                init=init, inputs=inputs, capture=capture,
+                final_neuron=[("h", "b", "output"), ("h", "E", "output"),
+                              ("h", "E", "voltage")],
+                final_conn=[("EE", k) for k in
+                            ("weights", "short term R", "short term w",
+                             "weight derivatives", "presyn trace")])
+
+
+def ragged_cases():
+    """Sizes that are not multiples of the 64-wide tiles: a 1x1 board
+    (every arbor visit wraps onto the one cell), a 1x70 strip, a 65-neuron
+    fully connected pair and an 18x22 / 9x11 E/I sheet."""
+    return [gol("gol_tiny", 1, 1, 12, 111),
+            gol("gol_strip", 1, 70, 40, 110),
+            iz_fc(size=65, steps=120, name="iz_fc_65", workload=112),
+            iz_sheet(18, 22, 60, "iz_sheet_ragged", noise=(1.0, 4.0),
+                     workload=109)]
+
+
 def all_cases():
     return [conn_zoo(), gol("gol_small", 24, 20, 80, 102),
             gol("gol_conway", 20, 24, 60, 103, conway_init=True),
@@ -345,4 +402,4 @@ def all_cases():
             iz_fc(), iz_sheet(),
             iz_sheet(24, 32, 80, "iz_sheet_hot", noise=(2.0, 5.0),
                      workload=107),
-            iz_ops(), rate(), iz_init()]
+            iz_ops(), rate(), iz_init(), history()] + ragged_cases()
