@@ -33,6 +33,8 @@ from synaptogenesis_amd import syngen, workloads  # noqa: E402
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libsyngen_ref.so")
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles",
+                               "iz_update_activate_e_i_sheet_traffic.txt")
 
 # Algorithmic bytes of one launch of the fused E->E update+activate kernel:
 # per synapse RW weight, stp R, stp w, short trace, long trace, presyn trace,
@@ -49,6 +51,20 @@ def fused_bytes(ne):
 def step_bytes(ne, ni):
     return (ne * 225 * 60 + ni * 225 * 4 + ne * 16 * 4 + (ne + ni) * 112 +
             ne * 8 + (ne + ne + ni) * 4)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused kernel, from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter
+    reads for a plain copy of known size), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def build(lib, wl):
@@ -221,7 +237,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "iz_update_activate (E->E)",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None, "avg_ms": fused_ms},
+                     "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
+                     "avg_ms": fused_ms},
         "kernel_avg_ms": kernels,
         "cpu_baseline": cpu,
     }
