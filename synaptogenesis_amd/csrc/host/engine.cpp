@@ -134,6 +134,16 @@ Props *Engine::run(const Props &args) {
     }
     check_hip(hipSetDevice(device), "hipSetDevice");
 
+    // The step order below is the parallel cluster's
+    // (engine/cluster/parallel_cluster.cpp:107-131).  Sequential and
+    // feedforward clusters interleave activation and state update per layer
+    // in a dependency order (sequential_cluster.cpp:16-77, 100-113); they
+    // are refused rather than run in the wrong order.
+    for (auto &st : net_->structures())
+        if (st->cluster != CLUSTER_PARALLEL)
+            throw Error("Structure " + st->name + ": only \"parallel\" "
+                        "clusters are supported by this engine");
+
     state_->to_device();
 
     const bool learning = args.get_bool("learning flag", true);

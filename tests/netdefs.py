@@ -395,6 +395,35 @@ def ragged_cases():
                      workload=109)]
 
 
+def gol_rule(name="gol_rule", workload=113):
+    """Game of life with non-default rule ranges: survive on 1-4, born on
+    3-5 neighbours (game_of_life_attributes.cpp:39-45)."""
+    case = gol(name, 20, 22, 50, workload)
+    case.net["structures"][0]["layers"][0].update(
+        {"survival_min": "1", "survival_max": "4", "birth_min": "3",
+         "birth_max": "5"})
+    return case
+
+
+def relay_ramp(steps=15, workload=114, name="relay_ramp"):
+    """Relay layers with "ramp" (output max(0, input)) between fully
+    connected projections, one delayed by two steps
+    (relay_attributes.cpp:10-27)."""
+    st = Stream(workload)
+    L = [layer("in", "relay", 6, 6), layer("r", "relay", 1, 20, ramp=True),
+         layer("o", "relay", 1, 5)]
+    C = [conn("in_r", "in", "r", "fully connected"),
+         conn("r_o", "r", "o", "fully connected", "sub", delay=2)]
+    weights = {c["name"]: st.normal(num_weights(c, L), stream=10 + i,
+                                    std=0.7) for i, c in enumerate(C)}
+    inputs = {("ramp", "in"): lambda t: st.normal(36, step=t + 1, stream=2)}
+    return Case(name, net("ramp", L, C), steps, weights=weights,
+                inputs=inputs,
+                capture=[("ramp", "r", "output"), ("ramp", "o", "input")],
+                final_neuron=[("ramp", "r", "output"),
+                              ("ramp", "o", "output")])
+
+
 def all_cases():
     return [conn_zoo(), gol("gol_small", 24, 20, 80, 102),
             gol("gol_conway", 20, 24, 60, 103, conway_init=True),
@@ -402,4 +431,5 @@ def all_cases():
             iz_fc(), iz_sheet(),
             iz_sheet(24, 32, 80, "iz_sheet_hot", noise=(2.0, 5.0),
                      workload=107),
-            iz_ops(), rate(), iz_init(), history()] + ragged_cases()
+            iz_ops(), rate(), iz_init(), history(), gol_rule(),
+            relay_ramp()] + ragged_cases()

@@ -81,6 +81,11 @@ PLANTED = [
     ("rate encoding passes negative inputs",
      "float value = (input > 0.0) ? tanhf(input) : 0.0;",
      "float value = tanhf(input);", "rate"),
+    ("relay ramp ignored", "float value = ramp ? fmax(0.0, input) : input;",
+     "float value = input;", "relay_ramp"),
+    ("birth range collapsed to its lower end",
+     ": (input >= birth_min && input <= birth_max);",
+     ": (input == birth_min);", "gol_rule"),
     ("Izhikevich settle loop stops on either variable",
      "} while (fabsf(delta_v) > 0.001 && fabsf(delta_r) > 0.001);",
      "} while (fabsf(delta_v) > 0.001 || fabsf(delta_r) > 0.001);",
