@@ -10,9 +10,12 @@
  *
  * Each entry point replaces one family of the reference's per-timestep
  * kernels; the reference file:line it restates is cited beside it.  Arrays
- * called "device layout" hold per-synapse data k-major ([k][destination]),
- * the transpose of the reference's host layout ([destination][k]), so one
- * lane per destination reads consecutive addresses.
+ * called "device layout" hold per-synapse data in 64-destination tiles:
+ * element (slot s, visit k) of a connection with K visits per slot lives at
+ * ((s / 64) * K + k) * 64 + s % 64, so the 64 lanes of a wavefront (one
+ * destination each) read 256 consecutive bytes per visit (see
+ * sg_hip_tile32).  Convolutional kernels stay a plain [k] array; one-to-one
+ * connections keep one value per destination.
  */
 #ifndef SG_HIP_H
 #define SG_HIP_H
