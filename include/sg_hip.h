@@ -68,6 +68,12 @@ typedef struct sg_conn {
     float *dst;
     /* weights, device layout */
     const float *weights;
+    /* Scratch for a BIT source's spike plane (ceil(from_size / 64) words),
+     * or NULL.  With skip_silent set, a tiled connection type and at most
+     * 1024 visits per destination, sg_hip_activate_base packs the bit each
+     * source shows this connection into it (64 sources per word) and sums
+     * only the spiking visits, in visit order. */
+    uint64_t *plane;
 } sg_conn;
 
 /* Fields of the interleaved per-synapse record of a tiled Izhikevich

@@ -352,6 +352,8 @@ __global__ __launch_bounds__(kBlock) void iz_gap_kernel(sg_conn c,
     }
 
 hipError_t launch_tile_base(const sg_conn &c, hipStream_t st);
+bool plane_path(const sg_conn &c);
+hipError_t launch_plane_base(const sg_conn &c, hipStream_t st);
 hipError_t launch_iz_tile(const sg_conn &c, const sg_iz_syn &s,
                           const sg_iz_dest &d, int kind, int sub, int upd,
                           int act, const float *st_, const float *dt_,
@@ -369,6 +371,7 @@ using namespace sg;
 extern "C" int sg_hip_activate_base(const sg_conn *c, void *stream) {
     if (c->num_dest <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
+    if (plane_path(*c)) return launch_plane_base(*c, s);
     if (tiled_type(c->type)) return launch_tile_base(*c, s);
     switch (c->type) {
         case SG_FULLY_CONNECTED: return launch_base_t<SG_FULLY_CONNECTED>(*c, s);

@@ -2,15 +2,17 @@
 // SUBSET, CONVERGENT, DIVERGENT): the base activator and the three
 // Izhikevich synapse passes (update, activate, fused update+activate).
 //
-// A 256-lane block owns a tile of 64 destinations.  Its four wavefronts
-// split the tile's synapse list k-wise in chunks of 32 (8 visits per
-// wavefront per chunk): every synapse's independent work -- loads, STDP /
-// short-term plasticity, trace decay, stores -- runs with four wavefronts in
-// flight per tile instead of one.  The only order-dependent part, the FP32
-// sum over a destination's synapses, is taken out of the per-synapse pass:
-// each synapse writes the value it adds to a double-buffered LDS chunk, and
-// after a barrier one wavefront per sum adds the chunk's 32 values in k
-// order.  The result is the reference's sequential sum, bit for bit.
+// A block owns a tile of 64 destinations.  Its W wavefronts (W = 3 or 4
+// for the Izhikevich passes, chosen per launch so that the whole grid is
+// resident at once where possible) split the tile's synapse list k-wise in
+// chunks of 8 * W (8 visits per wavefront per chunk): every synapse's
+// independent work -- loads, STDP / short-term plasticity, trace decay,
+// stores -- runs with W wavefronts in flight per tile instead of one.  The
+// only order-dependent part, the FP32 sum over a destination's synapses, is
+// taken out of the per-synapse pass: each synapse writes the value it adds
+// to a double-buffered LDS chunk, and after a barrier one wavefront per sum
+// adds the chunk's values in k order.  The result is the reference's
+// sequential sum, bit for bit.
 //
 // Per-synapse arrays are in the 64-destination tiled layout (tiled_index),
 // so each wavefront load is one contiguous 256-byte line and each tile
@@ -21,30 +23,12 @@
 
 namespace sg {
 
-constexpr int kTileWaves = 4;
-constexpr int kTileThreads = 64 * kTileWaves;
-constexpr int kChunk = 32;
-constexpr int kPerWave = kChunk / kTileWaves;
+constexpr int kPerWave = 8;      // visits per wavefront per chunk
 constexpr int kSub = 8;
 
 template <int TYPE>
 __device__ __forceinline__ int visit_count(const sg_conn &c) {
     return cursor<TYPE>(c, 0).count;
-}
-
-// Adds this chunk's staged values in k order (wavefront `which` of the
-// block, one lane per destination).  `walk` tracks which visits exist.
-template <int TYPE>
-__device__ __forceinline__ void ordered_sum(const float (*vals)[64],
-                                            SynapseIter<TYPE> &walk, int k0,
-                                            int count, int lane,
-                                            float &sum) {
-    const int n = min(kChunk, count - k0);
-    for (int j = 0; j < n; ++j) {
-        const bool valid = walk.source() >= 0;
-        walk.advance();
-        if (valid) sum += vals[j][lane];
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -165,13 +149,14 @@ __device__ __forceinline__ void tile_update(const sg_iz_syn &s, float dest_spike
     }
 }
 
-template <int TYPE, int KIND, int SUB, int UPD, int ACT>
-__global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
+template <int TYPE, int KIND, int SUB, int UPD, int ACT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
         sg_conn c, sg_iz_syn s, sg_iz_dest d, const float *src_table,
         const float *dst_table) {
     constexpr bool kLong = KIND == SG_IZ_ADD || KIND == SG_IZ_SUB;
     constexpr int F = SG_IZ_FIELDS;
-    __shared__ float vals[2][2][kChunk][64];
+    constexpr int CHUNK = WAVES * kPerWave;
+    __shared__ float vals[2][2][CHUNK][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = blockIdx.x * 64 + lane;
     const bool live = slot < c.num_dest;
@@ -188,7 +173,7 @@ __global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
     SynapseIter<TYPE> walk;
     walk.init(c, live ? slot : 0);
     float sum = 0.0f;
-    for (int k0 = 0, chunk = 0; k0 < count; k0 += kChunk, ++chunk) {
+    for (int k0 = 0, chunk = 0; k0 < count; k0 += CHUNK, ++chunk) {
         const int buf = chunk & 1;
         const int kw = k0 + wave * kPerWave;
         if (live && kw < count) {
@@ -260,7 +245,8 @@ __global__ __launch_bounds__(kTileThreads) void iz_tile_kernel(
         if (ACT) {
             __syncthreads();
             if (live && (wave == 0 || (kLong && wave == 1)))
-                ordered_sum(vals[buf][wave], walk, k0, count, lane, sum);
+                ordered_sum_n(vals[buf][wave], walk, min(CHUNK, count - k0),
+                              lane, sum);
         }
     }
     if (ACT && live) {
@@ -307,15 +293,56 @@ hipError_t launch_tile_base(const sg_conn &c, hipStream_t st) {
     return launch_tile_base_t<4, 32>(c, st);
 }
 
+// Number of compute units of the current device (cached per device).
+static int cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount,
+                                  dev) != hipSuccess || n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+// Wavefronts per tile for a fused Izhikevich launch: the count that lets
+// the whole grid be resident in the fewest rounds of blocks (a grid that
+// overflows the GPU by a fraction runs its last round mostly idle), ties
+// going to more wavefronts per tile.
+template <class KA, class KB>
+static int pick_waves(KA *four, KB *three, int tiles) {
+    int nb4 = 0, nb3 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb4, reinterpret_cast<const void *>(four), 256, 0) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb3, reinterpret_cast<const void *>(three), 192, 0) != hipSuccess ||
+        nb4 <= 0 || nb3 <= 0)
+        return 4;
+    const long cus = cu_count();
+    const long r4 = (tiles + nb4 * cus - 1) / (nb4 * cus);
+    const long r3 = (tiles + nb3 * cus - 1) / (nb3 * cus);
+    return r3 < r4 ? 3 : 4;
+}
+
 template <int UPD, int ACT>
 static hipError_t launch_iz_tile_t(const sg_conn &c, const sg_iz_syn &s,
                                    const sg_iz_dest &d, int kind, int sub,
                                    const float *st_, const float *dt_,
                                    hipStream_t st) {
-    dim3 g((c.num_dest + 63) / 64), b(kTileThreads);
+    const int tiles = (c.num_dest + 63) / 64;
+    dim3 g(tiles);
 #define SG_IZ(T, K, S)                                                       \
-    hipLaunchKernelGGL((iz_tile_kernel<T, K, S, UPD, ACT>), g, b, 0, st, c,  \
-                       s, d, st_, dt_)
+    do {                                                                     \
+        auto *k4 = iz_tile_kernel<T, K, S, UPD, ACT, 4>;                     \
+        auto *k3 = iz_tile_kernel<T, K, S, UPD, ACT, 3>;                     \
+        if (pick_waves(k4, k3, tiles) == 3)                                  \
+            hipLaunchKernelGGL(k3, g, dim3(192), 0, st, c, s, d, st_, dt_);  \
+        else                                                                 \
+            hipLaunchKernelGGL(k4, g, dim3(256), 0, st, c, s, d, st_, dt_);  \
+    } while (0)
 #define SG_IZK(T, S)                                                         \
     switch (kind) {                                                          \
         case SG_IZ_ADD: SG_IZ(T, SG_IZ_ADD, S); break;                       \

@@ -302,6 +302,7 @@ Props *Engine::run(const Props &args) {
         // Plastic base-activator weights stay finite: the updater clamps
         // them to [1e-4, max weight].
         d.skip_silent = d.out_type == SG_OUT_BIT && cs.finite_weights;
+        d.plane = cs.d_plane;
         return d;
     };
     auto iz_syn = [&](ConnState &cs) {

@@ -96,6 +96,7 @@ State::~State() {
         free_dev(cs->d_visits);
         free_dev(cs->d_tss[0]);
         free_dev(cs->d_tss[1]);
+        free_dev(cs->d_plane);
     }
     free_dev(scratch_);
     free_dev(d_tables_);
@@ -709,6 +710,13 @@ static void set_conn_layout(ConnState &cs, Array &a) {
 void State::upload_conn_derived(ConnState &cs) {
     Connection &c = *cs.conn;
     const bool conductance = cs.iz && !cs.direct && cs.iz_kind != SG_IZ_GAP;
+    // Base activators reading a BIT source get a spike-plane scratch buffer
+    // (one bit per source neuron).
+    if ((!cs.iz || cs.direct) && cs.from->out_type == SG_OUT_BIT &&
+        !cs.d_plane)
+        check_hip(hipMalloc((void **)&cs.d_plane,
+                            (size_t)(c.from->size + 63) / 64 * 8),
+                  "hipMalloc plane");
     if (!cs.iz) return;
     const int from_n = c.from->size;
     Array &tss = cs.arr("time since spike");

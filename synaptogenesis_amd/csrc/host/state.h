@@ -80,6 +80,7 @@ struct ConnState {
     uint8_t *d_rev_delays = nullptr;  // [source][32] last visits, reversed
     uint8_t *d_visits = nullptr;      // min(32, visits per source)
     int32_t *d_tss[2] = {nullptr, nullptr};
+    uint64_t *d_plane = nullptr;      // spike plane of a BIT source (base)
     int tss_cur = 0;
     bool pending_update = false;
     bool fused_gol = false;   // handled by sg_hip_gol_step
