@@ -152,7 +152,7 @@ struct SynapseCursor {
 };
 
 template <int TYPE>
-__device__ __forceinline__ SynapseCursor cursor(const sg_conn &c, int slot) {
+__host__ __device__ inline SynapseCursor cursor(const sg_conn &c, int slot) {
     SynapseCursor q;
     if (TYPE == SG_FULLY_CONNECTED) {
         q.count = c.from_size;

@@ -11,28 +11,27 @@
 // 1. pack_plane_kernel turns the source's history words into a plane: the
 //    bit each source neuron shows this connection (word delay/32, bit
 //    31 - delay%32), 64 neurons per uint64, one wavefront ballot each.
-// 2. plane_base_kernel: a 256-lane block owns a tile of 64 destinations.
-//    It stages the plane in LDS; its four wavefronts split the tile's visit
-//    list into mask words and record, per destination, which visits spike
-//    (order-free work); then wavefront 0 walks each destination's spiking
-//    visits in k order, loading their weights eight at a time, and adds
-//    them in order.
+// 2. plane_base_kernel: a block owns a tile of 64 destinations (one lane
+//    each) and stages the plane in LDS.  Its W wavefronts split the visit
+//    list into windows of 16 * W visits: each lane tests its 16 visits' bits
+//    in LDS and issues the weight loads of the spiking ones together
+//    (coalesced across the tile), writing the weight -- or +0 for a silent
+//    or skipped visit -- to a double-buffered LDS window.  After a barrier
+//    wavefront 0 adds the window in visit order while the others fetch the
+//    next one.  Adding +0 leaves the sum unchanged, so the result is the
+//    reference's sum bit for bit.
 //
 // Work per synapse is a bit test in LDS; weights are read only for the
-// spiking visits.  Planes larger than kPlaneMax words stay in global memory
-// (read through the caches); destinations with more than kMaxVisits visits
-// use the tile kernels.
+// spiking visits.  Planes larger than kPlaneMax words are read from global
+// memory (through the caches) instead of LDS.
 #include <hip/hip_runtime.h>
 
 #include "sg_device.h"
 
 namespace sg {
 
-constexpr int kPlaneThreads = 256;
-constexpr int kPlaneMax = 4096;           // uint64 words staged in LDS
-constexpr int kMaxVisits = 1024;
-constexpr int kMaskWords = kMaxVisits / 32;
-constexpr int kBatch = 8;
+constexpr int kPlaneMax = 4096;     // uint64 plane words staged in LDS
+constexpr int kVisitsPerWave = 16;  // visits per wavefront per window
 
 __global__ __launch_bounds__(256) void pack_plane_kernel(
         const uint32_t *words, int n, int delay, uint64_t *plane) {
@@ -48,102 +47,88 @@ __device__ __forceinline__ int plane_visits(const sg_conn &c) {
     return cursor<TYPE>(c, 0).count;
 }
 
-template <int TYPE, bool STAGED>
-__global__ __launch_bounds__(kPlaneThreads) void plane_base_kernel(
+template <int TYPE, int W, bool STAGED>
+__global__ __launch_bounds__(64 * W) void plane_base_kernel(
         sg_conn c, const uint64_t *plane) {
-    __shared__ uint64_t splane[STAGED ? kPlaneMax : 1];
-    __shared__ uint32_t masks[kMaskWords][64];
+    constexpr int WIN = kVisitsPerWave * W;
+    extern __shared__ uint64_t splane[];          // STAGED: the plane
+    __shared__ float vals[2][WIN][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int slot = blockIdx.x * 64 + lane;
     const bool live = slot < c.num_dest;
-    // count <= k_per_dest <= kMaxVisits (checked by plane_path); the clamp
-    // only keeps LDS indexing in range by construction.
-    const int count = min(plane_visits<TYPE>(c), kMaxVisits);
-    const int nwords = (count + 31) >> 5;
+    const int count = plane_visits<TYPE>(c);
     if (STAGED) {
         const int np = (c.from_size + 63) >> 6;
-        for (int i = threadIdx.x; i < np; i += kPlaneThreads)
-            splane[i] = plane[i];
+        for (int i = threadIdx.x; i < np; i += 64 * W) splane[i] = plane[i];
     }
     __syncthreads();
-
-    // Pass 1: spiking visits of mask words [j0, j1) of every destination.
-    const int per = (nwords + 3) >> 2;
-    const int j0 = wave * per, j1 = min(nwords, j0 + per);
-    if (live && j0 < j1) {
-        SynapseIter<TYPE> it;
-        it.seek(c, slot, j0 * 32);
-        for (int j = j0; j < j1; ++j) {
-            uint32_t m = 0;
-            const int n = min(32, count - j * 32);
-            for (int b = 0; b < n; ++b) {
-                const int from = it.source();
-                it.advance();
-                if (from >= 0) {
-                    const uint64_t p = STAGED ? splane[from >> 6]
-                                              : plane[from >> 6];
-                    m |= (uint32_t)((p >> (from & 63)) & 1u) << b;
+    const uint64_t *pl = STAGED ? splane : plane;
+    const float *wbase = c.convolutional
+        ? c.weights
+        : c.weights + tiled_index(live ? slot : 0, 0, c.k_per_dest);
+    const long wstep = c.convolutional ? 1 : 64;
+    float sum = 0.0f;
+    for (int k0 = 0, win = 0; k0 < count; k0 += WIN, ++win) {
+        const int buf = win & 1;
+        const int kw = k0 + wave * kVisitsPerWave;
+        if (live && kw < count) {
+            SynapseIter<TYPE> it;
+            it.seek(c, slot, kw);
+            const int n = min(kVisitsPerWave, count - kw);
+            float v[kVisitsPerWave];
+#pragma unroll
+            for (int b = 0; b < kVisitsPerWave; ++b) {
+                v[b] = 0.0f;
+                if (b < n) {
+                    const int from = it.source();
+                    it.advance();
+                    if (from >= 0 && ((pl[from >> 6] >> (from & 63)) & 1u))
+                        v[b] = wbase[(kw + b) * wstep];
                 }
             }
-            masks[j][lane] = m;
+#pragma unroll
+            for (int b = 0; b < kVisitsPerWave; ++b)
+                vals[buf][wave * kVisitsPerWave + b][lane] = v[b];
+        }
+        __syncthreads();
+        if (live && wave == 0) {
+            const int n = min(WIN, count - k0);
+            for (int j = 0; j < n; ++j) sum += vals[buf][j][lane];
         }
     }
-    __syncthreads();
-    if (wave != 0) return;
-
-    // Pass 2: in-order sum over the spiking visits.
-    const float *wbase = c.convolutional
-        ? c.weights : c.weights + tiled_index(live ? slot : 0, 0, c.k_per_dest);
-    const int wstep = c.convolutional ? 1 : 64;
-    float sum = 0.0f;
-    int j = 0;
-    uint32_t m = (live && nwords > 0) ? masks[0][lane] : 0u;
-    bool more = live && nwords > 0;
-    while (__any(more)) {
-        int ks[kBatch];
-#pragma unroll
-        for (int b = 0; b < kBatch; ++b) {
-            while (m == 0u && j + 1 < nwords && more) m = masks[++j][lane];
-            if (m != 0u) {
-                ks[b] = (j << 5) + __builtin_ctz(m);
-                m &= m - 1u;
-            } else {
-                ks[b] = -1;
-            }
-        }
-        float w[kBatch];
-#pragma unroll
-        for (int b = 0; b < kBatch; ++b)
-            w[b] = ks[b] >= 0 ? wbase[(long)ks[b] * wstep] : 0.0f;
-#pragma unroll
-        for (int b = 0; b < kBatch; ++b)
-            if (ks[b] >= 0) sum += w[b];
-        more = more && ks[kBatch - 1] >= 0;
-    }
-    if (live) {
+    if (live && wave == 0) {
         const int to = slot_to_index(c, slot);
         c.dst[to] = aggregate(c.opcode, c.dst[to], sum);
     }
 }
 
-template <int TYPE>
-static hipError_t launch_plane_t(const sg_conn &c, const uint64_t *plane,
-                                 hipStream_t st) {
-    dim3 g((c.num_dest + 63) / 64), b(kPlaneThreads);
-    if ((c.from_size + 63) / 64 <= kPlaneMax)
-        hipLaunchKernelGGL((plane_base_kernel<TYPE, true>), g, b, 0, st, c,
-                           plane);
+template <int TYPE, int W>
+static hipError_t launch_plane_w(const sg_conn &c, hipStream_t st) {
+    dim3 g((c.num_dest + 63) / 64), b(64 * W);
+    const int np = (c.from_size + 63) / 64;
+    if (np <= kPlaneMax)
+        hipLaunchKernelGGL((plane_base_kernel<TYPE, W, true>), g, b,
+                           (size_t)np * 8, st, c, c.plane);
     else
-        hipLaunchKernelGGL((plane_base_kernel<TYPE, false>), g, b, 0, st, c,
-                           plane);
+        hipLaunchKernelGGL((plane_base_kernel<TYPE, W, false>), g, b, 0, st,
+                           c, c.plane);
     return hipGetLastError();
 }
 
-// True when the plane path applies: BIT source, finite weights, a tiled
-// connection type and at most kMaxVisits visits per destination.
+// One wavefront per tile when a destination makes at most one wavefront's
+// share of visits, four otherwise.
+template <int TYPE>
+static hipError_t launch_plane_t(const sg_conn &c, hipStream_t st) {
+    if (cursor<TYPE>(c, 0).count <= kVisitsPerWave)
+        return launch_plane_w<TYPE, 1>(c, st);
+    return launch_plane_w<TYPE, 4>(c, st);
+}
+
+// True when the plane path applies: a BIT source, finite weights and a
+// tiled connection type.
 bool plane_path(const sg_conn &c) {
     return c.plane != nullptr && c.skip_silent && c.out_type == SG_OUT_BIT &&
-           c.type != SG_ONE_TO_ONE && c.k_per_dest <= kMaxVisits;
+           c.type != SG_ONE_TO_ONE;
 }
 
 hipError_t launch_plane_base(const sg_conn &c, hipStream_t st) {
@@ -154,11 +139,10 @@ hipError_t launch_plane_base(const sg_conn &c, hipStream_t st) {
     if (e != hipSuccess) return e;
     switch (c.type) {
         case SG_FULLY_CONNECTED:
-            return launch_plane_t<SG_FULLY_CONNECTED>(c, c.plane, st);
-        case SG_SUBSET: return launch_plane_t<SG_SUBSET>(c, c.plane, st);
-        case SG_CONVERGENT:
-            return launch_plane_t<SG_CONVERGENT>(c, c.plane, st);
-        case SG_DIVERGENT: return launch_plane_t<SG_DIVERGENT>(c, c.plane, st);
+            return launch_plane_t<SG_FULLY_CONNECTED>(c, st);
+        case SG_SUBSET: return launch_plane_t<SG_SUBSET>(c, st);
+        case SG_CONVERGENT: return launch_plane_t<SG_CONVERGENT>(c, st);
+        case SG_DIVERGENT: return launch_plane_t<SG_DIVERGENT>(c, st);
     }
     return hipErrorInvalidValue;
 }

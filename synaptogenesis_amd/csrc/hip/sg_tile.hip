@@ -16,7 +16,11 @@
 //
 // Per-synapse arrays are in the 64-destination tiled layout (tiled_index),
 // so each wavefront load is one contiguous 256-byte line and each tile
-// streams one contiguous K * 256-byte block per array.
+// streams one contiguous K * 256-byte block per array.  The Izhikevich
+// record is read and written once per step and is far larger than the
+// caches, so its loads and stores are non-temporal: the stream does not
+// evict the small, reused state (neuron arrays, spike words, other
+// connections' weights) from L2 and the Infinity Cache.
 #include <hip/hip_runtime.h>
 
 #include "sg_device.h"
@@ -112,6 +116,14 @@ __global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
 // (izhikevich_attributes.cpp:444-508) of the previous step (fused) or of
 // this step (UPD only).  ACT: conductance activator
 // (izhikevich_attributes.cpp:257-284).
+// Non-temporal access to field f of the record lane q points at.
+__device__ __forceinline__ float rec_ld(const float *q, int f) {
+    return __builtin_nontemporal_load(&q[f * 64]);
+}
+__device__ __forceinline__ void rec_st(float v, float *q, int f) {
+    __builtin_nontemporal_store(v, &q[f * 64]);
+}
+
 template <int SUB>
 __device__ __forceinline__ void tile_update(const sg_iz_syn &s, float dest_spike,
                                             float dest_exc_trace, int dest_tss,
@@ -195,19 +207,19 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
                 if (it.k < count) it.advance();
                 if (from[b] < 0) continue;
                 const float *q = rec + (h + b) * F * 64;
-                w[b] = q[SG_IZ_W * 64];
-                r[b] = q[SG_IZ_R * 64];
-                sw[b] = q[SG_IZ_SW * 64];
+                w[b] = rec_ld(q, SG_IZ_W);
+                r[b] = rec_ld(q, SG_IZ_R);
+                sw[b] = rec_ld(q, SG_IZ_SW);
                 if (UPD) {
-                    dw[b] = q[SG_IZ_DW * 64];
-                    if (!SUB) pre[b] = q[SG_IZ_PRE * 64];
+                    dw[b] = rec_ld(q, SG_IZ_DW);
+                    if (!SUB) pre[b] = rec_ld(q, SG_IZ_PRE);
                     tss[b] = s.tss_cur[from[b]];
                 }
                 if (ACT) {
                     dl[b] = dly[(h + b) * 64];
                     x[b] = c.src[from[b]];
-                    st[b] = q[SG_IZ_ST * 64];
-                    if (kLong) lt[b] = q[SG_IZ_LT * 64];
+                    st[b] = rec_ld(q, SG_IZ_ST);
+                    if (kLong) lt[b] = rec_ld(q, SG_IZ_LT);
                 }
             }
 #pragma unroll
@@ -218,12 +230,12 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
                     tile_update<SUB>(s, dest_spike, dest_trace, dest_tss,
                                      src_table, dst_table, tss[b], w[b], r[b],
                                      sw[b], pre[b], dw[b]);
-                    q[SG_IZ_W * 64] = w[b];
-                    q[SG_IZ_DW * 64] = dw[b];
-                    if (!SUB) q[SG_IZ_PRE * 64] = pre[b];
+                    rec_st(w[b], q, SG_IZ_W);
+                    rec_st(dw[b], q, SG_IZ_DW);
+                    if (!SUB) rec_st(pre[b], q, SG_IZ_PRE);
                     if (s.stp_flag) {
-                        q[SG_IZ_R * 64] = r[b];
-                        q[SG_IZ_SW * 64] = sw[b];
+                        rec_st(r[b], q, SG_IZ_R);
+                        rec_st(sw[b], q, SG_IZ_SW);
                     }
                 }
                 if (ACT) {
@@ -232,11 +244,11 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
                     const float inc = spike * weight * s.baseline_conductance;
                     const float st1 = st[b] + inc;
                     vals[buf][0][wave * kPerWave + h + b][lane] = st1;
-                    q[SG_IZ_ST * 64] = st1 * d.short_tau;
+                    rec_st(st1 * d.short_tau, q, SG_IZ_ST);
                     if (kLong) {
                         const float lt1 = lt[b] + inc;
                         vals[buf][1][wave * kPerWave + h + b][lane] = lt1;
-                        q[SG_IZ_LT * 64] = lt1 * d.long_tau;
+                        rec_st(lt1 * d.long_tau, q, SG_IZ_LT);
                     }
                 }
             }
