@@ -69,11 +69,18 @@ typedef struct sg_conn {
     /* weights, device layout */
     const float *weights;
     /* Scratch for a BIT source's spike plane (ceil(from_size / 64) words),
-     * or NULL.  With skip_silent set, a tiled connection type and at most
-     * 1024 visits per destination, sg_hip_activate_base packs the bit each
-     * source shows this connection into it (64 sources per word) and sums
-     * only the spiking visits, in visit order. */
+     * or NULL.  With skip_silent set and a tiled connection type,
+     * sg_hip_activate_base packs the bit each source shows this connection
+     * into it (64 sources per word) and loads the weights of the spiking
+     * visits only, summing in visit order. */
     uint64_t *plane;
+    /* The plane already holds this step's bits (written by the source
+     * layer's neuron kernel): skip packing it. */
+    int plane_ready;
+    /* Store each destination's sum itself into dst instead of aggregating
+     * it into the input; the neuron kernel then aggregates it into the
+     * layer's init-config value (sg_iz_layer.noise == 2). */
+    int store_sum;
 } sg_conn;
 
 /* Fields of the interleaved per-synapse record of a tiled Izhikevich
@@ -119,6 +126,20 @@ typedef struct sg_iz_dest {
 enum { SG_IZ_ADD = 0, SG_IZ_SUB = 1, SG_IZ_MULT = 2, SG_IZ_REWARD = 3,
        SG_IZ_GAP = 4 };
 
+/* Per-source time since spike of one conductance connection reading this
+ * layer, stepped by the layer's neuron kernel from its history words before
+ * the shift (the words that step's activation read); same rule and arrays
+ * as sg_hip_tss_step. */
+typedef struct sg_tss_out {
+    int32_t *next;
+    const int32_t *cur;
+    const uint8_t *visits;
+    const uint8_t *rev_delays;
+    int word;                     /* history word read: delay / 32 */
+} sg_tss_out;
+
+enum { SG_MAX_TSS_OUT = 2 };
+
 /* One Izhikevich layer (izhikevich_attributes.h:6-43). */
 typedef struct sg_iz_layer {
     int size, history;
@@ -128,6 +149,25 @@ typedef struct sg_iz_layer {
     float *voltage, *recovery, *post_trace;
     int32_t *time_since_spike;
     const float *a, *b, *c, *d;
+    /* Optional: plane of the new spike bits (ceil(size / 64) words), for
+     * base activators that read this layer at delay 0 next step. */
+    uint64_t *plane;
+    /* Optional: time-since-spike steps of outgoing conductance
+     * connections. */
+    int ntss;
+    sg_tss_out tss[SG_MAX_TSS_OUT];
+    /* Init-config value drawn here instead of by sg_hip_init_random
+     * (instruction.cpp:3-30 order: the value is written first, then the
+     * activators aggregate into it): noise 0 = none; 1 = the input is the
+     * drawn value (no base activator writes it); 2 = the input holds the
+     * sum of the one base activator that covers every neuron, stored raw
+     * (sg_conn.store_sum), and the input becomes
+     * aggregate(noise_op, value, sum) -- the reference's own operation.
+     * The result is written back to input.  dist: 0 uniform[a, b),
+     * 1 normal(a, b), 2 poisson (val with probability a), 3 flat a. */
+    int noise, noise_dist, noise_op;
+    float noise_a, noise_b, noise_val;
+    uint64_t noise_seed, noise_step;
 } sg_iz_layer;
 
 /* Base activator: every connection type, extractor and aggregator
@@ -171,6 +211,15 @@ int sg_hip_iz_update_activate(const sg_conn *c, const sg_iz_syn *s,
  * relay_attributes.cpp:10-18) including the history shift
  * (state/attributes.h:316-333). */
 int sg_hip_iz_neuron(const sg_iz_layer *l, void *stream);
+/* Every layer of a list in one launch (up to SG_MAX_IZ_BATCH layers per
+ * launch; longer lists are split). */
+enum { SG_MAX_IZ_BATCH = 8 };
+int sg_hip_iz_neurons(const sg_iz_layer *layers, int n, void *stream);
+
+/* Spike plane of a BIT layer for one delay: bit i of plane[i / 64] is bit
+ * 31 - delay % 32 of words[i] (words = history word delay / 32). */
+int sg_hip_pack_plane(const uint32_t *words, int n, int delay,
+                      uint64_t *plane, void *stream);
 int sg_hip_gol_neuron(float *input, uint32_t *output, int size, int history,
                       int survival_min, int survival_max, int birth_min,
                       int birth_max, void *stream);

@@ -41,6 +41,37 @@ __device__ __forceinline__ float aggregate(int op, float prior, float in) {
     }
 }
 
+// Writes a destination's sum: aggregated into its input, or stored raw for
+// the neuron kernel to aggregate into an init-config value (store_sum).
+__device__ __forceinline__ void put_sum(const sg_conn &c, int to, float sum) {
+    c.dst[to] = c.store_sum ? sum : aggregate(c.opcode, c.dst[to], sum);
+}
+
+// Time since spike of one source after one activation of an Izhikevich
+// conductance connection (izhikevich_attributes.cpp:258-262 applied once per
+// visit in serial order): with m = min(32, visits) and rev[i] the delay of
+// the i-th visit counted back from the last, the result is the first i
+// whose delayed bit of `word` is set, else min(32, cur + m).
+__device__ __forceinline__ int tss_next(int cur, int m, const uint8_t *rev,
+                                        uint32_t word) {
+    int t = min(32, cur + m);
+    if (word != 0u) {
+        const uint32_t *rd = (const uint32_t *)rev;
+        for (int q = 0; q < 8 && q * 4 < m; ++q) {
+            const uint32_t four = rd[q];
+            int hit = -1;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = q * 4 + b;
+                const int d = (four >> (8 * b)) & 31;
+                if (hit < 0 && i < m && ((word << d) >> 31)) hit = i;
+            }
+            if (hit >= 0) { t = hit; break; }
+        }
+    }
+    return t;
+}
+
 // Destination index of slot `slot` (SUBSET slots cover the to-rectangle).
 __device__ __forceinline__ int slot_to_index(const sg_conn &c, int slot) {
     if (c.type != SG_SUBSET) return slot;

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 
+#include "sg_device.h"
 #include "sg_hip.h"
 
 namespace sg {
@@ -45,69 +46,153 @@ __device__ __forceinline__ void shift_floats(float *out, int size, int nid,
     out[nid] = v;
 }
 
-// iz_attribute_kernel (state/impl/izhikevich_attributes.cpp:102-213).
-__global__ __launch_bounds__(kNeuronBlock) void iz_neuron_kernel(
-        sg_iz_layer l) {
-    const int nid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (nid >= l.size) return;
-
-    const float ampa = l.ampa[nid];
-    const float nmda = l.nmda[nid];
-    const float gabaa = l.gabaa[nid];
-    const float gabab = l.gabab[nid];
-    const float mult = l.mult[nid];
-    float voltage = l.voltage[nid];
-    float recovery = l.recovery[nid];
-    const float base_current = l.input[nid];
-    const float a = l.a[nid];
-    const float b = l.b[nid];
-
-    for (int i = 0; i < 10 && (double)voltage < 30.0; ++i) {
-        float current = -ampa * voltage;
-        // powf(x, 2) is folded to x * x by the reference's compiler.
-        const float x = (voltage + 80.0f) / 60.0f;
-        const float temp = x * x;
-        current = current - nmda * (temp / (1.0f + temp)) * voltage;
-        current = current - gabaa * (voltage + 70.0f);
-        current = current - gabab * (voltage + 90.0f);
-        current = current * (1.0f + mult);
-        current = current + base_current;
-
-        const double dv = (0.04 * (double)voltage) * (double)voltage +
-                          (double)(5.0f * voltage) + 140.0 -
-                          (double)recovery + (double)current;
-        const float delta_v = (float)dv;
-
-        const float sum_g = ampa + nmda + gabaa + gabab;
-        const float drive = delta_v + (gabaa * -70.0f) + (gabab * -90.0f);
-        voltage = (float)(((double)voltage + 0.1 * (double)drive) /
-                          (1.0 + 0.1 * (double)sum_g));
-        voltage = (voltage != voltage) ? 30.0f : voltage;
-
-        const float adjusted_tau = ((double)voltage > 30.0)
-            ? (float)((double)delta_v /
-                      (30.0 - (double)voltage + (double)delta_v) * 0.1)
-            : (float)0.1;
-        recovery = recovery + a * adjusted_tau * ((b * voltage) - recovery);
+// Philox4x32-10 (Salmon et al., SC'11) for the random init configs.
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0,
+                                       uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
     }
+}
 
-    l.ampa[nid] = 0.0f;
-    l.nmda[nid] = 0.0f;
-    l.gabaa[nid] = 0.0f;
-    l.gabab[nid] = 0.0f;
-    l.mult[nid] = 0.0f;
-    l.dopamine[nid] = (float)((double)l.dopamine[nid] * 0.95);
+// Value of one init config for neuron i at `step` (engine/kernel/
+// kernel.cpp:43-158 distributions, drawn from a counter-based Philox keyed
+// by (seed, step, i) so runs are reproducible): dist 0 uniform[a, b),
+// 1 normal(a, b) by Box-Muller, 2 poisson (val with probability a, else 0),
+// 3 flat a.
+__device__ __forceinline__ float init_value(int dist, float a, float b,
+                                            float val, uint64_t seed,
+                                            uint64_t step, int i,
+                                            bool &fired) {
+    fired = true;
+    if (dist == 3) return a;
+    uint32_t c[4] = {(uint32_t)step, (uint32_t)i, (uint32_t)(step >> 32), 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float u = (float)((double)(c[0] >> 8) * (1.0 / 16777216.0));
+    if (dist == 0) return u * (b - a) + a;
+    if (dist == 1) {
+        const double u1 = ((double)(c[0] >> 8) + 1.0) * (1.0 / 16777216.0);
+        const double u2 = (double)(c[1] >> 8) * (1.0 / 16777216.0);
+        const float z =
+            (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+        return z * b + a;
+    }
+    fired = u < a;
+    return fired ? val : 0.0f;
+}
 
-    const uint32_t spike = (double)voltage >= 30.0 ? 1u : 0u;
-    const bool prev_bit = shift_bits(l.output, l.size, nid, l.history, spike);
+// iz_attribute_kernel (state/impl/izhikevich_attributes.cpp:102-213) for
+// neuron nid of layer l, plus the optional side outputs of sg_iz_layer:
+// the init-config value, the outgoing connections' time since spike (from
+// the words before the shift) and the plane of the new spike bits.  Every
+// lane of the wavefront calls it (the plane is a wavefront ballot);
+// nid >= l.size only takes part in the ballot.
+__device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
+    const bool valid = nid < l.size;
+    uint32_t spike = 0u;
+    if (valid) {
+        const float ampa = l.ampa[nid];
+        const float nmda = l.nmda[nid];
+        const float gabaa = l.gabaa[nid];
+        const float gabab = l.gabab[nid];
+        const float mult = l.mult[nid];
+        float voltage = l.voltage[nid];
+        float recovery = l.recovery[nid];
+        float base_current = l.input[nid];
+        if (l.noise) {
+            bool fired;
+            const float v = init_value(l.noise_dist, l.noise_a, l.noise_b,
+                                       l.noise_val, l.noise_seed,
+                                       l.noise_step, nid, fired);
+            base_current = l.noise == 1
+                ? v : aggregate(l.noise_op, v, base_current);
+            l.input[nid] = base_current;
+        }
+        const float a = l.a[nid];
+        const float b = l.b[nid];
 
-    l.post_trace[nid] = prev_bit
-        ? (float)0.004
-        : (float)((double)l.post_trace[nid] * 0.95);
-    l.time_since_spike[nid] = prev_bit
-        ? 0 : min(32, l.time_since_spike[nid] + 1);
-    l.voltage[nid] = spike ? l.c[nid] : voltage;
-    l.recovery[nid] = recovery + ((float)spike * l.d[nid]);
+        for (int i = 0; i < 10 && (double)voltage < 30.0; ++i) {
+            float current = -ampa * voltage;
+            // powf(x, 2) is folded to x * x by the reference's compiler.
+            const float x = (voltage + 80.0f) / 60.0f;
+            const float temp = x * x;
+            current = current - nmda * (temp / (1.0f + temp)) * voltage;
+            current = current - gabaa * (voltage + 70.0f);
+            current = current - gabab * (voltage + 90.0f);
+            current = current * (1.0f + mult);
+            current = current + base_current;
+
+            const double dv = (0.04 * (double)voltage) * (double)voltage +
+                              (double)(5.0f * voltage) + 140.0 -
+                              (double)recovery + (double)current;
+            const float delta_v = (float)dv;
+
+            const float sum_g = ampa + nmda + gabaa + gabab;
+            const float drive = delta_v + (gabaa * -70.0f) + (gabab * -90.0f);
+            voltage = (float)(((double)voltage + 0.1 * (double)drive) /
+                              (1.0 + 0.1 * (double)sum_g));
+            voltage = (voltage != voltage) ? 30.0f : voltage;
+
+            const float adjusted_tau = ((double)voltage > 30.0)
+                ? (float)((double)delta_v /
+                          (30.0 - (double)voltage + (double)delta_v) * 0.1)
+                : (float)0.1;
+            recovery = recovery + a * adjusted_tau * ((b * voltage) - recovery);
+        }
+
+        l.ampa[nid] = 0.0f;
+        l.nmda[nid] = 0.0f;
+        l.gabaa[nid] = 0.0f;
+        l.gabab[nid] = 0.0f;
+        l.mult[nid] = 0.0f;
+        l.dopamine[nid] = (float)((double)l.dopamine[nid] * 0.95);
+
+        for (int t = 0; t < l.ntss; ++t) {
+            const sg_tss_out &o = l.tss[t];
+            o.next[nid] = tss_next(o.cur[nid], o.visits[nid],
+                                   o.rev_delays + (long)nid * 32,
+                                   l.output[(long)o.word * l.size + nid]);
+        }
+
+        spike = (double)voltage >= 30.0 ? 1u : 0u;
+        const bool prev_bit =
+            shift_bits(l.output, l.size, nid, l.history, spike);
+
+        l.post_trace[nid] = prev_bit
+            ? (float)0.004
+            : (float)((double)l.post_trace[nid] * 0.95);
+        l.time_since_spike[nid] = prev_bit
+            ? 0 : min(32, l.time_since_spike[nid] + 1);
+        l.voltage[nid] = spike ? l.c[nid] : voltage;
+        l.recovery[nid] = recovery + ((float)spike * l.d[nid]);
+    }
+    if (l.plane) {
+        const uint64_t m = __ballot(spike != 0u);
+        // lane 0 of a wavefront: nid is a multiple of 64
+        if ((threadIdx.x & 63) == 0 && valid) l.plane[nid >> 6] = m;
+    }
+}
+
+// Up to SG_MAX_IZ_BATCH layers in one grid: layer j owns blocks
+// first[j] .. first[j + 1] - 1.
+struct IzBatch {
+    int n;
+    int first[SG_MAX_IZ_BATCH + 1];
+    sg_iz_layer l[SG_MAX_IZ_BATCH];
+};
+
+__global__ __launch_bounds__(kNeuronBlock) void iz_neuron_kernel(IzBatch b) {
+    int j = 0;
+    while (j + 1 < b.n && (int)blockIdx.x >= b.first[j + 1]) ++j;
+    iz_step(b.l[j], (blockIdx.x - b.first[j]) * kNeuronBlock + threadIdx.x);
 }
 
 // gol_attribute_kernel (state/impl/game_of_life_attributes.cpp:14-33):
@@ -152,44 +237,17 @@ __global__ __launch_bounds__(kNeuronBlock) void set_f32_kernel(
     if (i < n) p[i] = overwrite ? val : p[i] + val;
 }
 
-// Philox4x32-10 (Salmon et al., SC'11) for the random init configs.
-__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0,
-                                       uint32_t k1) {
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
-        c[1] = (uint32_t)p1;
-        c[3] = (uint32_t)p0;
-        c[0] = n0;
-        c[2] = n2;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
 __global__ __launch_bounds__(kNeuronBlock) void init_random_kernel(
         float *p, int n, int dist, float a, float b, float val, int overwrite,
         uint64_t seed, uint64_t step) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t c[4] = {(uint32_t)step, (uint32_t)i, (uint32_t)(step >> 32), 0u};
-    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    const double u1 = ((double)(c[0] >> 8) + 1.0) * (1.0 / 16777216.0);
-    const double u2 = (double)(c[1] >> 8) * (1.0 / 16777216.0);
-    const float u = (float)((double)(c[0] >> 8) * (1.0 / 16777216.0));
-    if (dist == 0) {
-        const float v = u * (b - a) + a;
-        p[i] = overwrite ? v : p[i] + v;
-    } else if (dist == 1) {
-        const float z = (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
-        const float v = z * b + a;
-        p[i] = overwrite ? v : p[i] + v;
+    bool fired;
+    const float v = init_value(dist, a, b, val, seed, step, i, fired);
+    if (dist == 2 && !overwrite) {
+        if (fired) p[i] += v;               // poisson adds only on a spike
     } else {
-        const bool fire = u < a;
-        if (overwrite) p[i] = fire ? val : 0.0f;
-        else if (fire) p[i] += val;
+        p[i] = overwrite ? v : p[i] + v;
     }
 }
 
@@ -246,11 +304,33 @@ __global__ __launch_bounds__(kNeuronBlock) void copy32_kernel(
 
 using namespace sg;
 
+extern "C" int sg_hip_iz_neurons(const sg_iz_layer *layers, int n,
+                                 void *stream) {
+    for (int s0 = 0; s0 < n; s0 += SG_MAX_IZ_BATCH) {
+        IzBatch b;
+        b.n = 0;
+        int blocks = 0;
+        for (int j = s0; j < n && j < s0 + SG_MAX_IZ_BATCH; ++j) {
+            if (layers[j].size <= 0) continue;
+            if (layers[j].ntss < 0 || layers[j].ntss > SG_MAX_TSS_OUT)
+                return (int)hipErrorInvalidValue;
+            b.l[b.n] = layers[j];
+            b.first[b.n] = blocks;
+            blocks += nblocks(layers[j].size);
+            ++b.n;
+        }
+        if (b.n == 0) continue;
+        b.first[b.n] = blocks;
+        hipLaunchKernelGGL(iz_neuron_kernel, dim3(blocks), dim3(kNeuronBlock),
+                           0, (hipStream_t)stream, b);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
+    }
+    return 0;
+}
+
 extern "C" int sg_hip_iz_neuron(const sg_iz_layer *l, void *stream) {
-    if (l->size <= 0) return 0;
-    hipLaunchKernelGGL(iz_neuron_kernel, dim3(nblocks(l->size)),
-                       dim3(kNeuronBlock), 0, (hipStream_t)stream, *l);
-    return hipGetLastError();
+    return sg_hip_iz_neurons(l, 1, stream);
 }
 
 extern "C" int sg_hip_gol_neuron(float *input, uint32_t *output, int size,

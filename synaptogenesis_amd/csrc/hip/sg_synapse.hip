@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void activate_base_kernel(sg_conn c) {
         consume(wb, xb, fb);
     }
     const int to = slot_to_index(c, slot);
-    c.dst[to] = aggregate(c.opcode, c.dst[to], sum);
+    put_sum(c, to, sum);
 }
 
 template <int TYPE>
@@ -108,24 +108,7 @@ __global__ __launch_bounds__(kBlock) void tss_step_kernel(
         const uint8_t *rev_delays, const uint32_t *words, int n) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n) return;
-    const int m = visits[f];
-    const uint32_t word = words[f];
-    int t = min(32, cur[f] + m);
-    if (word != 0u) {
-        const uint32_t *rd = (const uint32_t *)(rev_delays + (long)f * 32);
-        for (int q = 0; q < 8 && q * 4 < m; ++q) {
-            const uint32_t four = rd[q];
-            int hit = -1;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int i = q * 4 + b;
-                const int d = (four >> (8 * b)) & 31;
-                if (hit < 0 && i < m && ((word << d) >> 31)) hit = i;
-            }
-            if (hit >= 0) { t = hit; break; }
-        }
-    }
-    next[f] = t;
+    next[f] = tss_next(cur[f], visits[f], rev_delays + (long)f * 32, words[f]);
 }
 
 // ---------------------------------------------------------------------------

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
     }
     if (live && wave == 0) {
         const int to = slot_to_index(c, slot);
-        c.dst[to] = aggregate(c.opcode, c.dst[to], sum);
+        put_sum(c, to, sum);
     }
 }
 

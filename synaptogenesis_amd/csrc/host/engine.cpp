@@ -264,6 +264,15 @@ Props *Engine::run(const Props &args) {
     kt.on = args.get_bool("kernel timing", false);
     kt.stream = stream;
 
+    // ---- work folded into the Izhikevich neuron launch ---------------------
+    // (filled before the main loop, read by the helpers below)
+    struct InitFold { int noise = 0, dist = 3, op = SG_ADD;
+                      float a = 0.0f, b = 0.0f, val = 0.0f; };
+    std::map<LayerState *, InitFold> init_fold;     // init value drawn there
+    std::set<ConnState *> store_sum;                // raw-sum activators
+    std::map<LayerState *, std::vector<ConnState *>> tss_out;
+    std::set<ConnState *> tss_folded;               // stepped there
+
     // ---- per-connection launch helpers -----------------------------------
     auto conn_desc = [&](ConnState &cs) {
         Connection &c = *cs.conn;
@@ -302,7 +311,17 @@ Props *Engine::run(const Props &args) {
         // Plastic base-activator weights stay finite: the updater clamps
         // them to [1e-4, max weight].
         d.skip_silent = d.out_type == SG_OUT_BIT && cs.finite_weights;
-        d.plane = cs.d_plane;
+        // Delay-0 reads of an Izhikevich layer use the plane its neuron
+        // kernel wrote last step (packed once at run start); other BIT
+        // sources are packed per activation into the connection's own.
+        if (cs.from->model == Model::IZHIKEVICH && c.delay == 0 &&
+            cs.from->d_plane) {
+            d.plane = cs.from->d_plane;
+            d.plane_ready = 1;
+        } else {
+            d.plane = cs.d_plane;
+        }
+        d.store_sum = store_sum.count(&cs) ? 1 : 0;
         return d;
     };
     auto iz_syn = [&](ConnState &cs) {
@@ -375,7 +394,7 @@ Props *Engine::run(const Props &args) {
             });
             return;
         }
-        kt.time("tss_step", [&] {
+        if (!tss_folded.count(&cs)) kt.time("tss_step", [&] {
             check_hip(sg_hip_tss_step(cs.d_tss[1 - cs.tss_cur],
                                       cs.d_tss[cs.tss_cur], cs.d_visits,
                                       cs.d_rev_delays, d.src,
@@ -397,7 +416,7 @@ Props *Engine::run(const Props &args) {
                           "iz activate");
             });
         }
-        cs.tss_cur = 1 - cs.tss_cur;
+        if (!tss_folded.count(&cs)) cs.tss_cur = 1 - cs.tss_cur;
     };
     auto update = [&](ConnState &cs) {
         sg_conn d = conn_desc(cs);
@@ -480,6 +499,75 @@ Props *Engine::run(const Props &args) {
         }
     };
 
+    // ---- folding plans ----------------------------------------------------
+    // Init configs (instruction.cpp:3-30) of non-input Izhikevich layers:
+    // the value -- 0 for a plain clear -- is drawn by the neuron kernel when
+    // no base activator writes the input (it is then the input), or when
+    // exactly one does and covers every neuron (it stores its raw sum and
+    // the neuron kernel applies the activator's own aggregate to the value
+    // and the sum, the reference's operation).  Gap junctions add to the
+    // input too, so their layers keep the separate pass.
+    for (auto &lsp : state_->layers()) {
+        LayerState *ls = lsp.get();
+        if (ls->model != Model::IZHIKEVICH || io_types.count(ls)) continue;
+        InitFold f;
+        if (Props *ic = ls->layer->config.get_child("init config")) {
+            const std::string type = ic->get("type", "normal");
+            if (type == "flat") {
+                f.dist = 3; f.a = ic->get_float("value", 1.0f);
+            } else if (type == "uniform") {
+                f.dist = 0; f.a = ic->get_float("min", 0.0f);
+                f.b = ic->get_float("max", 1.0f);
+            } else if (type == "normal") {
+                f.dist = 1; f.a = ic->get_float("mean", 1.0f);
+                f.b = ic->get_float("std dev", 0.1f);
+            } else if (type == "poisson") {
+                f.dist = 2;
+                f.a = (float)(0.001 * ic->get_float("rate", 1.0f));
+                f.val = ic->get_float("value", 20.0f);
+            } else {
+                continue;    // refused by init_layer with its message
+            }
+        }
+        std::vector<ConnState *> base;
+        bool gap = false;
+        for (auto *c : ls->layer->inputs) {
+            ConnState *cs = state_->conn_state(c);
+            if (!cs->iz || cs->direct) base.push_back(cs);
+            else if (cs->iz_kind == SG_IZ_GAP) gap = true;
+        }
+        if (gap) continue;
+        if (base.empty()) {
+            f.noise = 1;
+        } else if (base.size() == 1 && base[0]->conn->type != SG_SUBSET) {
+            f.noise = 2;
+            f.op = base[0]->conn->opcode;
+            store_sum.insert(base[0]);
+        } else {
+            continue;
+        }
+        init_fold[ls] = f;
+    }
+    // Time since spike of conductance connections from Izhikevich layers is
+    // stepped by the source layer's neuron kernel (same rule as
+    // sg_hip_tss_step, on the same pre-shift words, after the activation
+    // that reads the previous value).
+    for (auto &csp : state_->conns()) {
+        ConnState *cs = csp.get();
+        if (!cs->iz || cs->direct || cs->iz_kind < 0 ||
+            cs->iz_kind == SG_IZ_GAP || cs->from->model != Model::IZHIKEVICH)
+            continue;
+        auto &v = tss_out[cs->from];
+        if ((int)v.size() >= SG_MAX_TSS_OUT) continue;
+        v.push_back(cs);
+        tss_folded.insert(cs);
+    }
+    for (auto &lsp : state_->layers())
+        if (lsp->d_plane)
+            check_hip(sg_hip_pack_plane(lsp->output.du(), lsp->layer->size, 0,
+                                        lsp->d_plane, stream),
+                      "pack plane");
+
     // ---- main loop --------------------------------------------------------
     auto &layers = state_->layers();
     long done = 0;
@@ -490,7 +578,7 @@ Props *Engine::run(const Props &args) {
         for (auto &lsp : layers) {
             LayerState *ls = lsp.get();
             if (is_input(ls) || gol_fast.count(ls)) continue;
-            init_layer(ls, true, it);
+            if (!init_fold.count(ls)) init_layer(ls, true, it);
             for (auto *c : ls->layer->inputs) activate(*state_->conn_state(c));
         }
         // 2. environment input
@@ -547,17 +635,24 @@ Props *Engine::run(const Props &args) {
         for (auto &lsp : layers) {
             LayerState *ls = lsp.get();
             if (!is_input(ls)) continue;
-            init_layer(ls, false, it);
+            // Only an "input" key makes the layer's input register the
+            // environment's (cluster_node.cpp:20-44); auxiliary input keys
+            // leave it to the usual clear.
+            const auto ik = in_keys.find(ls);
+            init_layer(ls, ik == in_keys.end() || !ik->second.count("input"),
+                       it);
             for (auto *c : ls->layer->inputs) activate(*state_->conn_state(c));
         }
-        // 4. neuron updates
+        // 4. neuron updates: every Izhikevich layer in one launch
         const bool last = iterations != 0 && it + 1 == iterations;
+        std::vector<sg_iz_layer> iz_layers;
         for (auto &lsp : layers) {
             LayerState *ls = lsp.get();
             const int n = ls->layer->size;
             switch (ls->model) {
                 case Model::IZHIKEVICH: {
                     sg_iz_layer l;
+                    std::memset(&l, 0, sizeof(l));
                     l.size = n; l.history = ls->history;
                     l.input = ls->input.df(); l.output = ls->output.du();
                     l.ampa = ls->var("ampa").df(); l.nmda = ls->var("nmda").df();
@@ -571,9 +666,25 @@ Props *Engine::run(const Props &args) {
                     l.time_since_spike = ls->var("time since spike").di();
                     l.a = ls->var("a").df(); l.b = ls->var("b").df();
                     l.c = ls->var("c").df(); l.d = ls->var("d").df();
-                    kt.time("iz_neuron", [&] {
-                        check_hip(sg_hip_iz_neuron(&l, stream), "iz neuron");
-                    });
+                    l.plane = ls->d_plane;
+                    for (ConnState *cs : tss_out[ls]) {
+                        sg_tss_out &o = l.tss[l.ntss++];
+                        o.next = cs->d_tss[1 - cs->tss_cur];
+                        o.cur = cs->d_tss[cs->tss_cur];
+                        o.visits = cs->d_visits;
+                        o.rev_delays = cs->d_rev_delays;
+                        o.word = cs->conn->delay / 32;
+                    }
+                    auto fi = init_fold.find(ls);
+                    if (fi != init_fold.end()) {
+                        const InitFold &f = fi->second;
+                        l.noise = f.noise; l.noise_dist = f.dist;
+                        l.noise_op = f.op;
+                        l.noise_a = f.a; l.noise_b = f.b; l.noise_val = f.val;
+                        l.noise_seed = ls->layer->id;
+                        l.noise_step = (uint64_t)it;
+                    }
+                    iz_layers.push_back(l);
                     break;
                 }
                 case Model::GAME_OF_LIFE: {
@@ -619,6 +730,14 @@ Props *Engine::run(const Props &args) {
                               "relay neuron");
                     break;
             }
+        }
+        if (!iz_layers.empty()) {
+            kt.time("iz_neuron", [&] {
+                check_hip(sg_hip_iz_neurons(iz_layers.data(),
+                                            (int)iz_layers.size(), stream),
+                          "iz neuron");
+            });
+            for (ConnState *cs : tss_folded) cs->tss_cur = 1 - cs->tss_cur;
         }
         // 5. weight updates
         if (learning) {

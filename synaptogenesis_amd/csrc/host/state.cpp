@@ -86,6 +86,7 @@ State::~State() {
     for (auto &ls : layers_) {
         free_dev(ls->input.dev);
         free_dev(ls->output.dev);
+        free_dev(ls->d_plane);
         for (auto &kv : ls->vars) free_dev(kv.second.dev);
     }
     for (auto &cs : conns_) {
@@ -791,6 +792,10 @@ void State::to_device() {
         upload(ls->input);
         upload(ls->output);
         for (auto &kv : ls->vars) upload(kv.second);
+        if (ls->model == Model::IZHIKEVICH && !ls->d_plane)
+            check_hip(hipMalloc((void **)&ls->d_plane,
+                                (size_t)(ls->layer->size + 63) / 64 * 8),
+                      "hipMalloc layer plane");
     }
     for (auto &cs : conns_) {
         cs->packed = record_packed(*cs);

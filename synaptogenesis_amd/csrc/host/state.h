@@ -59,6 +59,10 @@ struct LayerState {
     int survival_min = 2, survival_max = 3, birth_min = 3, birth_max = 3;
     bool ramp = false;
 
+    // Izhikevich layers: plane of the newest spike bits (one bit per
+    // neuron), written by the neuron kernel for delay-0 base activators.
+    uint64_t *d_plane = nullptr;
+
     Array &var(const std::string &k) { return vars.at(k); }
 };
 
