@@ -97,3 +97,64 @@ def test_learning_flag_off_freezes_weights(gpu_lib):
     _, finals = _check(case, gpu_lib)
     w0 = case.weights["EE"]
     assert np.array_equal(finals[("conn", "EE", "weights")], w0)
+
+
+def _noise_net(zero_links):
+    """Small E/I sheet plus a third layer P, every layer with an init
+    config (E normal, I uniform, P poisson).  With zero_links each layer
+    also gets two direct one-to-one self-connections of weight 0: they add
+    +0 to the input every step, so the dynamics are the same, but the
+    engine then draws the init values in their own pass instead of inside
+    the neuron launch."""
+    case = netdefs.iz_sheet(16, 20, 40, "noise_fold", workload=212)
+    net = copy.deepcopy(case.net)
+    layers = net["structures"][0]["layers"]
+    layers.append(netdefs.layer("P", "izhikevich", 8, 10, params="chattering"))
+    inits = {"E": {"type": "normal", "mean": "2.0", "std dev": "4.0"},
+             "I": {"type": "uniform", "min": "-1.0", "max": "6.0"},
+             "P": {"type": "poisson", "rate": "50", "value": "30"}}
+    for l in layers:
+        l["init config"] = inits[l["name"]]
+    net["connections"].append(
+        netdefs.conn("EP", "E", "P", "convergent", plastic=True,
+                     arbor={"field size": 3, "stride": 2, "wrap": True},
+                     myelinated=True))
+    weights = dict(case.weights)
+    weights["EP"] = np.full(8 * 10 * 9, 0.4, np.float32)
+    sizes = {"E": 16 * 20, "I": 8 * 10, "P": 8 * 10}
+    if zero_links:
+        for name, n in sizes.items():
+            for j in range(2):
+                cname = "Z%s%d" % (name, j)
+                net["connections"].append(
+                    netdefs.conn(cname, name, name, "one to one",
+                                 direct=True, myelinated=True))
+                weights[cname] = np.zeros(n, np.float32)
+    init = dict(case.init)
+    init[("sheet", "P", "voltage")] = np.full(80, -65.0, np.float32)
+    init[("sheet", "P", "recovery")] = np.full(
+        80, np.float32(0.2) * np.float32(-65.0), np.float32)
+    return harness.Case(
+        "noise_fold" + ("_z" if zero_links else ""), net, 40,
+        weights=weights, delays=case.delays, init=init, inputs={},
+        capture=[("sheet", l, k) for l in ("E", "I", "P")
+                 for k in ("output", "voltage", "input")],
+        final_conn=[("EE", "weights"), ("EP", "weights")])
+
+
+@pytest.mark.gpu
+def test_init_values_drawn_in_neuron_launch_match_separate_pass(gpu_lib):
+    """Init-config values drawn inside the Izhikevich neuron launch give
+    the same run, bit for bit, as the separate init pass plus activators
+    aggregating into it."""
+    a_cap, a_fin, _ = harness.run_case(_noise_net(False), gpu_lib)
+    b_cap, b_fin, _ = harness.run_case(_noise_net(True), gpu_lib)
+    for k in a_cap:
+        for t, (x, y) in enumerate(zip(a_cap[k], b_cap[k])):
+            assert x.tobytes() == y.tobytes(), (k, t)
+    for k in a_fin:
+        assert a_fin[k].tobytes() == b_fin[k].tobytes(), k
+    e_in = np.stack(a_cap[("sheet", "E", "input")])
+    assert abs(float(e_in.mean()) - 2.0) < 0.5 and e_in.std() > 2.0
+    spikes = sum(int(np.sum(c >> 31)) for c in a_cap[("sheet", "P", "output")])
+    assert spikes > 0
