@@ -84,8 +84,13 @@ typedef struct sg_conn {
 } sg_conn;
 
 /* Fields of the interleaved per-synapse record of a tiled Izhikevich
- * connection: for destination tile t and visit k the 7 fields' 64 lanes are
- * consecutive, ((t * K + k) * SG_IZ_FIELDS + field) * 64 + lane. */
+ * connection.  Visits are grouped by eight and the groups are the outer
+ * index: element (destination slot s, visit k, field f) lives at
+ * ((((k / 8) * T + s / 64) * 8 + k % 8) * SG_IZ_FIELDS + f) * 64 + s % 64,
+ * T = ceil(slots / 64), so one wavefront's eight visits are 14 KB of
+ * contiguous memory and the tiles' current groups sit side by side (see
+ * sg_hip_record32).  The record holds ceil(K / 8) * 8 * T * 64 *
+ * SG_IZ_FIELDS floats. */
 enum { SG_IZ_W = 0, SG_IZ_R = 1, SG_IZ_SW = 2, SG_IZ_PRE = 3, SG_IZ_DW = 4,
        SG_IZ_ST = 5, SG_IZ_LT = 6, SG_IZ_FIELDS = 7 };
 
@@ -259,6 +264,12 @@ int sg_hip_transpose32(const void *src, void *dst, int rows, int cols,
  * ceil(slots / 64) * 64 * K * nfields elements. */
 int sg_hip_tile32(const void *src, void *dst, int slots, int K, int nfields,
                   int field, int to_tiled, void *stream);
+
+/* Layout change between one [slots][K] host-layout array and field
+ * `field` of an Izhikevich record (layout above); to_record != 0: host ->
+ * record, else record -> host.  Lanes past `slots` are written as 0. */
+int sg_hip_record32(const void *src, void *dst, int slots, int K, int field,
+                    int to_record, void *stream);
 
 /* Plain copy of n 32-bit words (a known-size reference for the hardware
  * byte counters). */

@@ -293,6 +293,29 @@ __global__ __launch_bounds__(kNeuronBlock) void tile32_kernel(
     else dst[h] = src[t];
 }
 
+__global__ __launch_bounds__(kNeuronBlock) void record32_kernel(
+        const uint32_t *src, uint32_t *dst, int slots, int K, int field,
+        int to_record) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tiles = (slots + 63) / 64;
+    if (i >= tiles * 64 * K) return;
+    // i enumerates (tile, k, lane) of one field
+    const int lane = (int)(i & 63);
+    const long tk = i >> 6;
+    const int k = (int)(tk % K);
+    const long tile = tk / K;
+    const long slot = tile * 64 + lane;
+    const long r = ((((long)(k >> 3) * tiles + tile) * 8 + (k & 7)) *
+                        SG_IZ_FIELDS + field) * 64 + lane;
+    if (slot >= slots) {
+        if (to_record) dst[r] = 0u;
+        return;
+    }
+    const long h = slot * K + k;
+    if (to_record) dst[r] = src[h];
+    else dst[h] = src[r];
+}
+
 __global__ __launch_bounds__(kNeuronBlock) void copy32_kernel(
         const uint4 *src, uint4 *dst, long n4) {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
@@ -399,6 +422,18 @@ extern "C" int sg_hip_tile32(const void *src, void *dst, int slots, int K,
     hipLaunchKernelGGL(tile32_kernel, dim3(nblocks(n)), dim3(kNeuronBlock), 0,
                        (hipStream_t)stream, (const uint32_t *)src,
                        (uint32_t *)dst, slots, K, nfields, field, to_tiled);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_record32(const void *src, void *dst, int slots,
+                               int K, int field, int to_record,
+                               void *stream) {
+    if (slots <= 0 || K <= 0) return 0;
+    if (field < 0 || field >= SG_IZ_FIELDS) return (int)hipErrorInvalidValue;
+    const long n = (long)((slots + 63) / 64) * 64 * K;
+    hipLaunchKernelGGL(record32_kernel, dim3(nblocks(n)), dim3(kNeuronBlock),
+                       0, (hipStream_t)stream, (const uint32_t *)src,
+                       (uint32_t *)dst, slots, K, field, to_record);
     return hipGetLastError();
 }
 

@@ -191,10 +191,12 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
         if (live && kw < count) {
             SynapseIter<TYPE> it;
             it.seek(c, slot, kw);
-            // Record of visit kw of this lane's destination; visit kw + b is
+            // Record of visit kw (a multiple of 8) of this lane's
+            // destination: group kw / 8 of tile blockIdx.x; visit kw + b is
             // b * F * 64 further, its delay byte b * 64 further.
-            float *rec = s.record + (((long)blockIdx.x * K + kw) * F) * 64 +
-                         lane;
+            float *rec = s.record +
+                ((((long)(kw >> 3) * gridDim.x + blockIdx.x) * 8) * F) * 64 +
+                lane;
             const uint8_t *dly = s.delays + tiled_index(slot, kw, K);
             for (int h = 0; h < kPerWave; h += kSub) {
             float w[kSub], r[kSub], sw[kSub], pre[kSub], dw[kSub], st[kSub],

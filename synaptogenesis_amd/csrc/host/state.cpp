@@ -800,7 +800,8 @@ void State::to_device() {
     for (auto &cs : conns_) {
         cs->packed = record_packed(*cs);
         if (cs->packed) {
-            const size_t n = tiled_elems(cs->num_dest, cs->k_per_dest) *
+            const size_t n = tiled_elems(cs->num_dest,
+                                         (cs->k_per_dest + 7) / 8 * 8) *
                              SG_IZ_FIELDS;
             if (!cs->d_record)
                 check_hip(hipMalloc((void **)&cs->d_record, n * 4),
@@ -810,9 +811,9 @@ void State::to_device() {
                 void *tmp = scratch(a.bytes());
                 check_hip(hipMemcpy(tmp, a.host.data(), a.bytes(),
                                     hipMemcpyHostToDevice), "hipMemcpy H2D");
-                check_hip(sg_hip_tile32(tmp, cs->d_record, cs->num_dest,
-                                        cs->k_per_dest, SG_IZ_FIELDS, f, 1,
-                                        nullptr), "tile record");
+                check_hip(sg_hip_record32(tmp, cs->d_record, cs->num_dest,
+                                          cs->k_per_dest, f, 1, nullptr),
+                          "record upload");
                 check_hip(hipDeviceSynchronize(), "tile sync");
             }
         }
@@ -850,9 +851,9 @@ void State::to_host() {
             for (int f = 0; f < SG_IZ_FIELDS; ++f) {
                 Array &a = cs->arr(kRecordFields[f]);
                 void *tmp = scratch(a.bytes());
-                check_hip(sg_hip_tile32(cs->d_record, tmp, cs->num_dest,
-                                        cs->k_per_dest, SG_IZ_FIELDS, f, 0,
-                                        nullptr), "untile record");
+                check_hip(sg_hip_record32(cs->d_record, tmp, cs->num_dest,
+                                          cs->k_per_dest, f, 0, nullptr),
+                          "record download");
                 check_hip(hipMemcpy(a.host.data(), tmp, a.bytes(),
                                     hipMemcpyDeviceToHost), "hipMemcpy D2H");
             }
