@@ -1,11 +1,12 @@
-"""Throughput of the MI355X simulation core on the Izhikevich E/I sheet.
+"""Throughput of the MI355X simulation core.
 
-One "step" is one simulation timestep of the whole network: every synapse
-activation (E->E on the conductance path with STDP + short-term plasticity
-fused into the same pass, E->I and I->E direct), every neuron update and
-every weight update, with the state resident in HBM.  The metric is synaptic
-updates per second (visited (source, destination) pairs per step x steps /
-time); neuron-steps per second is reported beside it.
+Default workload: the Izhikevich E/I sheet.  One "step" is one simulation
+timestep of the whole network: every synapse activation (E->E on the
+conductance path with STDP + short-term plasticity fused into the same pass,
+E->I and I->E direct), every neuron update and every weight update, with the
+state resident in HBM.  The metric is synaptic updates per second (visited
+(source, destination) pairs per step x steps / time); neuron-steps per
+second is reported beside it.
 
 Timed region: one run() of --steps iterations through the C-ABI after a
 --warmup run that moved the state to the GPU.  A second run of the same
@@ -14,7 +15,12 @@ to get the dominant kernel's average duration for the roofline.  The CPU
 baseline times the reference's serial build (oracle/_ref, compiled from the
 reference's sources) on a bounded number of steps of the same workload.
 
+--workload also measures the other single-GPU configurations: izfc (two
+Izhikevich layers of 1000, fully connected) and gol1024 / gol16384 (game of
+life boards; the larger one does not fit the 256 MB Infinity Cache).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
+                    [--workload sheet|izfc|gol1024|gol16384]
 """
 
 import argparse
@@ -65,6 +71,43 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+# Per workload: the dominant kernel (its "kernel timing" key and a label),
+# its algorithmic bytes per launch, the whole step's algorithmic bytes, and
+# the bounded CPU-baseline sample (steps of the reference's serial build).
+def _workloads():
+    ne, ni = 256 * 320, 128 * 160
+    return {
+        "sheet": dict(make=workloads.sheet, kernel="iz_update_activate",
+                      label="iz_update_activate (E->E)",
+                      launch_bytes=fused_bytes(ne),
+                      step_bytes=step_bytes(ne, ni), cpu_steps=20,
+                      detail=" (E->E conv 15x15 s1 wrap STDP+STP, E->I conv "
+                             "15x15 s2 direct, I->E div 9x9 s2 direct)",
+                      data="synthetic (Philox-seeded weights U[0,0.5), "
+                           "delays U{0..31}; device-drawn N(0,3) noise)"),
+        # weights 4 B/synapse + source words + destination input RW
+        "izfc": dict(make=workloads.izfc, kernel="activate_base",
+                     label="activate_base (L1->L2 fully connected)",
+                     launch_bytes=1000 * 1000 * 4 + 1000 * 4 + 1000 * 8,
+                     step_bytes=4228000, cpu_steps=300, detail="",
+                     data="synthetic (Philox-seeded weights U[0,0.5); "
+                          "device-drawn N(5,2) input into L1)"),
+        "gol1024": dict(make=lambda: workloads.gol(1024), kernel="gol_step",
+                        label="gol_step (fused 3x3 count + rule + shift)",
+                        launch_bytes=8 * 1024 * 1024,
+                        step_bytes=8 * 1024 * 1024, cpu_steps=40,
+                        detail=" (3x3 convolutional, wrap, self excluded)",
+                        data="synthetic (Philox Bernoulli(0.5) cells)"),
+        "gol16384": dict(make=lambda: workloads.gol(16384),
+                         kernel="gol_step",
+                         label="gol_step (fused 3x3 count + rule + shift)",
+                         launch_bytes=8 * 16384 * 16384,
+                         step_bytes=8 * 16384 * 16384, cpu_steps=1,
+                         detail=" (3x3 convolutional, wrap, self excluded)",
+                         data="synthetic (Philox Bernoulli(0.5) cells)"),
+    }
 
 
 def build(lib, wl):
@@ -139,7 +182,7 @@ def cpu_baseline(wl, steps):
             "unit": "synaptic updates/s", "cores": 1, "kind": "reference",
             "sample": "%d steps of the full %s on the reference's serial "
                       "CPU build (worker threads 0, single thread), "
-                      "%.2f s/step" % (steps, wl["name"], dt / steps)}
+                      "%.4g s/step" % (steps, wl["name"], dt / steps)}
 
 
 def copy_calibration(mb):
@@ -179,8 +222,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="CPU-baseline sample (0: the workload's default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="sheet",
+                    choices=sorted(_workloads()))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference "
                          "for hardware byte counters) and exit")
@@ -195,7 +241,8 @@ def main():
         copy_calibration(a.copy_calibration)
         return
 
-    wl = workloads.sheet()
+    spec = _workloads()[a.workload]
+    wl = spec["make"]()
     lib = syngen.load_library()
     lib.set_suppress_output(True)
     lib.set_warnings(False)
@@ -206,16 +253,17 @@ def main():
     timing = kreport.children["kernel timing"].properties
     network.free()
 
-    ne, ni = 256 * 320, 128 * 160
     ms_per_step = dt * 1000.0 / a.steps
     syn_per_s = wl["visits_per_step"] * a.steps / dt
-    fused_ms = float(timing["iz_update_activate ms"]) / \
-        float(timing["iz_update_activate launches"])
-    achieved = fused_bytes(ne) / (fused_ms * 1e-3) / 1e9
+    key = spec["kernel"]
+    kernel_ms = float(timing[key + " ms"]) / float(timing[key + " launches"])
+    achieved = spec["launch_bytes"] / (kernel_ms * 1e-3) / 1e9
     kernels = {k[:-3]: float(v) / float(timing[k[:-3] + " launches"])
                for k, v in timing.items() if k.endswith(" ms")}
 
-    cpu = None if a.no_cpu else cpu_baseline(wl, a.cpu_steps)
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(wl, a.cpu_steps or spec["cpu_steps"])
     out = {
         "metric": "synaptic updates/sec (neuron-steps/sec beside it) on 1 "
                   "MI355X; % HBM roofline",
@@ -225,21 +273,21 @@ def main():
         "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (Philox-seeded weights U[0,0.5), delays "
-                "U{0..31}; device-drawn N(0,3) noise)",
-        "config": {"workload": wl["name"] + " (E->E conv 15x15 s1 wrap "
-                   "STDP+STP, E->I conv 15x15 s2 direct, I->E div 9x9 s2 "
-                   "direct)", "synapse_visits_per_step": wl["visits_per_step"],
+        "data": spec["data"],
+        "config": {"workload": wl["name"] + spec["detail"],
+                   "synapse_visits_per_step": wl["visits_per_step"],
                    "neurons": wl["neurons"]},
         "neuron_steps_per_s": wl["neurons"] * a.steps / dt,
-        "step_hbm_fraction": step_bytes(ne, ni) / (ms_per_step * 1e-3) /
+        "step_hbm_fraction": spec["step_bytes"] / (ms_per_step * 1e-3) /
                              1e9 / HBM_PEAK_GBPS,
-        "roofline": {"bound": "hbm", "kernel": "iz_update_activate (E->E)",
+        "roofline": {"bound": "hbm", "kernel": spec["label"],
                      "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": measured_traffic(),
-                     "traffic_source": os.path.relpath(TRAFFIC_SUMMARY, ROOT),
-                     "avg_ms": fused_ms},
+                     "traffic": (measured_traffic()
+                                 if a.workload == "sheet" else None),
+                     "traffic_source": (os.path.relpath(TRAFFIC_SUMMARY, ROOT)
+                                        if a.workload == "sheet" else None),
+                     "avg_ms": kernel_ms},
         "kernel_avg_ms": kernels,
         "cpu_baseline": cpu,
     }

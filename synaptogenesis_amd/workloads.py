@@ -100,3 +100,29 @@ def gol(n=1024):
             "visits_per_step": 9 * n * n, "neurons": n * n,
             "init": {("gol", "board", "output"): words},
             "name": "game of life %dx%d" % (n, n)}
+
+
+def izfc(size=1000):
+    """Two Izhikevich layers of `size` (regular), FULLY_CONNECTED L1 -> L2
+    direct and myelinated, weights U[0,0.5); the N(5, 2) input current of
+    L1 is its init config (drawn on the device every step)."""
+    st = Stream(1)
+    layers = [_layer("L1", "izhikevich", 1, size, params="regular",
+                     **{"init config": {"type": "normal", "mean": "5",
+                                        "std dev": "2"}}),
+              _layer("L2", "izhikevich", 1, size, params="regular")]
+    conns = [{"name": "fc", "from layer": "L1", "to layer": "L2",
+              "type": "fully connected", "opcode": "add", "plastic": "false",
+              "direct": "true", "myelinated": "true", "delay": "0"}]
+    net = {"structures": [{"name": "izfc", "type": "parallel",
+                           "layers": layers}], "connections": conns}
+    v0 = np.full(size, -65.0, np.float32)
+    u0 = np.full(size, np.float32(0.2) * np.float32(-65.0), np.float32)
+    return {"net": net,
+            "weights": {"fc": lambda: st.uniform(size * size, stream=1,
+                                                 low=0.0, high=0.5)},
+            "delays": {}, "visits_per_step": size * size,
+            "neurons": 2 * size,
+            "init": {("izfc", l, k): v for l in ("L1", "L2")
+                     for k, v in (("voltage", v0), ("recovery", u0))},
+            "name": "izhikevich fully connected %d->%d" % (size, size)}

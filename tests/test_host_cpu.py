@@ -191,3 +191,71 @@ def test_run_without_device_fails_loudly(lib):
     case = netdefs.gol("g", 6, 6, 2, 1)
     _, _, report = harness.run_case(case, lib)
     assert report is not None and report.get("iterations") is None
+
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libsyngen_ref.so")
+
+
+@pytest.fixture(scope="module")
+def ref_lib():
+    if not os.path.exists(REF_LIB):
+        pytest.skip("reference serial build not present (oracle/_ref)")
+    lib = syngen.load_library(REF_LIB)
+    lib.set_suppress_output(True)
+    lib.set_warnings(False)
+    return lib
+
+
+def _state_arrays(network, net):
+    out = {}
+    for layer in net.order:
+        for k in neuron_keys(layer):
+            if k == "input":
+                continue
+            out[(layer.name, k)] = harness._view(
+                network.get_neuron_data(layer.structure, layer.name, k), k)
+    for c in net.conns:
+        # The reference's getter serves weights and registered variables
+        # only (weight_matrix.cpp:105-114); delays are not one of them.
+        for k in conn_keys(c):
+            if k == "delays":
+                continue
+            out[(c.name, k)] = harness._view(
+                network.get_weight_matrix(c.name, k), k)
+    return out
+
+
+@pytest.mark.parametrize("direction", ["reference_to_here", "here_to_reference"])
+def test_state_files_cross_between_implementations(lib, ref_lib, tmp_path,
+                                                   direction):
+    """A state file written by one implementation loads into the other with
+    every neuron and synapse array identical (state/state.cpp:365-505
+    format: PointerKey records in host layout)."""
+    case = netdefs.iz_ops()
+    net = stepper.Network(case)
+    src_lib, dst_lib = ((ref_lib, lib) if direction == "reference_to_here"
+                        else (lib, ref_lib))
+    # Distinct values in every array, written through the source's views.
+    src = syngen.Network(case.net, src_lib)
+    src.build_state()
+    rng = np.random.default_rng(11)
+    for key, arr in _state_arrays(src, net).items():
+        name, k = key
+        view = harness._writable(
+            src.get_weight_matrix(name, k) if any(c.name == name
+                                                  for c in net.conns)
+            else src.get_neuron_data("ops", name, k), k)
+        if view.dtype == np.float32:
+            view[:] = rng.random(view.size, dtype=np.float32)
+        else:
+            view[:] = rng.integers(0, 31, view.size).astype(view.dtype)
+    want = _state_arrays(src, net)
+    path = str(tmp_path / "state.bin")
+    assert src.save_state(path)
+    dst = syngen.Network(case.net, dst_lib)
+    assert dst.build_state(path)
+    got = _state_arrays(dst, net)
+    bad = [k for k in want if want[k].tobytes() != got[k].tobytes()]
+    assert not bad, bad
+    src.free()
+    dst.free()
