@@ -66,18 +66,25 @@ class Geom(ctypes.Structure):
 _lib = None
 
 
+def open_library(path):
+    """Opens a build of oracle.c and declares the non-void signatures."""
+    lib = ctypes.CDLL(path)
+    lib.orc_enumerate.restype = ctypes.c_long
+    lib.orc_iz_init.restype = ctypes.c_int
+    lib.orc_iz_init.argtypes = [ctypes.c_char_p, ctypes.c_long] + \
+        [ctypes.c_void_p] * 8
+    return lib
+
+
 def load():
-    """Loads liboracle.so (built by ``make -C oracle``)."""
+    """The oracle library every call of this module uses (liboracle.so,
+    built by ``make -C oracle``)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("oracle/liboracle.so is not built "
                               "(make -C oracle)")
-        _lib = ctypes.CDLL(LIB_PATH)
-        _lib.orc_enumerate.restype = ctypes.c_long
-        _lib.orc_iz_init.restype = ctypes.c_int
-        _lib.orc_iz_init.argtypes = [ctypes.c_char_p, ctypes.c_long] + \
-            [ctypes.c_void_p] * 8
+        _lib = open_library(LIB_PATH)
     return _lib
 
 
