@@ -103,7 +103,9 @@ typedef struct sg_iz_syn {
     float *weights;
     float *short_traces, *long_traces, *presyn_traces;
     float *stp_r, *stp_w, *weight_derivatives;
-    /* per-synapse delay & 31 (the bit of the source word the synapse reads) */
+    /* per-synapse delay & 31 (the bit of the source word the synapse
+     * reads); tiled types: the record's visit-group-major order with one
+     * field, (((k / 8) * T + s / 64) * 8 + k % 8) * 64 + s % 64 */
     const uint8_t *delays;
     /* time since spike per source after the last activation (read by the
      * updater) */

@@ -169,6 +169,7 @@ class Conn:
         self.type = TYPES[d.get("type", "fully connected")]
         self.opcode = OPCODES[d.get("opcode", "add")]
         self.plastic = _bool(d.get("plastic"), True)
+        self.recurrent = _bool(d.get("recurrent"), False)
         self.delay = int(d.get("delay", "0"))
         self.max_weight = stof(d.get("max weight", "1.0"))
         self.convolutional = _bool(d.get("convolutional"), False)
@@ -307,6 +308,73 @@ def _init_iz_matrix(conn, delays):
     a["time since spike"] = np.full(max(nw, conn.src.n), 32, np.int32)
 
 
+def sequential_order(layers, conns):
+    """Layer order of a sequential / feedforward cluster
+    (engine/cluster/sequential_cluster.cpp:16-77): breadth-first backwards
+    from the structure's output layers (network/layer.cpp:54-60) and layers
+    without outgoing connections; a layer is placed in front once all the
+    layers it feeds (same structure, not itself, not recurrent) are
+    placed."""
+    members = set(id(l) for l in layers)
+
+    def outs(l):
+        return [c for c in conns if c.src is l]
+
+    def ins(l):
+        return [c for c in conns if c.dst is l]
+
+    def same(l):
+        return id(l) in members
+
+    def is_output(l):
+        return not any(same(c.dst) and not c.recurrent for c in outs(l))
+
+    queue = [l for l in layers if is_output(l) or not outs(l)]
+    visited, order, stalled = set(), [], 0
+    while queue:
+        cur = queue.pop(0)
+        if id(cur) in visited:
+            continue
+        if any(c.dst is not cur and same(c.dst) and not c.recurrent and
+               id(c.dst) not in visited for c in outs(cur)):
+            queue.append(cur)
+            stalled += 1
+            if stalled > 2 * len(queue) + len(layers):
+                raise ValueError("cycle of non-recurrent connections")
+            continue
+        stalled = 0
+        visited.add(id(cur))
+        order.insert(0, cur)
+        for c in ins(cur):
+            if c.src is not cur and same(c.src) and not c.recurrent and \
+                    id(c.src) not in visited:
+                queue.append(c.src)
+    if len(visited) < len(layers):
+        raise ValueError("sequential cluster failed to process all layers")
+    return order
+
+
+def check_feedforward(layers, conns):
+    """feedforward_cluster.cpp:8-38: no cycle (self-connections included)
+    reachable from a structure input layer (network/layer.cpp:46-52)."""
+    members = set(id(l) for l in layers)
+
+    def dfs(l, path):
+        if id(l) in path:
+            return False
+        path.add(id(l))
+        for c in conns:
+            if c.src is l and id(c.dst) in members and not dfs(c.dst, path):
+                return False
+        path.discard(id(l))
+        return True
+
+    for l in layers:
+        if not any(c.dst is l and id(c.src) in members and not c.recurrent
+                   for c in conns) and not dfs(l, set()):
+            raise ValueError("feedforward structure with a cycle")
+
+
 class Network:
     """A case's network on the oracle."""
 
@@ -314,17 +382,29 @@ class Network:
         net = case.net
         self.layers = {}
         self.order = []
+        self.structures = []      # (cluster type, [layers])
         for s in net["structures"]:
-            if s.get("type", "parallel") != "parallel":
-                raise NotImplementedError("oracle: cluster type " +
-                                          s.get("type"))
+            ctype = s.get("type", "parallel")
+            if ctype not in ("parallel", "sequential", "feedforward"):
+                raise NotImplementedError("oracle: cluster type " + ctype)
+            members = []
             for d in s.get("layers", []):
                 layer = Layer(s["name"], d)
                 self.layers[(s["name"], layer.name)] = layer
                 self.order.append(layer)
+                members.append(layer)
+            self.structures.append((ctype, members))
         default = net["structures"][0]["name"]
         self.conns = [Conn(d, self.layers, default)
                       for d in net.get("connections", [])]
+        self.seq = {}
+        for i, (ctype, members) in enumerate(self.structures):
+            if ctype == "parallel":
+                continue
+            if ctype == "feedforward":
+                check_feedforward(members, self.conns)
+            self.seq[i] = sequential_order(members, self.conns)
+        self.in_seq = set(id(l) for i in self.seq for l in self.seq[i])
         for c in self.conns:
             if c.src.out_type == OUT_BIT:
                 regs = c.delay // 32
@@ -420,20 +500,44 @@ class Network:
                           c.src.out_type, ctypes.c_float(c.learning_rate),
                           ctypes.c_float(c.max_weight), int(c.stp_flag))
 
-    def step(self, inputs, t, learning=True):
-        for layer in self.order:
-            if not layer.is_input:
-                layer.input[:] = 0.0
+    def _activate_layer(self, layer):
+        """A layer's activate instructions: clear (non-input layers), then
+        its input connections in order (cluster_node.cpp:40-67,
+        instruction.cpp:3-30)."""
+        if not layer.is_input:
+            layer.input[:] = 0.0
         for c in self.conns:
-            if not c.dst.is_input:
+            if c.dst is layer:
                 self._activate(c)
+
+    def step(self, inputs, t, learning=True):
+        """engine.cpp:231-264: every cluster's pre-input work, the input,
+        every cluster's post-input work, every cluster's state update, then
+        the weight updates.  Parallel clusters activate non-input layers
+        before the input and update all layers afterwards
+        (parallel_cluster.cpp:107-131); sequential and feedforward clusters
+        activate and update layer by layer after the input
+        (sequential_cluster.cpp:100-113)."""
+        par = [(ctype, m) for ctype, m in self.structures
+               if ctype == "parallel"]
+        for _, members in par:
+            for layer in members:
+                if not layer.is_input:
+                    self._activate_layer(layer)
         for (s, l), fn in inputs.items():
             self.layers[(s, l)].input[:] = np.asarray(fn(t), np.float32)
-        for c in self.conns:
-            if c.dst.is_input:
-                self._activate(c)
-        for layer in self.order:
-            self._neuron(layer)
+        for i, (ctype, members) in enumerate(self.structures):
+            if ctype == "parallel":
+                for layer in members:
+                    if layer.is_input:
+                        self._activate_layer(layer)
+            else:
+                for layer in self.seq[i]:
+                    self._activate_layer(layer)
+                    self._neuron(layer)
+        for _, members in par:
+            for layer in members:
+                self._neuron(layer)
         if learning:
             for c in self.conns:
                 if c.iz and c.plastic:

@@ -197,7 +197,8 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
             float *rec = s.record +
                 ((((long)(kw >> 3) * gridDim.x + blockIdx.x) * 8) * F) * 64 +
                 lane;
-            const uint8_t *dly = s.delays + tiled_index(slot, kw, K);
+            const uint8_t *dly = s.delays +
+                (((long)(kw >> 3) * gridDim.x + blockIdx.x) * 8) * 64 + lane;
             for (int h = 0; h < kPerWave; h += kSub) {
             float w[kSub], r[kSub], sw[kSub], pre[kSub], dw[kSub], st[kSub],
                 lt[kSub];

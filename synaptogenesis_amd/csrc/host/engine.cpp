@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <deque>
+#include <functional>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -88,6 +90,79 @@ struct KernelTimer {
     }
 };
 
+// Layer order of a sequential or feedforward cluster
+// (engine/cluster/sequential_cluster.cpp:16-77): a breadth-first search
+// backwards from the structure's output layers (no non-recurrent connection
+// into the structure, network/layer.cpp:54-60) or layers without outgoing
+// connections; a layer is placed once every layer it feeds in the structure
+// (not itself, not through recurrent connections) is placed, and it goes in
+// front of them.  Where the reference would requeue forever (a cycle of
+// non-recurrent connections) this throws.
+std::vector<Layer *> sequential_order(const Structure &st) {
+    auto same = [&](const Layer *l) { return l->structure == &st; };
+    auto is_output = [&](const Layer *l) {
+        for (auto *c : l->outputs)
+            if (same(c->to) && !c->recurrent) return false;
+        return true;
+    };
+    std::deque<Layer *> queue;
+    for (auto &l : st.layers)
+        if (is_output(l.get()) || l->outputs.empty()) queue.push_back(l.get());
+    std::set<Layer *> visited;
+    std::vector<Layer *> order;
+    size_t stalled = 0;
+    while (!queue.empty()) {
+        Layer *curr = queue.front();
+        queue.pop_front();
+        if (visited.count(curr)) continue;
+        bool later = false;
+        for (auto *c : curr->outputs)
+            if (c->to != curr && same(c->to) && !c->recurrent &&
+                !visited.count(c->to))
+                later = true;
+        if (later) {
+            queue.push_back(curr);
+            if (++stalled > 2 * queue.size() + st.layers.size())
+                throw Error("Sequential cluster " + st.name + ": the layers "
+                            "form a cycle of non-recurrent connections");
+            continue;
+        }
+        stalled = 0;
+        visited.insert(curr);
+        order.insert(order.begin(), curr);
+        for (auto *c : curr->inputs)
+            if (c->from != curr && same(c->from) && !c->recurrent &&
+                !visited.count(c->from))
+                queue.push_back(c->from);
+    }
+    if (visited.size() < st.layers.size())
+        throw Error("Sequential cluster failed to process all layers!");
+    return order;
+}
+
+// Feedforward clusters refuse cycles reachable from a structure input layer
+// (feedforward_cluster.cpp:8-38; every same-structure connection counts,
+// self-connections included).
+void check_feedforward(const Structure &st) {
+    std::set<const Layer *> path;
+    std::function<bool(const Layer *)> dfs = [&](const Layer *l) {
+        if (path.count(l)) return false;
+        path.insert(l);
+        for (auto *c : l->outputs)
+            if (c->to->structure == &st && !dfs(c->to)) return false;
+        path.erase(l);
+        return true;
+    };
+    for (auto &l : st.layers) {
+        bool input = true;      // network/layer.cpp:46-52
+        for (auto *c : l->inputs)
+            if (c->from->structure == &st && !c->recurrent) input = false;
+        if (input && !dfs(l.get()))
+            throw Error("Feedforward engine requires a structure with no "
+                        "cycles!");
+    }
+}
+
 void check_layer_model_supports(LayerState *ls, const std::string &key) {
     if (key == "input" || key == "output") return;
     if (!ls->vars.count(key))
@@ -134,15 +209,19 @@ Props *Engine::run(const Props &args) {
     }
     check_hip(hipSetDevice(device), "hipSetDevice");
 
-    // The step order below is the parallel cluster's
-    // (engine/cluster/parallel_cluster.cpp:107-131).  Sequential and
-    // feedforward clusters interleave activation and state update per layer
-    // in a dependency order (sequential_cluster.cpp:16-77, 100-113); they
-    // are refused rather than run in the wrong order.
-    for (auto &st : net_->structures())
-        if (st->cluster != CLUSTER_PARALLEL)
-            throw Error("Structure " + st->name + ": only \"parallel\" "
-                        "clusters are supported by this engine");
+    // Cluster orders (engine/cluster/*.cpp): parallel structures activate
+    // every layer, then update every layer; sequential and feedforward
+    // structures activate and update layer by layer in dependency order
+    // (sequential_cluster.cpp:100-113), weights backwards.
+    std::vector<std::vector<Layer *>> seq_order(net_->structures().size());
+    std::set<const Layer *> in_seq;
+    for (size_t i = 0; i < net_->structures().size(); ++i) {
+        const Structure &st = *net_->structures()[i];
+        if (st.cluster == CLUSTER_PARALLEL) continue;
+        if (st.cluster == CLUSTER_FEEDFORWARD) check_feedforward(st);
+        seq_order[i] = sequential_order(st);
+        for (auto &l : st.layers) in_seq.insert(l.get());
+    }
 
     state_->to_device();
 
@@ -260,6 +339,20 @@ Props *Engine::run(const Props &args) {
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg_{stream};
+    // A second stream for the base activators of non-input layers, which
+    // then run beside the conductance passes (see the main loop).
+    hipStream_t side;
+    check_hip(hipStreamCreate(&side), "hipStreamCreate");
+    StreamGuard sg2_{side};
+    hipEvent_t ev_fork, ev_join;
+    check_hip(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming),
+              "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming),
+              "hipEventCreate");
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } eg_{ev_fork, ev_join};
     KernelTimer kt;
     kt.on = args.get_bool("kernel timing", false);
     kt.stream = stream;
@@ -376,12 +469,12 @@ Props *Engine::run(const Props &args) {
         return cs.iz && !cs.direct && cs.iz_kind >= 0 &&
                cs.iz_kind != SG_IZ_GAP && cs.update_kind >= 0;
     };
-    auto activate = [&](ConnState &cs) {
+    auto activate = [&](ConnState &cs, hipStream_t st) {
         if (cs.fused_gol) return;
         sg_conn d = conn_desc(cs);
         if (!cs.iz || cs.direct) {
             kt.time("activate_base", [&] {
-                check_hip(sg_hip_activate_base(&d, stream), "activate");
+                check_hip(sg_hip_activate_base(&d, st), "activate");
             });
             return;
         }
@@ -389,7 +482,7 @@ Props *Engine::run(const Props &args) {
         sg_iz_syn s = iz_syn(cs);
         if (cs.iz_kind == SG_IZ_GAP) {
             kt.time("iz_gap", [&] {
-                check_hip(sg_hip_iz_activate(&d, &s, &dd, SG_IZ_GAP, stream),
+                check_hip(sg_hip_iz_activate(&d, &s, &dd, SG_IZ_GAP, st),
                           "gap activate");
             });
             return;
@@ -398,7 +491,7 @@ Props *Engine::run(const Props &args) {
             check_hip(sg_hip_tss_step(cs.d_tss[1 - cs.tss_cur],
                                       cs.d_tss[cs.tss_cur], cs.d_visits,
                                       cs.d_rev_delays, d.src,
-                                      cs.conn->from->size, stream),
+                                      cs.conn->from->size, st),
                       "tss step");
         });
         if (cs.pending_update) {
@@ -406,13 +499,13 @@ Props *Engine::run(const Props &args) {
                 check_hip(sg_hip_iz_update_activate(
                               &d, &s, &dd, cs.iz_kind, cs.update_kind,
                               state_->d_src_table(), state_->d_dst_table(),
-                              stream),
+                              st),
                           "update+activate");
             });
             cs.pending_update = false;
         } else {
             kt.time("iz_activate", [&] {
-                check_hip(sg_hip_iz_activate(&d, &s, &dd, cs.iz_kind, stream),
+                check_hip(sg_hip_iz_activate(&d, &s, &dd, cs.iz_kind, st),
                           "iz activate");
             });
         }
@@ -439,7 +532,8 @@ Props *Engine::run(const Props &args) {
     for (auto &lsp : state_->layers()) {
         LayerState *ls = lsp.get();
         if (ls->model != Model::GAME_OF_LIFE || ls->history != 1) continue;
-        if (io_types.count(ls) || ls->layer->config.has_child("init config"))
+        if (io_types.count(ls) || ls->layer->config.has_child("init config") ||
+            in_seq.count(ls->layer))
             continue;
         if (ls->layer->inputs.size() != 1) continue;
         Connection *c = ls->layer->inputs[0];
@@ -557,6 +651,10 @@ Props *Engine::run(const Props &args) {
         if (!cs->iz || cs->direct || cs->iz_kind < 0 ||
             cs->iz_kind == SG_IZ_GAP || cs->from->model != Model::IZHIKEVICH)
             continue;
+        // In a sequential structure the source may update before this
+        // connection activates within a step; step it at activation then.
+        if (in_seq.count(cs->from->layer) || in_seq.count(cs->to->layer))
+            continue;
         auto &v = tss_out[cs->from];
         if ((int)v.size() >= SG_MAX_TSS_OUT) continue;
         v.push_back(cs);
@@ -568,18 +666,59 @@ Props *Engine::run(const Props &args) {
                                         lsp->d_plane, stream),
                       "pack plane");
 
+    // Split the pre-input activators over two streams only when both kinds
+    // are there.
+    auto is_base = [&](const ConnState &cs) {
+        return (!cs.iz || cs.direct) && !cs.fused_gol;
+    };
+    bool any_base = false, any_other = false;
+    for (auto &lsp : state_->layers()) {
+        if (is_input(lsp.get()) || gol_fast.count(lsp.get()) ||
+            in_seq.count(lsp->layer))
+            continue;
+        for (auto *c : lsp->layer->inputs) {
+            const ConnState &cs = *state_->conn_state(c);
+            if (cs.fused_gol) continue;
+            (is_base(cs) ? any_base : any_other) = true;
+        }
+    }
+    const bool split_ok = any_base && any_other;
+
     // ---- main loop --------------------------------------------------------
     auto &layers = state_->layers();
     long done = 0;
     const auto t0 = std::chrono::steady_clock::now();
     bool interrupted = false;
     for (long it = 0; iterations == 0 || it < iterations; ++it) {
-        // 1. pre-input
+        // 1. pre-input: init values, then every activator.  Base activators
+        // (latency-bound) go to the second stream and run beside the
+        // conductance passes (HBM-bound): they write the layers' inputs, the
+        // conductance passes write conductances and synapse records, and
+        // both read only last step's spikes; the join below precedes the
+        // neuron launch.  With kernel timing on, one stream.
         for (auto &lsp : layers) {
             LayerState *ls = lsp.get();
-            if (is_input(ls) || gol_fast.count(ls)) continue;
+            if (is_input(ls) || gol_fast.count(ls) || in_seq.count(ls->layer))
+                continue;
             if (!init_fold.count(ls)) init_layer(ls, true, it);
-            for (auto *c : ls->layer->inputs) activate(*state_->conn_state(c));
+        }
+        const bool split = split_ok && !kt.on;
+        if (split) {
+            check_hip(hipEventRecord(ev_fork, stream), "hipEventRecord");
+            check_hip(hipStreamWaitEvent(side, ev_fork, 0), "wait fork");
+        }
+        for (auto &lsp : layers) {
+            LayerState *ls = lsp.get();
+            if (is_input(ls) || gol_fast.count(ls) || in_seq.count(ls->layer))
+                continue;
+            for (auto *c : ls->layer->inputs) {
+                ConnState &cs = *state_->conn_state(c);
+                activate(cs, split && is_base(cs) ? side : stream);
+            }
+        }
+        if (split) {
+            check_hip(hipEventRecord(ev_join, side), "hipEventRecord");
+            check_hip(hipStreamWaitEvent(stream, ev_join, 0), "wait join");
         }
         // 2. environment input
         bool any_input = false;
@@ -631,113 +770,146 @@ Props *Engine::run(const Props &args) {
                 }
             }
         }
-        // 3. post-input
-        for (auto &lsp : layers) {
-            LayerState *ls = lsp.get();
-            if (!is_input(ls)) continue;
-            // Only an "input" key makes the layer's input register the
-            // environment's (cluster_node.cpp:20-44); auxiliary input keys
-            // leave it to the usual clear.
-            const auto ik = in_keys.find(ls);
-            init_layer(ls, ik == in_keys.end() || !ik->second.count("input"),
-                       it);
-            for (auto *c : ls->layer->inputs) activate(*state_->conn_state(c));
-        }
-        // 4. neuron updates: every Izhikevich layer in one launch
+        // 3. post-input, per structure in order: the input layers of a
+        // parallel structure activate; a sequential structure runs its
+        // layers one by one (init, activators, neuron update)
+        // (parallel_cluster.cpp:112-120, sequential_cluster.cpp:100-108).
         const bool last = iterations != 0 && it + 1 == iterations;
-        std::vector<sg_iz_layer> iz_layers;
-        for (auto &lsp : layers) {
-            LayerState *ls = lsp.get();
-            const int n = ls->layer->size;
-            switch (ls->model) {
-                case Model::IZHIKEVICH: {
-                    sg_iz_layer l;
-                    std::memset(&l, 0, sizeof(l));
-                    l.size = n; l.history = ls->history;
-                    l.input = ls->input.df(); l.output = ls->output.du();
-                    l.ampa = ls->var("ampa").df(); l.nmda = ls->var("nmda").df();
-                    l.gabaa = ls->var("gabaa").df();
-                    l.gabab = ls->var("gabab").df();
-                    l.mult = ls->var("mult").df();
-                    l.dopamine = ls->var("dopamine").df();
-                    l.voltage = ls->var("voltage").df();
-                    l.recovery = ls->var("recovery").df();
-                    l.post_trace = ls->var("post trace").df();
-                    l.time_since_spike = ls->var("time since spike").di();
-                    l.a = ls->var("a").df(); l.b = ls->var("b").df();
-                    l.c = ls->var("c").df(); l.d = ls->var("d").df();
-                    l.plane = ls->d_plane;
-                    for (ConnState *cs : tss_out[ls]) {
-                        sg_tss_out &o = l.tss[l.ntss++];
-                        o.next = cs->d_tss[1 - cs->tss_cur];
-                        o.cur = cs->d_tss[cs->tss_cur];
-                        o.visits = cs->d_visits;
-                        o.rev_delays = cs->d_rev_delays;
-                        o.word = cs->conn->delay / 32;
+        auto update_neurons = [&](const std::vector<LayerState *> &lss) {
+            std::vector<sg_iz_layer> iz_layers;
+            std::vector<ConnState *> stepped;   // time since spike stepped
+            for (LayerState *ls : lss) {
+                const int n = ls->layer->size;
+                switch (ls->model) {
+                    case Model::IZHIKEVICH: {
+                        sg_iz_layer l;
+                        std::memset(&l, 0, sizeof(l));
+                        l.size = n; l.history = ls->history;
+                        l.input = ls->input.df(); l.output = ls->output.du();
+                        l.ampa = ls->var("ampa").df(); l.nmda = ls->var("nmda").df();
+                        l.gabaa = ls->var("gabaa").df();
+                        l.gabab = ls->var("gabab").df();
+                        l.mult = ls->var("mult").df();
+                        l.dopamine = ls->var("dopamine").df();
+                        l.voltage = ls->var("voltage").df();
+                        l.recovery = ls->var("recovery").df();
+                        l.post_trace = ls->var("post trace").df();
+                        l.time_since_spike = ls->var("time since spike").di();
+                        l.a = ls->var("a").df(); l.b = ls->var("b").df();
+                        l.c = ls->var("c").df(); l.d = ls->var("d").df();
+                        l.plane = ls->d_plane;
+                        for (ConnState *cs : tss_out[ls]) {
+                            stepped.push_back(cs);
+                            sg_tss_out &o = l.tss[l.ntss++];
+                            o.next = cs->d_tss[1 - cs->tss_cur];
+                            o.cur = cs->d_tss[cs->tss_cur];
+                            o.visits = cs->d_visits;
+                            o.rev_delays = cs->d_rev_delays;
+                            o.word = cs->conn->delay / 32;
+                        }
+                        auto fi = init_fold.find(ls);
+                        if (fi != init_fold.end()) {
+                            const InitFold &f = fi->second;
+                            l.noise = f.noise; l.noise_dist = f.dist;
+                            l.noise_op = f.op;
+                            l.noise_a = f.a; l.noise_b = f.b; l.noise_val = f.val;
+                            l.noise_seed = ls->layer->id;
+                            l.noise_step = (uint64_t)it;
+                        }
+                        iz_layers.push_back(l);
+                        break;
                     }
-                    auto fi = init_fold.find(ls);
-                    if (fi != init_fold.end()) {
-                        const InitFold &f = fi->second;
-                        l.noise = f.noise; l.noise_dist = f.dist;
-                        l.noise_op = f.op;
-                        l.noise_a = f.a; l.noise_b = f.b; l.noise_val = f.val;
-                        l.noise_seed = ls->layer->id;
-                        l.noise_step = (uint64_t)it;
+                    case Model::GAME_OF_LIFE: {
+                        auto gf = gol_fast.find(ls);
+                        if (gf != gol_fast.end()) {
+                            uint32_t *spare = gol_spare[ls];
+                            float k9[9];
+                            std::memcpy(k9, gf->second->arr("weights").f(), 36);
+                            kt.time("gol_step", [&] {
+                                check_hip(sg_hip_gol_step(
+                                              ls->output.du(), spare,
+                                              last ? ls->input.df() : nullptr,
+                                              ls->layer->rows, ls->layer->cols,
+                                              gf->second->conn->arbor.wrap, k9,
+                                              ls->survival_min, ls->survival_max,
+                                              ls->birth_min, ls->birth_max,
+                                              stream),
+                                          "gol step");
+                            });
+                            gol_spare[ls] = ls->output.du();
+                            ls->output.dev = spare;
+                        } else {
+                            check_hip(sg_hip_gol_neuron(
+                                          ls->input.df(), ls->output.du(), n,
+                                          ls->history, ls->survival_min,
+                                          ls->survival_max, ls->birth_min,
+                                          ls->birth_max, stream),
+                                      "gol neuron");
+                        }
+                        break;
                     }
-                    iz_layers.push_back(l);
-                    break;
+                    case Model::RATE_ENCODING:
+                        check_hip(sg_hip_rate_neuron(ls->input.df(),
+                                                     ls->output.df(), n,
+                                                     ls->history, stream),
+                                  "rate neuron");
+                        break;
+                    case Model::RELAY:
+                        check_hip(sg_hip_relay_neuron(ls->input.df(),
+                                                      ls->output.df(), n,
+                                                      ls->history, ls->ramp,
+                                                      stream),
+                                  "relay neuron");
+                        break;
                 }
-                case Model::GAME_OF_LIFE: {
-                    auto gf = gol_fast.find(ls);
-                    if (gf != gol_fast.end()) {
-                        uint32_t *spare = gol_spare[ls];
-                        float k9[9];
-                        std::memcpy(k9, gf->second->arr("weights").f(), 36);
-                        kt.time("gol_step", [&] {
-                            check_hip(sg_hip_gol_step(
-                                          ls->output.du(), spare,
-                                          last ? ls->input.df() : nullptr,
-                                          ls->layer->rows, ls->layer->cols,
-                                          gf->second->conn->arbor.wrap, k9,
-                                          ls->survival_min, ls->survival_max,
-                                          ls->birth_min, ls->birth_max,
-                                          stream),
-                                      "gol step");
-                        });
-                        gol_spare[ls] = ls->output.du();
-                        ls->output.dev = spare;
-                    } else {
-                        check_hip(sg_hip_gol_neuron(
-                                      ls->input.df(), ls->output.du(), n,
-                                      ls->history, ls->survival_min,
-                                      ls->survival_max, ls->birth_min,
-                                      ls->birth_max, stream),
-                                  "gol neuron");
-                    }
-                    break;
+            }
+            if (!iz_layers.empty()) {
+                kt.time("iz_neuron", [&] {
+                    check_hip(sg_hip_iz_neurons(iz_layers.data(),
+                                                (int)iz_layers.size(), stream),
+                              "iz neuron");
+                });
+                for (ConnState *cs : stepped) cs->tss_cur = 1 - cs->tss_cur;
+            }
+        };
+        for (size_t si = 0; si < net_->structures().size(); ++si) {
+            const Structure &st = *net_->structures()[si];
+            if (st.cluster == CLUSTER_PARALLEL) {
+                for (auto &l : st.layers) {
+                    LayerState *ls = state_->layer_state(l.get());
+                    if (!is_input(ls)) continue;
+                    // Only an "input" key makes the layer's input register
+                    // the environment's (cluster_node.cpp:20-44); auxiliary
+                    // input keys leave it to the usual clear.
+                    const auto ik = in_keys.find(ls);
+                    init_layer(ls, ik == in_keys.end() ||
+                                   !ik->second.count("input"), it);
+                    for (auto *c : ls->layer->inputs)
+                        activate(*state_->conn_state(c), stream);
                 }
-                case Model::RATE_ENCODING:
-                    check_hip(sg_hip_rate_neuron(ls->input.df(),
-                                                 ls->output.df(), n,
-                                                 ls->history, stream),
-                              "rate neuron");
-                    break;
-                case Model::RELAY:
-                    check_hip(sg_hip_relay_neuron(ls->input.df(),
-                                                  ls->output.df(), n,
-                                                  ls->history, ls->ramp,
-                                                  stream),
-                              "relay neuron");
-                    break;
+                continue;
+            }
+            for (Layer *l : seq_order[si]) {
+                LayerState *ls = state_->layer_state(l);
+                if (is_input(ls)) {
+                    const auto ik = in_keys.find(ls);
+                    init_layer(ls, ik == in_keys.end() ||
+                                   !ik->second.count("input"), it);
+                } else if (!init_fold.count(ls)) {
+                    init_layer(ls, true, it);
+                }
+                for (auto *c : ls->layer->inputs)
+                    activate(*state_->conn_state(c), stream);
+                update_neurons({ls});
             }
         }
-        if (!iz_layers.empty()) {
-            kt.time("iz_neuron", [&] {
-                check_hip(sg_hip_iz_neurons(iz_layers.data(),
-                                            (int)iz_layers.size(), stream),
-                          "iz neuron");
-            });
-            for (ConnState *cs : tss_folded) cs->tss_cur = 1 - cs->tss_cur;
+        // 4. neuron updates of the parallel structures (every Izhikevich
+        // layer in one launch)
+        {
+            std::vector<LayerState *> par;
+            for (auto &lsp : layers)
+                if (!in_seq.count(lsp->layer)) par.push_back(lsp.get());
+            update_neurons(par);
         }
         // 5. weight updates
         if (learning) {

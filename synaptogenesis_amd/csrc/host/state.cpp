@@ -79,6 +79,12 @@ State::State(Network *net) : net_(net) {
     for (auto *l : net->layers()) build_layer(l);
     for (auto *l : net->layers())
         for (auto *c : l->inputs) build_connection(c);
+    // Index arrays needed only while weights and delays are initialised are
+    // dropped once every connection is built, so they are not part of a
+    // state file (state.cpp:108-119, weight_matrix.cpp:345-353).
+    for (auto &cs : conns_)
+        for (const char *k : {"used", "to row indices", "to column indices"})
+            cs->arrays.erase(k);
 }
 
 State::~State() {
@@ -743,13 +749,15 @@ void State::upload_conn_derived(ConnState &cs) {
         ++visits[r * c.from->cols + col];
     });
     const bool tiled = conn_tiled(cs);
-    const size_t nw = tiled ? tiled_elems(cs.num_dest, cs.k_per_dest)
+    // Tiled types: the record's visit-group-major order (see sg_hip.h).
+    const long tiles = (cs.num_dest + 63) / 64;
+    const size_t nw = tiled ? tiled_elems(cs.num_dest,
+                                          (cs.k_per_dest + 7) / 8 * 8)
                             : (size_t)c.num_weights;
     std::vector<uint8_t> dly(nw, 0);
     std::vector<uint8_t> rev((size_t)from_n * 32, 0);
     std::vector<int32_t> seen(from_n, 0);
     const int32_t *dl = cs.arr("delays").i();
-    const int K = cs.k_per_dest;
     enumerate_synapses(c, cs.num_dest, [&](int slot, int, int k, int r,
                                            int col, int, int, long wi) {
         const int f = r * c.from->cols + col;
@@ -757,8 +765,10 @@ void State::upload_conn_derived(ConnState &cs) {
         const int back = visits[f] - j;    // visits of f after this one
         const uint8_t d = (uint8_t)(dl[wi] & 31);
         if (back < 32) rev[(size_t)f * 32 + back] = d;
-        const long di = tiled ? ((long)(slot >> 6) * K + k) * 64 + (slot & 63)
-                              : wi;
+        const long di = tiled
+            ? (((long)(k >> 3) * tiles + (slot >> 6)) * 8 + (k & 7)) * 64 +
+                  (slot & 63)
+            : wi;
         dly[di] = d;
     });
     std::vector<uint8_t> vcap(from_n);

@@ -424,6 +424,126 @@ def relay_ramp(steps=15, workload=114, name="relay_ramp"):
                               ("ramp", "o", "output")])
 
 
+def seq_chain(steps=20, workload=115, name="seq_chain", cluster="sequential"):
+    """A relay -> rate -> relay chain in a sequential (or feedforward)
+    structure: each layer activates and updates in turn, so a spike of the
+    input reaches the last layer in the same step
+    (sequential_cluster.cpp:100-108).  With the sequential cluster a
+    recurrent connection o -> h (left out of the ordering) feeds back with
+    the one-step lag."""
+    st = Stream(workload)
+    L = [layer("in", "relay", 6, 6), layer("h", "rate_encoding", 1, 20),
+         layer("o", "relay", 1, 5)]
+    C = [conn("in_h", "in", "h", "fully connected"),
+         conn("h_o", "h", "o", "fully connected", delay=1),
+         conn("in_o", "in", "o", "convergent", "sub",
+              arbor={"field size": 3, "stride": 1})]
+    if cluster == "sequential":
+        C.append(conn("o_h", "o", "h", "fully connected", recurrent=True))
+    # the 5x1 "o" layer reading the 6x6 input through a 3x3 convergent
+    # field: columns 0..4 of row 0
+    weights = {c["name"]: st.normal(num_weights(c, L), stream=10 + i,
+                                    std=0.6) for i, c in enumerate(C)}
+    inputs = {("seq", "in"): lambda t: st.normal(36, step=t + 1, stream=2)}
+    return Case(name, net("seq", L, C, cluster=cluster), steps,
+                weights=weights, inputs=inputs,
+                capture=[("seq", "h", "output"), ("seq", "o", "output"),
+                         ("seq", "o", "input")],
+                final_neuron=[("seq", l, "output") for l in ("h", "o")])
+
+
+def seq_iz(steps=50, workload=116, name="seq_iz"):
+    """Izhikevich E/I layers in a sequential structure: E -> I convergent
+    (direct) activates after E has updated in the same step; the recurrent
+    I -> E divergent return (sub, direct) and the plastic E -> E
+    conductance self-connection read last step's spikes."""
+    st = Stream(workload)
+    L = [layer("E", "izhikevich", 10, 12, params="regular"),
+         layer("I", "izhikevich", 5, 6, params="fast")]
+    C = [conn("EE", "E", "E", "convergent", plastic=True,
+              arbor={"field size": 5, "stride": 1, "wrap": True},
+              learning_rate=0.05),
+         conn("EI", "E", "I", "convergent",
+              arbor={"field size": 5, "stride": 2, "wrap": True},
+              direct=True, myelinated=True),
+         conn("IE", "I", "E", "divergent", "sub",
+              arbor={"field size": 4, "stride": 2, "wrap": True},
+              direct=True, myelinated=True, recurrent=True)]
+    weights = {c["name"]: st.uniform(num_weights(c, L), stream=10 + i,
+                                     low=0.0, high=0.8)
+               for i, c in enumerate(C)}
+    delays = {"EE": st.integers(num_weights(C[0], L), 0, 31, stream=20)}
+    init = {}
+    for lname, n in (("E", 120), ("I", 30)):
+        init[("seq", lname, "voltage")] = np.full(n, -65.0, np.float32)
+        init[("seq", lname, "recovery")] = np.full(
+            n, np.float32(0.2) * np.float32(-65.0), np.float32)
+    inputs = {("seq", "E"): lambda t: st.normal(120, step=t + 1, stream=3,
+                                                mean=3.0, std=5.0)}
+    return Case(name, net("seq", L, C, cluster="sequential"), steps,
+                weights=weights, delays=delays, init=init, inputs=inputs,
+                capture=[("seq", "E", "output"), ("seq", "I", "output"),
+                         ("seq", "I", "voltage")],
+                final_neuron=[("seq", l, k) for l in ("E", "I")
+                              for k in ("voltage", "recovery", "output",
+                                        "ampa", "nmda", "post trace")],
+                final_conn=[("EE", k) for k in
+                            ("weights", "short term R", "short term w",
+                             "weight derivatives", "presyn trace",
+                             "time since spike")])
+
+
+def mixed_clusters(steps=40, workload=117, name="mixed_clusters"):
+    """Two structures: a parallel Izhikevich pair and a sequential readout
+    (relay -> rate) fed from it; the engine runs each phase structure by
+    structure (engine.cpp:231-264)."""
+    st = Stream(workload)
+    A = [layer("E", "izhikevich", 8, 8, params="regular"),
+         layer("I", "izhikevich", 4, 4, params="fast")]
+    B = [layer("r", "relay", 1, 16), layer("q", "rate_encoding", 1, 6)]
+    C = [conn("EI", "E", "I", "convergent",
+              arbor={"field size": 3, "stride": 2, "wrap": True},
+              direct=True, myelinated=True),
+         conn("IE", "I", "E", "fully connected", "sub", direct=True,
+              myelinated=True),
+         conn("Er", "E", "r", "fully connected"),
+         conn("rq", "r", "q", "fully connected"),
+         conn("Iq", "I", "q", "fully connected", "sub")]
+    C[2]["from structure"] = "par"
+    C[2]["to structure"] = "seq"
+    C[3]["from structure"] = C[3]["to structure"] = "seq"
+    C[4]["from structure"] = "par"
+    C[4]["to structure"] = "seq"
+    for c in C[:2]:
+        c["from structure"] = c["to structure"] = "par"
+    sizes = {"E": 64, "I": 16, "r": 16, "q": 6}
+    nw = {"EI": 16 * 9, "IE": 16 * 64, "Er": 64 * 16, "rq": 16 * 6,
+          "Iq": 16 * 6}
+    weights = {k: st.uniform(n, stream=10 + i, low=0.0, high=0.6)
+               for i, (k, n) in enumerate(nw.items())}
+    init = {}
+    for lname, n in (("E", 64), ("I", 16)):
+        init[("par", lname, "voltage")] = np.full(n, -65.0, np.float32)
+        init[("par", lname, "recovery")] = np.full(
+            n, np.float32(0.2) * np.float32(-65.0), np.float32)
+    inputs = {("par", "E"): lambda t: st.normal(64, step=t + 1, stream=3,
+                                                mean=4.0, std=5.0)}
+    n = {"structures": [{"name": "par", "type": "parallel", "layers": A},
+                        {"name": "seq", "type": "sequential", "layers": B}],
+         "connections": C}
+    return Case(name, n, steps, weights=weights, init=init, inputs=inputs,
+                capture=[("par", "E", "output"), ("seq", "r", "output"),
+                         ("seq", "q", "output")],
+                final_neuron=[("seq", "q", "output"),
+                              ("par", "I", "voltage")])
+
+
+def cluster_cases():
+    return [seq_chain(), seq_chain(name="ff_chain", cluster="feedforward",
+                                   workload=118),
+            seq_iz(), mixed_clusters()]
+
+
 def all_cases():
     return [conn_zoo(), gol("gol_small", 24, 20, 80, 102),
             gol("gol_conway", 20, 24, 60, 103, conway_init=True),
@@ -432,4 +552,4 @@ def all_cases():
             iz_sheet(24, 32, 80, "iz_sheet_hot", noise=(2.0, 5.0),
                      workload=107),
             iz_ops(), rate(), iz_init(), history(), gol_rule(),
-            relay_ramp()] + ragged_cases()
+            relay_ramp()] + ragged_cases() + cluster_cases()

@@ -197,10 +197,19 @@ REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libsyngen_ref.so")
 
 
 @pytest.fixture(scope="module")
-def ref_lib():
+def ref_lib(tmp_path_factory):
+    """A private copy of the reference's serial build.  Its pointer stash
+    is process-wide (pointer_stash.h:11-27): after a state is loaded from a
+    file, later states whose layers and connections carry the same ids
+    are handed the stashed arrays (state.cpp:120-136).  A copy under its
+    own path is a separate instance, so the files loaded here cannot leak
+    into other tests that run the reference."""
     if not os.path.exists(REF_LIB):
         pytest.skip("reference serial build not present (oracle/_ref)")
-    lib = syngen.load_library(REF_LIB)
+    import shutil
+    private = str(tmp_path_factory.mktemp("ref") / "libsyngen_ref_copy.so")
+    shutil.copyfile(REF_LIB, private)
+    lib = syngen.load_library(private)
     lib.set_suppress_output(True)
     lib.set_warnings(False)
     return lib
@@ -223,6 +232,36 @@ def _state_arrays(network, net):
             out[(c.name, k)] = harness._view(
                 network.get_weight_matrix(c.name, k), k)
     return out
+
+
+def _records(path):
+    """(id, hash(key), bytes) of every record of a state file, in order."""
+    data = open(path, "rb").read()
+    out, off = [], 0
+    while off < len(data):
+        key = tuple(int(x) for x in np.frombuffer(data, np.uint64, 3, off))
+        out.append(key)
+        off += 24 + key[2]
+    assert off == len(data)
+    return out
+
+
+@pytest.mark.parametrize("name", ["iz_ops", "conn_zoo", "iz_sheet", "gol_small",
+                                  "rate", "seq_iz"])
+def test_state_file_records_match_reference(lib, ref_lib, tmp_path, name):
+    """A freshly built state saves the same records, in the same order and
+    sizes, as the reference's: index arrays used only during construction
+    are dropped (state.cpp:108-119), empty arrays are not written."""
+    case = [c for c in netdefs.all_cases() if c.name == name][0]
+    recs = {}
+    for tag, l in (("ref", ref_lib), ("here", lib)):
+        network = syngen.Network(case.net, l)
+        assert network.build_state()
+        path = str(tmp_path / (tag + ".bin"))
+        assert network.save_state(path)
+        recs[tag] = _records(path)
+        network.free()
+    assert recs["here"] == recs["ref"]
 
 
 @pytest.mark.parametrize("direction", ["reference_to_here", "here_to_reference"])

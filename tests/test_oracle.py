@@ -237,3 +237,41 @@ def test_philox_known_answers():
     for ctr, key, want in kat:
         got = philox4x32(*ctr, *key)
         assert tuple(int(x) for x in got) == want
+
+
+@pytest.mark.parametrize("case", ["seq_chain", "seq_iz"])
+def test_planted_reversed_sequential_order_is_caught(monkeypatch, case):
+    """Layer order is what a sequential cluster adds: running the layers
+    backwards must fail the reference's fixture."""
+    real = stepper.sequential_order
+    monkeypatch.setattr(stepper, "sequential_order",
+                        lambda layers, conns: real(layers, conns)[::-1])
+    c = CASES[case]
+    captures, finals = stepper.run(c)
+    problems = golden_check.compare(c, captures, finals,
+                                    golden_check.load(case))
+    assert problems
+
+
+def test_tolerant_compare_keeps_spike_words_exact():
+    """The tanh tolerance used for rate-encoding cases applies only to the
+    float layers it names: an ulp on a rate output passes, a flipped spike
+    bit in an Izhikevich layer of the same case does not."""
+    c = CASES["mixed_clusters"]
+    captures, finals = stepper.run(c)
+    fx = golden_check.load("mixed_clusters")
+    loose = {("seq", "r"), ("seq", "q")}
+    kw = dict(exact=False, rtol=1e-5, atol=1e-6, loose=loose)
+    assert not golden_check.compare(c, captures, finals, fx, **kw)
+    q = [a.copy() for a in captures[("seq", "q", "output")]]
+    nz = np.flatnonzero(q[3].view(np.float32))
+    q[3].view(np.uint32)[nz[0]] += 1
+    bumped = dict(captures)
+    bumped[("seq", "q", "output")] = q
+    assert golden_check.compare(c, bumped, finals, fx)
+    assert not golden_check.compare(c, bumped, finals, fx, **kw)
+    e = [a.copy() for a in captures[("par", "E", "output")]]
+    e[5].view(np.uint32)[0] ^= np.uint32(1 << 31)
+    flipped = dict(captures)
+    flipped[("par", "E", "output")] = e
+    assert golden_check.compare(c, flipped, finals, fx, **kw)
