@@ -217,16 +217,147 @@ def copy_calibration(mb):
                       "copy_GBps_read_plus_write": 2 * n * 4 / best / 1e6}))
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def bench_ensemble(a):
+    """The batched rate ensemble: 256 three-layer fully connected rate
+    networks, widths uniform in [128, 4096] per layer, BF16 weights N(0,
+    1/sqrt(fan_in)), FP32 state, leak 0.1, tanh; --steps steps (default
+    500) captured into one HIP graph and launched once.  The launch shape
+    is autotuned first.  The same ensemble with FP32 weights (the plain
+    vector-ALU path) is timed beside it, and both are checked against the
+    FP64 model on two instances over the full run."""
+    from synaptogenesis_amd.ensemble import RateEnsemble, synthetic_widths
+    steps = a.steps or 500
+    widths = synthetic_widths()
+    e = RateEnsemble(widths, "bf16", leak=0.1)
+    rt, ks, tune_ms = e.autotune(steps=5)
+    e.run(a.warmup, graph=True)
+    e.run(steps, graph=True)              # captures the timed graph
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.run(steps, graph=True)
+    e.synchronize()
+    dt = time.perf_counter() - t0
+    kernel_ms = e.time_steps(20)
+    c = e.counts()
+    e.close()
+
+    f = RateEnsemble(widths, "fp32", leak=0.1)
+    frt, _, _ = f.autotune(steps=3)
+    fsteps = max(1, steps // 5)
+    f.run(fsteps, graph=True)
+    f.synchronize()
+    t0 = time.perf_counter()
+    f.run(fsteps, graph=True)
+    f.synchronize()
+    fdt = time.perf_counter() - t0
+    fkernel_ms = f.time_steps(10)
+    fc = f.counts()
+    f.close()
+
+    from oracle import ensemble as oe
+    errors = {}
+    total_steps = 100
+    for prec in ("bf16", "fp32"):
+        sub = RateEnsemble(widths[:2], prec, leak=0.1)
+        sub.run(total_steps, graph=True)
+        worst = 0.0
+        for i in range(2):
+            h1, h2 = oe.run_fp64(sub.weights(i, 1), sub.weights(i, 2),
+                                 sub.state(i, 0), total_steps, 0.1, False)
+            worst = max(worst, float(np.abs(sub.state(i, 1) - h1).max()),
+                        float(np.abs(sub.state(i, 2) - h2).max()))
+        sub.close()
+        errors[prec] = worst
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = ensemble_cpu_baseline(widths)
+    syn = c["synapses"]
+    achieved = c["algorithmic_bytes"] / (kernel_ms * 1e-3) / 1e9
+    useful_tflops = 2.0 * syn / (kernel_ms * 1e-3) / 1e12
+    issued_tflops = 32.0 * (c["stored_bytes"] / 2) / (kernel_ms * 1e-3) / 1e12
+    out = {
+        "metric": "synaptic updates/sec (neuron-steps/sec beside it) on 1 "
+                  "MI355X; % HBM roofline",
+        "value": syn * steps / dt,
+        "unit": "synaptic updates/s",
+        "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": dt * 1000.0 / steps,
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic: Philox widths, BF16 N(0,1/sqrt(fan_in)) "
+                "weights and U[0,1) inputs generated on the device",
+        "config": {"workload": "batched rate ensemble: 256 instances, "
+                               "x0 -> h1 -> h2 fully connected, widths "
+                               "U{128..4096}, leak 0.1, tanh, %d steps in "
+                               "one HIP graph launch" % steps,
+                   "synapses_per_step": syn,
+                   "neurons": c["neurons"],
+                   "launch_shape": {"row_tiles": rt, "k_split": ks}},
+        "neuron_steps_per_s": c["neurons"] * steps / dt,
+        "roofline": {"bound": "hbm", "kernel": "ens_bf16_kernel",
+                     "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": None, "avg_ms": kernel_ms},
+        "mfma": {"useful_tflops": useful_tflops,
+                 "issued_tflops": issued_tflops,
+                 "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                 "useful_frac": useful_tflops / MFMA_BF16_PEAK_TFLOPS},
+        "fp32_path": {"value": fc["synapses"] * fsteps / fdt,
+                      "ms_per_step": fdt * 1000.0 / fsteps,
+                      "kernel_avg_ms": fkernel_ms,
+                      "achieved_gbps": fc["algorithmic_bytes"] /
+                                       (fkernel_ms * 1e-3) / 1e9,
+                      "row_groups_per_block": frt,
+                      "bf16_speedup": (syn * steps / dt) /
+                                      (fc["synapses"] * fsteps / fdt)},
+        "fp64_oracle_max_abs_error": {
+            "bf16": errors["bf16"], "fp32": errors["fp32"],
+            "sample": "instances 0-1 over %d steps" % total_steps},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def ensemble_cpu_baseline(widths, seconds=10.0):
+    """The FP64 NumPy model (oracle/ensemble.py), one thread, on the
+    ensemble's instances in order until `seconds` have passed."""
+    from threadpoolctl import threadpool_limits
+    from oracle import ensemble as oe
+    syn, done, t = 0, 0, 0.0
+    with threadpool_limits(1):
+        for i in range(len(widths)):
+            W1 = oe.synthetic_weights(widths, i, 1)
+            W2 = oe.synthetic_weights(widths, i, 2)
+            x0 = oe.synthetic_input(widths, i)
+            t0 = time.perf_counter()
+            oe.run_fp64(W1, W2, x0, 5, 0.1, False)
+            t += time.perf_counter() - t0
+            syn += 5 * (int(widths[i][0]) * int(widths[i][1]) +
+                        int(widths[i][1]) * int(widths[i][2]))
+            done += 1
+            if t >= seconds:
+                break
+    return {"value": syn / t, "unit": "synaptic updates/s", "cores": 1,
+            "kind": "port",
+            "sample": "5 steps of each of the first %d instances in the FP64 "
+                      "NumPy model, one thread (%.1f s)" % (done, t)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 1000; ensemble 500)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="CPU-baseline sample (0: the workload's default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="sheet",
-                    choices=sorted(_workloads()))
+                    choices=sorted(_workloads()) + ["ensemble"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference "
                          "for hardware byte counters) and exit")
@@ -240,6 +371,12 @@ def main():
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
+
+    if a.workload == "ensemble":
+        bench_ensemble(a)
+        return
+    if a.steps is None:
+        a.steps = 1000
 
     spec = _workloads()[a.workload]
     wl = spec["make"]()

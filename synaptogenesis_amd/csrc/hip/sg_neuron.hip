@@ -46,23 +46,6 @@ __device__ __forceinline__ void shift_floats(float *out, int size, int nid,
     out[nid] = v;
 }
 
-// Philox4x32-10 (Salmon et al., SC'11) for the random init configs.
-__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0,
-                                       uint32_t k1) {
-    for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
-        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
-        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
-        c[1] = (uint32_t)p1;
-        c[3] = (uint32_t)p0;
-        c[0] = n0;
-        c[2] = n2;
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-}
-
 // Value of one init config for neuron i at `step` (engine/kernel/
 // kernel.cpp:43-158 distributions, drawn from a counter-based Philox keyed
 // by (seed, step, i) so runs are reproducible): dist 0 uniform[a, b),

@@ -25,7 +25,8 @@ from synaptogenesis_amd import syngen
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 PKG = os.path.join(ROOT, "synaptogenesis_amd")
-HEADERS = {"syngen.h": "libsyngen.so", "sg_hip.h": "libsg_hip.so"}
+HEADERS = {"syngen.h": "libsyngen.so", "sg_hip.h": "libsg_hip.so",
+           "sg_ensemble.h": "libsg_hip.so"}
 
 
 def declared_functions(header):
