@@ -257,24 +257,9 @@ def bench_ensemble(a):
     fc = f.counts()
     f.close()
 
-    from oracle import ensemble as oe
-    errors = {}
-    total_steps = 100
-    for prec in ("bf16", "fp32"):
-        sub = RateEnsemble(widths[:2], prec, leak=0.1)
-        sub.run(total_steps, graph=True)
-        worst = 0.0
-        for i in range(2):
-            h1, h2 = oe.run_fp64(sub.weights(i, 1), sub.weights(i, 2),
-                                 sub.state(i, 0), total_steps, 0.1, False)
-            worst = max(worst, float(np.abs(sub.state(i, 1) - h1).max()),
-                        float(np.abs(sub.state(i, 2) - h2).max()))
-        sub.close()
-        errors[prec] = worst
-
-    cpu = None
+    cpu, errors = None, None
     if not a.no_cpu:
-        cpu = ensemble_cpu_baseline(widths)
+        cpu, errors = ensemble_cpu_leg(widths)
     syn = c["synapses"]
     achieved = c["algorithmic_bytes"] / (kernel_ms * 1e-3) / 1e9
     useful_tflops = 2.0 * syn / (kernel_ms * 1e-3) / 1e12
@@ -314,12 +299,194 @@ def bench_ensemble(a):
                       "row_groups_per_block": frt,
                       "bf16_speedup": (syn * steps / dt) /
                                       (fc["synapses"] * fsteps / fdt)},
-        "fp64_oracle_max_abs_error": {
-            "bf16": errors["bf16"], "fp32": errors["fp32"],
-            "sample": "instances 0-1 over %d steps" % total_steps},
+        "fp64_oracle_max_abs_error": errors,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def snn_flops(cfg):
+    """FLOPs of one forward + backward iteration, recomputation not
+    counted: forward 2 B (n_in n_hid + n_hid n_out) T, backward 2 B (n_in
+    n_hid + 2 n_hid n_out) T."""
+    B, T = cfg.batch, cfg.steps
+    fwd = 2 * B * (cfg.n_in * cfg.n_hid + cfg.n_hid * cfg.n_out) * T
+    bwd = 2 * B * (cfg.n_in * cfg.n_hid + 2 * cfg.n_hid * cfg.n_out) * T
+    return fwd + bwd
+
+
+def torch_fp32_iteration(x, w1, w2, labels, cfg):
+    """The same model as plain PyTorch FP32 operations on the GPU with
+    autograd (the surrogate derivative in a custom Function): the baseline
+    GPU path the hand-written kernels are compared with."""
+    import torch
+
+    class Spike(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, u):
+            ctx.save_for_backward(u)
+            return (u >= 0).float()
+
+        @staticmethod
+        def backward(ctx, g):
+            (u,) = ctx.saved_tensors
+            return g / (1.0 + cfg.slope * u.abs()) ** 2
+
+    xf = x.float()
+    B = cfg.batch
+    v1 = torch.zeros(B, cfg.n_hid, device=x.device)
+    s1 = torch.zeros_like(v1)
+    v2 = torch.zeros(B, cfg.n_out, device=x.device)
+    s2 = torch.zeros_like(v2)
+    count = torch.zeros_like(v2)
+    for t in range(cfg.steps):
+        v1 = cfg.beta * v1 - cfg.reset * cfg.theta * s1 + xf[t] @ w1.t()
+        s1 = Spike.apply(v1 - cfg.theta)
+        v2 = cfg.beta * v2 - cfg.reset * cfg.theta * s2 + s1 @ w2.t()
+        s2 = Spike.apply(v2 - cfg.theta)
+        count = count + s2
+    loss = torch.nn.functional.cross_entropy(count, labels.long())
+    loss.backward()
+    return loss
+
+
+def bench_adjoint(a):
+    """The differentiable leaky network: batch 128, 100 steps, 784 -> 1024
+    -> 10, BF16 weights, spike mode with the fast-sigmoid surrogate (slope
+    25), checkpoints every 10 steps.  One step = one forward + backward
+    iteration through the torch.autograd.Function."""
+    import torch
+    from synaptogenesis_amd import adjoint as adj
+    iters = a.steps or 20
+    cfg = adj.Config.make()
+    x, w1, w2, labels = adj.synthetic_tensors(cfg)
+    w1.requires_grad_(True)
+    w2.requires_grad_(True)
+
+    def iteration():
+        w1.grad = None
+        w2.grad = None
+        loss, _ = adj.snn_loss(x, w1, w2, labels, cfg)
+        loss.backward()
+
+    for _ in range(max(2, min(a.warmup, 5))):
+        iteration()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    for _ in range(iters):
+        iteration()
+    ev[1].record()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = ev[0].elapsed_time(ev[1]) / iters
+    flops = snn_flops(cfg)
+
+    # plain FP32 PyTorch path on the same GPU
+    f1 = w1.detach().to(torch.bfloat16).float().requires_grad_(True)
+    f2 = w2.detach().to(torch.bfloat16).float().requires_grad_(True)
+    torch_fp32_iteration(x, f1, f2, labels, cfg)
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(3):
+        f1.grad = None
+        f2.grad = None
+        torch_fp32_iteration(x, f1, f2, labels, cfg)
+    ev[1].record()
+    torch.cuda.synchronize()
+    fp32_ms = ev[0].elapsed_time(ev[1]) / 3
+
+    cpu, errors = None, None
+    if not a.no_cpu:
+        cpu, errors = adjoint_cpu_leg(cfg, x, w1, w2, labels)
+    achieved = flops / (ms * 1e-3) / 1e12
+    out = {
+        "metric": "training iterations/sec (forward + adjoint backward) on "
+                  "1 MI355X; % MFMA roofline",
+        "value": 1000.0 / ms,
+        "unit": "iterations/s",
+        "n_gpus": 1, "steps": iters, "warmup": max(2, min(a.warmup, 5)),
+        "ms_per_step": ms, "wall_ms_per_step": dt * 1000.0 / iters,
+        "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic: Philox rate-coded inputs, random labels, "
+                "N(0,1/sqrt(fan_in)) weights",
+        "config": {"workload": "surrogate-spike network 784->1024->10, "
+                               "batch 128, 100 steps, checkpoint every 10, "
+                               "fast sigmoid slope 25",
+                   "flops_per_iteration": flops},
+        "roofline": {"bound": "mfma", "kernel": "whole iteration",
+                     "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_BF16_PEAK_TFLOPS,
+                     "traffic": None},
+        "fp32_torch_path": {"ms_per_iteration": fp32_ms,
+                            "speedup": fp32_ms / ms},
+        "fp64_oracle": errors,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def adjoint_cpu_leg(cfg, x, w1, w2, labels):
+    """The CPU leg: the FP64 NumPy forward + adjoint (oracle/snn.py) timed
+    on one thread for one full iteration and, as the checker, the GPU
+    gradient's deviation from it (spike mode: cosine similarity and the
+    share of identical output counts)."""
+    from threadpoolctl import threadpool_limits
+    from synaptogenesis_amd import adjoint as adj
+    from oracle import snn
+    import torch
+    w1.grad = None
+    w2.grad = None
+    loss, counts = adj.snn_loss(x, w1, w2, labels, cfg)
+    loss.backward()
+    g1 = w1.grad.cpu().numpy()
+    g2 = w2.grad.cpu().numpy()
+    ocfg = snn.Config()
+    X = x.float().cpu().numpy()
+    W1 = w1.detach().to(torch.bfloat16).float().cpu().numpy()
+    W2 = w2.detach().to(torch.bfloat16).float().cpu().numpy()
+    L = labels.cpu().numpy()
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        oloss, d1, d2 = snn.adjoint(X, W1, W2, L, ocfg)
+        dt = time.perf_counter() - t0
+    _, ocounts, _ = snn.forward(X, W1, W2, L, ocfg)
+    cos = [float((g * d).sum() / np.linalg.norm(g) / np.linalg.norm(d))
+           for g, d in ((g1, d1), (g2, d2))]
+    errors = {"loss_abs_error": abs(float(loss.detach()) - oloss),
+              "counts_identical": float(np.mean(counts.cpu().numpy() ==
+                                                ocounts)),
+              "dw1_cosine": cos[0], "dw2_cosine": cos[1]}
+    cpu = {"value": 1.0 / dt, "unit": "iterations/s", "cores": 1,
+           "kind": "port",
+           "sample": "one full forward + adjoint iteration of the FP64 "
+                     "NumPy model, one thread (%.1f s)" % dt}
+    return cpu, errors
+
+
+def ensemble_cpu_leg(widths, seconds=10.0, error_steps=100):
+    """The CPU leg: the FP64 NumPy model (oracle/ensemble.py) timed on one
+    thread over the ensemble's instances in order until `seconds` have
+    passed, and, as the checker, the largest deviation of both GPU paths
+    from it on instances 0-1 over `error_steps` steps."""
+    from synaptogenesis_amd.ensemble import RateEnsemble
+    from oracle import ensemble as oe
+    errors = {"sample": "instances 0-1 over %d steps" % error_steps}
+    for prec in ("bf16", "fp32"):
+        sub = RateEnsemble(widths[:2], prec, leak=0.1)
+        sub.run(error_steps, graph=True)
+        worst = 0.0
+        for i in range(2):
+            h1, h2 = oe.run_fp64(sub.weights(i, 1), sub.weights(i, 2),
+                                 sub.state(i, 0), error_steps, 0.1, False)
+            worst = max(worst, float(np.abs(sub.state(i, 1) - h1).max()),
+                        float(np.abs(sub.state(i, 2) - h2).max()))
+        sub.close()
+        errors[prec] = worst
+    return ensemble_cpu_baseline(widths, seconds), errors
 
 
 def ensemble_cpu_baseline(widths, seconds=10.0):
@@ -351,13 +518,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (default 1000; ensemble 500)")
+                    help="timed steps (default 1000; ensemble 500; "
+                         "adjoint 20 iterations)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="CPU-baseline sample (0: the workload's default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="sheet",
-                    choices=sorted(_workloads()) + ["ensemble"])
+                    choices=sorted(_workloads()) + ["ensemble", "adjoint"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference "
                          "for hardware byte counters) and exit")
@@ -374,6 +542,9 @@ def main():
 
     if a.workload == "ensemble":
         bench_ensemble(a)
+        return
+    if a.workload == "adjoint":
+        bench_adjoint(a)
         return
     if a.steps is None:
         a.steps = 1000

@@ -1,0 +1,177 @@
+"""The differentiable two-layer leaky network of include/sg_adjoint.h as a
+torch.autograd.Function.
+
+PyTorch holds the device memory and supplies the stream; the forward pass
+and the adjoint are the library's HIP kernels.  The parameters are FP32
+master weights; each call rounds them to BF16 for the kernels, and the
+gradient returned for them is the FP32 gradient of the loss with respect to
+those BF16 weights.  There is no CPU fallback: the inputs must be on a HIP
+device.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libsg_hip.so")
+
+SPIKE, RATE = 0, 1
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("batch", ctypes.c_int), ("steps", ctypes.c_int),
+                ("n_in", ctypes.c_int), ("n_hid", ctypes.c_int),
+                ("n_out", ctypes.c_int), ("checkpoint", ctypes.c_int),
+                ("mode", ctypes.c_int), ("beta", ctypes.c_float),
+                ("theta", ctypes.c_float), ("slope", ctypes.c_float),
+                ("reset", ctypes.c_float)]
+
+    @classmethod
+    def make(cls, batch=128, steps=100, n_in=784, n_hid=1024, n_out=10,
+             checkpoint=10, mode=SPIKE, beta=0.9, theta=1.0, slope=25.0,
+             reset=None):
+        """reset defaults to 1 (reset by subtraction) for spiking units and
+        0 for rate units."""
+        if reset is None:
+            reset = 1.0 if mode == SPIKE else 0.0
+        return cls(batch, steps, n_in, n_hid, n_out, checkpoint, mode, beta,
+                   theta, slope, reset)
+
+
+_lib = None
+
+
+def library():
+    global _lib
+    if _lib is not None:
+        return _lib
+    L = ctypes.CDLL(LIBRARY)
+    P = ctypes.c_void_p
+    cp = ctypes.POINTER(Config)
+    L.sg_snn_workspace_bytes.argtypes = [cp, ctypes.POINTER(ctypes.c_size_t)]
+    L.sg_snn_workspace_bytes.restype = ctypes.c_int
+    L.sg_snn_forward.argtypes = [cp, P, P, P, P, P, P, P, P]
+    L.sg_snn_forward.restype = ctypes.c_int
+    L.sg_snn_backward.argtypes = [cp, P, P, P, P, P, ctypes.c_float, P, P, P]
+    L.sg_snn_backward.restype = ctypes.c_int
+    L.sg_snn_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("sg_snn: " + library().sg_snn_error().decode())
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def workspace_bytes(cfg):
+    n = ctypes.c_size_t()
+    _check(library().sg_snn_workspace_bytes(ctypes.byref(cfg),
+                                            ctypes.byref(n)))
+    return n.value
+
+
+def _function():
+    import torch
+
+    class SNNLoss(torch.autograd.Function):
+        """loss, counts = SNNLoss.apply(x, w1, w2, labels, cfg)."""
+
+        @staticmethod
+        def forward(ctx, x, w1, w2, labels, cfg):
+            for t in (x, w1, w2, labels):
+                if not t.is_cuda:
+                    raise RuntimeError("sg_snn runs on a HIP device only")
+            assert x.dtype == torch.bfloat16 and labels.dtype == torch.int32
+            assert tuple(x.shape) == (cfg.steps, cfg.batch, cfg.n_in)
+            w1b = w1.detach().to(torch.bfloat16).contiguous()
+            w2b = w2.detach().to(torch.bfloat16).contiguous()
+            x = x.contiguous()
+            labels = labels.contiguous()
+            ws = torch.empty(workspace_bytes(cfg), dtype=torch.uint8,
+                             device=x.device)
+            loss = torch.empty((), dtype=torch.float32, device=x.device)
+            counts = torch.empty(cfg.batch, cfg.n_out, dtype=torch.float32,
+                                 device=x.device)
+            _check(library().sg_snn_forward(
+                ctypes.byref(cfg), _ptr(x), _ptr(w1b), _ptr(w2b),
+                _ptr(labels), _ptr(ws), _ptr(loss), _ptr(counts), _stream()))
+            ctx.save_for_backward(x, w1b, w2b, labels, ws)
+            ctx.cfg = cfg
+            ctx.mark_non_differentiable(counts)
+            return loss, counts
+
+        @staticmethod
+        def backward(ctx, grad_loss, grad_counts):
+            x, w1b, w2b, labels, ws = ctx.saved_tensors
+            cfg = ctx.cfg
+            dw1 = torch.empty(cfg.n_hid, cfg.n_in, dtype=torch.float32,
+                              device=x.device)
+            dw2 = torch.empty(cfg.n_out, cfg.n_hid, dtype=torch.float32,
+                              device=x.device)
+            _check(library().sg_snn_backward(
+                ctypes.byref(cfg), _ptr(x), _ptr(w1b), _ptr(w2b),
+                _ptr(labels), _ptr(ws), ctypes.c_float(1.0), _ptr(dw1),
+                _ptr(dw2), _stream()))
+            return None, dw1 * grad_loss, dw2 * grad_loss, None, None
+
+    return SNNLoss
+
+
+_SNNLoss = None
+
+
+def snn_loss(x, w1, w2, labels, cfg):
+    """Mean cross-entropy of the summed outputs (differentiable in w1, w2)
+    and the counts [B][n_out]."""
+    global _SNNLoss
+    if _SNNLoss is None:
+        _SNNLoss = _function()
+    return _SNNLoss.apply(x, w1, w2, labels, cfg)
+
+
+def synthetic_arrays(cfg, seed=None, workload=5):
+    """Seeded inputs, labels and weights as NumPy arrays.
+
+    Pixel intensities p[b][i] ~ U[0,1) (stream 0, index b * n_in + i); the
+    input spike of step t is (u < p) with u ~ U[0,1) from counter (t,
+    b * n_in + i, 1); labels are uniform integers in [0, n_out) (stream 2);
+    W1 and W2 are normal with standard deviation 1/sqrt(fan_in) (streams 3
+    and 4).  Returns x [T][B][n_in] float32 (0/1), labels int32 and the
+    FP32 weights W1 [n_hid][n_in], W2 [n_out][n_hid].
+    """
+    from .inputs import MASTER_SEED, Stream
+    st = Stream(workload, MASTER_SEED if seed is None else seed)
+    B, T, I = cfg.batch, cfg.steps, cfg.n_in
+    p = st.uniform(B * I, stream=0)
+    x = np.empty((T, B, I), np.float32)
+    for t in range(T):
+        x[t] = (st.uniform(B * I, step=t, stream=1) < p).reshape(B, I)
+    labels = st.integers(B, 0, cfg.n_out - 1, stream=2).astype(np.int32)
+    W1 = st.normal(cfg.n_hid * I, stream=3,
+                   std=1.0 / np.sqrt(I)).reshape(cfg.n_hid, I)
+    W2 = st.normal(cfg.n_out * cfg.n_hid, stream=4,
+                   std=1.0 / np.sqrt(cfg.n_hid)).reshape(cfg.n_out, cfg.n_hid)
+    return x, labels, W1, W2
+
+
+def synthetic_tensors(cfg, device="cuda", seed=None, workload=5):
+    """synthetic_arrays as torch tensors on `device`: x in BF16 (exact for
+    0/1), FP32 master weights, int32 labels."""
+    import torch
+    x, labels, W1, W2 = synthetic_arrays(cfg, seed, workload)
+    return (torch.from_numpy(x).to(device=device, dtype=torch.bfloat16),
+            torch.from_numpy(W1).to(device),
+            torch.from_numpy(W2).to(device),
+            torch.from_numpy(labels).to(device))
