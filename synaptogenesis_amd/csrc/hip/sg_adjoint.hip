@@ -1,18 +1,26 @@
 // Forward pass and adjoint of the two-layer leaky network of
 // include/sg_adjoint.h.
 //
-// Per block of `checkpoint` steps (M = checkpoint * batch rows, row = step
-// * batch + b):
+// Layer 1's recurrence does not read layer 2, so each block of
+// `checkpoint` steps (M = checkpoint * batch rows, row = step * batch + b)
+// runs as parallel passes with only elementwise recurrences in time:
 //   forward   gemm_nt      A1[M][n_hid] = X[M][n_in] . W1^T   (MFMA BF16)
-//             scan_fwd     per batch row: both layers' recurrences over
-//                          the block's steps, W2 . s1 reduced in LDS
-//   backward  gemm_nt + scan_fwd (recomputing the block from its
-//                          checkpoint, recording v and s of every step)
-//             scan_bwd     per batch row, steps in reverse: the adjoint
-//                          states dL/dv1, dL/dv2
-//             gemm_tn      dW1[n_hid][n_in] += Lambda^T . X   (MFMA BF16,
-//                          Lambda as hi + mid + lo BF16 terms)
-//             dw2          dW2[n_out][n_hid] += G2^T . S1
+//             scan1_fwd    one thread per (b, hidden unit): v1, s1 over
+//                          the block's steps
+//             out_current  A2[M][n_out] = S1 . W2^T, one wavefront per row
+//             scan2_fwd    one thread per (b, output): v2, s2, counts
+//   backward  the forward passes again from the block's checkpoint,
+//             recording v1 and v2, then
+//             scan2_bwd    one thread per (b, output), steps in reverse:
+//                          d2 = dL/dv2
+//             scan1_bwd    one thread per (b, hidden unit), steps in
+//                          reverse: d1 = dL/dv1 from W2^T d2
+//             gemm_tn      D1^T . X (MFMA BF16, D1 as hi + mid + lo BF16
+//                          terms), split over row ranges; each range's
+//                          tile accumulates over the blocks of steps
+//             dw2_partial  D2^T . S1 over 16-row chunks, likewise
+// and at the end add_parts / dw2_reduce add the ranges and chunks in
+// order into dW1 and dW2.
 // Every sum runs in a fixed order: results are deterministic.
 
 #include <hip/hip_runtime.h>
@@ -30,10 +38,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kMaxHid = 1024;   // hidden units per batch row (4 per thread)
 constexpr int kMaxOut = 16;
-constexpr int kScanThreads = 256;
-constexpr int kUnits = kMaxHid / kScanThreads;
+constexpr int kTnSplit = 4;     // row ranges of the dW1 product
 
 __device__ __forceinline__ uint32_t bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -114,25 +120,28 @@ __global__ __launch_bounds__(256) void gemm_nt(const uint16_t *__restrict__ A,
         }
 }
 
-// D[N][K] += L[M][N]^T . X[M][K]: FP32 L as three BF16 terms, BF16 X.
-// 64 (n) x 64 (k) tile per block; the M rows are the reduction.
+// part[z][N][K] += L[rows of range z][N]^T . X[rows of range z][K]: FP32 L
+// as three BF16 terms, BF16 X.  64 (n) x 64 (k) tile per block, the M rows
+// split into gridDim.z contiguous ranges.
 __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
                                                const uint16_t *__restrict__ X,
-                                               float *__restrict__ D, int M,
+                                               float *__restrict__ part, int M,
                                                int N, int K) {
     __shared__ __align__(16) uint16_t Ah[3][64][40];
     __shared__ __align__(16) uint16_t Bs[64][40];
     const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
+    const int per = (M + gridDim.z - 1) / gridDim.z;
+    const int rb = blockIdx.z * per, re = min(M, rb + per);
     const int t = threadIdx.x, lr = t >> 3, lc = (t & 7) * 8;
     const int w = t >> 6, lane = t & 63;
     const int wn = (w >> 1) * 32, wk = (w & 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
     f32x4 acc[2][2] = {};
-    for (int r0 = 0; r0 < M; r0 += 32) {
+    for (int r0 = rb; r0 < re; r0 += 32) {
         const int r = r0 + lr;
         for (int e = 0; e < 8; ++e) {
             const int n = n0 + lc + e;
-            const float v = (r < M && n < N) ? L[(long)r * N + n] : 0.0f;
+            const float v = (r < re && n < N) ? L[(long)r * N + n] : 0.0f;
             const uint32_t hi = bf16_rne(v);
             const float rem = v - bf16_value(hi);
             const uint32_t mid = bf16_rne(rem);
@@ -141,7 +150,7 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
             Ah[1][lc + e][lr] = (uint16_t)mid;
             Ah[2][lc + e][lr] = (uint16_t)lo;
         }
-        const u32x4 xv = load8(X, M, K, r, k0 + lc);
+        const u32x4 xv = load8(X, re, K, r, k0 + lc);
         uint16_t xe[8];
         memcpy(xe, &xv, 16);
         for (int e = 0; e < 8; ++e) Bs[lc + e][lr] = xe[e];
@@ -158,14 +167,26 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
             }
         __syncthreads();
     }
+    float *P = part + (long)blockIdx.z * N * K;
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) {
             const int col = k0 + wk + j * 16 + li;
             for (int q = 0; q < 4; ++q) {
                 const int row = n0 + wn + i * 16 + (lane >> 4) * 4 + q;
-                if (row < N && col < K) D[(long)row * K + col] += acc[i][j][q];
+                if (row < N && col < K) P[(long)row * K + col] += acc[i][j][q];
             }
         }
+}
+
+// D[i] = part[0][i] + part[1][i] + ... in range order.
+__global__ void add_parts(const float *__restrict__ part, float *__restrict__ D,
+                          long n, int parts) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x) {
+        float acc = part[i];
+        for (int z = 1; z < parts; ++z) acc += part[(long)z * n + i];
+        D[i] = acc;
+    }
 }
 
 struct ScanArgs {
@@ -175,154 +196,197 @@ struct ScanArgs {
     float reset_theta;           // reset * theta
 };
 
-// Recurrences of one block of steps for batch row blockIdx.x.  state_in /
-// state_out: [B][2H + 2O] (v1, s1, v2, s2); count_mode 0: leave counts,
-// 1: start them at 0, 2: add to them.  rec_*: per-step records or null.
-__global__ __launch_bounds__(kScanThreads) void scan_fwd(
-    ScanArgs a, const float *__restrict__ a1, const uint16_t *__restrict__ w2,
-    const float *__restrict__ state_in, float *__restrict__ state_out,
-    float *__restrict__ counts, int count_mode, float *__restrict__ rec_v1,
-    float *__restrict__ rec_s1, float *__restrict__ rec_v2,
-    float *__restrict__ rec_s2) {
-    extern __shared__ float w2s[];           // [O][H]
-    __shared__ float red[kScanThreads / 64][kMaxOut];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int H = a.H, O = a.O;
-    for (int i = tid; i < O * H; i += kScanThreads) w2s[i] = bf16_value(w2[i]);
-    const float *st = state_in + (long)b * (2 * H + 2 * O);
-    float v1[kUnits], s1[kUnits];
-    for (int q = 0; q < kUnits; ++q) {
-        const int u = tid + q * kScanThreads;
-        v1[q] = u < H ? st[u] : 0.0f;
-        s1[q] = u < H ? st[H + u] : 0.0f;
-    }
-    float v2 = 0.0f, s2 = 0.0f, cnt = 0.0f;
-    if (tid < O) {
-        v2 = st[2 * H + tid];
-        s2 = st[2 * H + O + tid];
-        if (count_mode == 2) cnt = counts[b * O + tid];
-    }
-    __syncthreads();
+// Checkpoint slot: v1 [B][H], s1 [B][H], v2 [B][O], s2 [B][O].
+__host__ __device__ inline long slot_floats(int B, int H, int O) {
+    return 2L * B * H + 2L * B * O;
+}
+
+// Layer 1 over one block of steps; thread = (b, u).  s1 of every step goes
+// to s1_out; rec_v1 (or null) records v1.
+__global__ void scan1_fwd(ScanArgs a, const float *__restrict__ a1,
+                          const float *__restrict__ state_in,
+                          float *__restrict__ state_out,
+                          float *__restrict__ s1_out,
+                          float *__restrict__ rec_v1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = a.H;
+    if (i >= a.B * H) return;
+    const long BH = (long)a.B * H;
+    float v = state_in[i], s = state_in[BH + i];
+    float next = a1[i];
     for (int t = 0; t < a.steps; ++t) {
-        const long row = (long)t * a.B + b;
-        for (int q = 0; q < kUnits; ++q) {
-            const int u = tid + q * kScanThreads;
-            if (u < H) {
-                v1[q] = a.beta * v1[q] - a.reset_theta * s1[q] +
-                        a1[row * H + u];
-                s1[q] = act(v1[q] - a.theta, a.mode, a.slope);
-                if (rec_v1) {
-                    rec_v1[row * H + u] = v1[q];
-                    rec_s1[row * H + u] = s1[q];
-                }
-            }
-        }
-        for (int o = 0; o < O; ++o) {
-            float p = 0.0f;
-            for (int q = 0; q < kUnits; ++q) {
-                const int u = tid + q * kScanThreads;
-                if (u < H) p += w2s[o * H + u] * s1[q];
-            }
-            p = wave_sum(p);
-            if (lane == 0) red[wave][o] = p;
-        }
-        __syncthreads();
-        if (tid < O) {
-            float a2 = red[0][tid];
-            for (int w = 1; w < kScanThreads / 64; ++w) a2 += red[w][tid];
-            v2 = a.beta * v2 - a.reset_theta * s2 + a2;
-            s2 = act(v2 - a.theta, a.mode, a.slope);
-            cnt += s2;
-            if (rec_v2) {
-                rec_v2[row * O + tid] = v2;
-                rec_s2[row * O + tid] = s2;
-            }
-        }
-        __syncthreads();
+        const long idx = (long)t * BH + i;
+        const float in = next;
+        if (t + 1 < a.steps) next = a1[idx + BH];
+        v = a.beta * v - a.reset_theta * s + in;
+        s = act(v - a.theta, a.mode, a.slope);
+        s1_out[idx] = s;
+        if (rec_v1) rec_v1[idx] = v;
     }
     if (state_out) {
-        float *so = state_out + (long)b * (2 * H + 2 * O);
-        for (int q = 0; q < kUnits; ++q) {
-            const int u = tid + q * kScanThreads;
-            if (u < H) {
-                so[u] = v1[q];
-                so[H + u] = s1[q];
-            }
-        }
-        if (tid < O) {
-            so[2 * H + tid] = v2;
-            so[2 * H + O + tid] = s2;
-        }
+        state_out[i] = v;
+        state_out[BH + i] = s;
     }
-    if (count_mode != 0 && tid < O) counts[b * O + tid] = cnt;
 }
 
-// Adjoint of one block of steps, in reverse, for batch row blockIdx.x
-// (r = reset): dL/ds2 = g - r theta d2', d2 = dL/ds2 S'(v2 - theta) +
-// beta d2'; dL/ds1 = W2^T d2 - r theta d1', d1 = dL/ds1 S'(v1 - theta) +
-// beta d1' (primes: the next step's adjoint states, carried between
-// blocks).
-__global__ __launch_bounds__(kScanThreads) void scan_bwd(
-    ScanArgs a, const uint16_t *__restrict__ w2,
-    const float *__restrict__ rec_v1, const float *__restrict__ rec_v2,
-    const float *__restrict__ g, float *__restrict__ carry1,
-    float *__restrict__ carry2, float *__restrict__ lam,
-    float *__restrict__ g2) {
-    extern __shared__ float w2s[];
-    __shared__ float d2s[kMaxOut];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const int H = a.H, O = a.O;
-    for (int i = tid; i < O * H; i += kScanThreads) w2s[i] = bf16_value(w2[i]);
-    float d1n[kUnits];
-    for (int q = 0; q < kUnits; ++q) {
-        const int u = tid + q * kScanThreads;
-        d1n[q] = u < H ? carry1[(long)b * H + u] : 0.0f;
+// A2[row][o] = sum over u of W2[o][u] s1[row][u], one wavefront per row.
+// W2 is staged in LDS transposed, [u][16] BF16 (outputs padded to 16), so
+// a lane reads all outputs of its unit with two 16 B loads; lane partial
+// sums over u = lane + 64 j (j ascending), then a fixed butterfly across
+// the wavefront.
+__global__ __launch_bounds__(256) void out_current(
+    const float *__restrict__ s1, const uint16_t *__restrict__ w2,
+    float *__restrict__ a2, int M, int H, int O) {
+    extern __shared__ __align__(16) uint16_t w2t[];   // [H][16]
+    for (int i = threadIdx.x; i < H * kMaxOut; i += blockDim.x) {
+        const int u = i % H, o = i / H;
+        w2t[u * kMaxOut + o] = o < O ? w2[o * H + u] : (uint16_t)0;
     }
-    float d2n = tid < O ? carry2[b * O + tid] : 0.0f;
-    const float gb = tid < O ? g[b * O + tid] : 0.0f;
     __syncthreads();
-    for (int t = a.steps - 1; t >= 0; --t) {
-        const long row = (long)t * a.B + b;
-        if (tid < O) {
-            const float ds2 = gb - a.reset_theta * d2n;
-            const float d2 = ds2 * act_grad(rec_v2[row * O + tid] - a.theta,
-                                            a.mode, a.slope) + a.beta * d2n;
-            g2[row * O + tid] = d2;
-            d2s[tid] = d2;
-            d2n = d2;
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= M) return;
+    float p[kMaxOut];
+    for (int o = 0; o < kMaxOut; ++o) p[o] = 0.0f;
+    for (int u = lane; u < H; u += 64) {
+        const float sv = s1[(long)row * H + u];
+        const u32x4 lo = *(const u32x4 *)&w2t[u * kMaxOut];
+        const u32x4 hi = *(const u32x4 *)&w2t[u * kMaxOut + 8];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            p[2 * q] += bf16_value(lo[q] & 0xFFFFu) * sv;
+            p[2 * q + 1] += bf16_value(lo[q] >> 16) * sv;
+            p[8 + 2 * q] += bf16_value(hi[q] & 0xFFFFu) * sv;
+            p[9 + 2 * q] += bf16_value(hi[q] >> 16) * sv;
         }
-        __syncthreads();
-        for (int q = 0; q < kUnits; ++q) {
-            const int u = tid + q * kScanThreads;
-            if (u >= H) continue;
-            float gs = 0.0f;
-            for (int o = 0; o < O; ++o) gs += w2s[o * H + u] * d2s[o];
-            const float ds1 = gs - a.reset_theta * d1n[q];
-            const float d1 = ds1 * act_grad(rec_v1[row * H + u] - a.theta,
-                                            a.mode, a.slope) + a.beta * d1n[q];
-            lam[row * H + u] = d1;
-            d1n[q] = d1;
+    }
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o)
+        if (o < O) {
+            const float v = wave_sum(p[o]);
+            if (lane == 0) a2[(long)row * O + o] = v;
         }
-        __syncthreads();
-    }
-    for (int q = 0; q < kUnits; ++q) {
-        const int u = tid + q * kScanThreads;
-        if (u < H) carry1[(long)b * H + u] = d1n[q];
-    }
-    if (tid < O) carry2[b * O + tid] = d2n;
 }
 
-// dW2[o][u] += sum over rows r (ascending) of G2[r][o] * S1[r][u].
-__global__ void dw2_kernel(const float *__restrict__ g2,
-                           const float *__restrict__ s1, float *__restrict__ dw2,
-                           int M, int H, int O) {
+// Layer 2 over one block of steps; thread = (b, o).  count_mode 0: leave
+// counts, 1: start them at 0, 2: add to them.
+__global__ void scan2_fwd(ScanArgs a, const float *__restrict__ a2,
+                          const float *__restrict__ state_in,
+                          float *__restrict__ state_out,
+                          float *__restrict__ counts, int count_mode,
+                          float *__restrict__ rec_v2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int O = a.O;
+    if (i >= a.B * O) return;
+    const long BH = (long)a.B * a.H, BO = (long)a.B * O;
+    float v = state_in[2 * BH + i], s = state_in[2 * BH + BO + i];
+    float cnt = count_mode == 2 ? counts[i] : 0.0f;
+    for (int t = 0; t < a.steps; ++t) {
+        const long idx = (long)t * BO + i;
+        v = a.beta * v - a.reset_theta * s + a2[idx];
+        s = act(v - a.theta, a.mode, a.slope);
+        cnt += s;
+        if (rec_v2) rec_v2[idx] = v;
+    }
+    if (state_out) {
+        state_out[2 * BH + i] = v;
+        state_out[2 * BH + BO + i] = s;
+    }
+    if (count_mode != 0) counts[i] = cnt;
+}
+
+// Adjoint of layer 2 over one block of steps, in reverse; thread = (b, o):
+// d2 = (g - r theta d2') S'(v2 - theta) + beta d2' (d2': the next step's,
+// carried between blocks; r = reset).
+__global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
+                          const float *__restrict__ g,
+                          float *__restrict__ carry2, float *__restrict__ g2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int O = a.O;
+    if (i >= a.B * O) return;
+    const long BO = (long)a.B * O;
+    const float gb = g[i];
+    float dn = carry2[i];
+    for (int t = a.steps - 1; t >= 0; --t) {
+        const long idx = (long)t * BO + i;
+        const float d = (gb - a.reset_theta * dn) *
+                        act_grad(rec_v2[idx] - a.theta, a.mode, a.slope) +
+                        a.beta * dn;
+        g2[idx] = d;
+        dn = d;
+    }
+    carry2[i] = dn;
+}
+
+// Adjoint of layer 1 over one block of steps, in reverse; thread = (b, u):
+// d1 = (sum over o of W2[o][u] d2[o] - r theta d1') S'(v1 - theta) +
+// beta d1'.
+__global__ void scan1_bwd(ScanArgs a, const uint16_t *__restrict__ w2,
+                          const float *__restrict__ rec_v1,
+                          const float *__restrict__ g2,
+                          float *__restrict__ carry1,
+                          float *__restrict__ lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = a.H, O = a.O;
+    if (i >= a.B * H) return;
+    const int b = i / H, u = i % H;
+    const long BH = (long)a.B * H, BO = (long)a.B * O;
+    float wc[kMaxOut];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o)
+        wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
+    float dn = carry1[i];
+    float vnext = rec_v1[(long)(a.steps - 1) * BH + i];
+    for (int t = a.steps - 1; t >= 0; --t) {
+        const long idx = (long)t * BH + i;
+        const float v = vnext;
+        if (t > 0) vnext = rec_v1[idx - BH];
+        const float *d2 = g2 + (long)t * BO + (long)b * O;
+        float gs = 0.0f;
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o)
+            if (o < O) gs += wc[o] * d2[o];
+        const float d = (gs - a.reset_theta * dn) *
+                        act_grad(v - a.theta, a.mode, a.slope) + a.beta * dn;
+        lam[idx] = d;
+        dn = d;
+    }
+    carry1[i] = dn;
+}
+
+constexpr int kDw2Rows = 16;   // rows of one partial sum of dW2
+
+// dW2 partial sums: part[c][o][u] += sum over the rows r of chunk c
+// (ascending) of G2[r][o] * S1[r][u]; every block of steps adds to the
+// same chunk slots, latest block first.
+__global__ void dw2_partial(const float *__restrict__ g2,
+                            const float *__restrict__ s1,
+                            float *__restrict__ part, int M, int H, int O) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (u >= H) return;
+    float acc[kMaxOut];
+    for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
+    const int r1 = min(M, (c + 1) * kDw2Rows);
+    for (int r = c * kDw2Rows; r < r1; ++r) {
+        const float sv = s1[(long)r * H + u];
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o)
+            if (o < O) acc[o] += g2[(long)r * O + o] * sv;
+    }
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o)
+        if (o < O) part[((long)c * O + o) * H + u] += acc[o];
+}
+
+// dW2[o][u] = the chunks' partial sums, in chunk order.
+__global__ void dw2_reduce(const float *__restrict__ part,
+                           float *__restrict__ dw2, int chunks, int H, int O) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= O * H) return;
-    const int o = i / H, u = i % H;
     float acc = 0.0f;
-    for (int r = 0; r < M; ++r) acc += g2[(long)r * O + o] * s1[(long)r * H + u];
-    dw2[i] += acc;
+    for (int c = 0; c < chunks; ++c) acc += part[(long)c * O * H + i];
+    dw2[i] = acc;
 }
 
 // Mean softmax cross-entropy over the batch (one block; sums in batch
@@ -339,7 +403,8 @@ __global__ void loss_kernel(const float *__restrict__ counts,
         float s = 0.0f;
         for (int o = 0; o < O; ++o) s += expf(c[o] - m);
         const int lab = labels[b];
-        lb[b] = (lab >= 0 && lab < O) ? m + logf(s) - c[lab] : __int_as_float(0x7fc00000);
+        lb[b] = (lab >= 0 && lab < O) ? m + logf(s) - c[lab]
+                                      : __int_as_float(0x7fc00000);
         if (counts_out)
             for (int o = 0; o < O; ++o) counts_out[b * O + o] = c[o];
     }
@@ -392,8 +457,8 @@ size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace carve-up, in bytes from the base.
 struct Layout {
-    size_t state, counts, grad, a1, rv1, rs1, rv2, rs2, carry1, carry2, lam,
-        g2, total;
+    size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, lam,
+        g2, part2, part1, total;
 };
 
 Layout layout(const sg_snn_cfg &c) {
@@ -403,18 +468,20 @@ Layout layout(const sg_snn_cfg &c) {
     Layout l;
     size_t off = 0;
     auto take = [&](size_t floats) { size_t o = off; off += al(floats * 4); return o; };
-    l.state = take(slots * B * (2 * H + 2 * O));
+    l.state = take(slots * slot_floats(c.batch, c.n_hid, c.n_out));
     l.counts = take(B * O);
     l.grad = take(B * O);
     l.a1 = take(M * H);
+    l.s1 = take(M * H);
+    l.a2 = take(M * O);
     l.rv1 = take(M * H);
-    l.rs1 = take(M * H);
     l.rv2 = take(M * O);
-    l.rs2 = take(M * O);
     l.carry1 = take(B * H);
     l.carry2 = take(B * O);
     l.lam = take(M * H);
     l.g2 = take(M * O);
+    l.part2 = take((M + kDw2Rows - 1) / kDw2Rows * O * H);
+    l.part1 = take((size_t)kTnSplit * H * c.n_in);
     l.total = off;
     return l;
 }
@@ -424,10 +491,10 @@ int check(const sg_snn_cfg *c) {
     if (c->batch < 1 || c->steps < 1 || c->n_in < 1 || c->n_hid < 1 ||
         c->n_out < 1)
         return fail("sizes must be positive");
-    if (c->n_hid > kMaxHid) return fail("n_hid above 1024");
+    if (c->n_hid > 8192) return fail("n_hid above 8192");
     if (c->n_out > kMaxOut) return fail("n_out above 16");
-    if ((long)c->n_out * c->n_hid > 16000)
-        return fail("n_out * n_hid above 16000 (W2 is staged in LDS)");
+    if (c->n_hid > 2048)
+        return fail("n_hid above 2048 (W2 is staged in LDS)");
     if (c->batch > 16384) return fail("batch above 16384");
     if (c->checkpoint < 1 || c->steps % c->checkpoint != 0)
         return fail("checkpoint must divide steps");
@@ -454,11 +521,28 @@ ScanArgs scan_args(const sg_snn_cfg &c) {
     return a;
 }
 
-void input_current(const sg_snn_cfg &c, const uint16_t *x_block,
-                   const uint16_t *w1, float *a1, hipStream_t s) {
+unsigned blocks(long threads) { return (unsigned)((threads + 255) / 256); }
+
+// The forward passes of one block of steps from checkpoint `in`: state to
+// `out` (or null), counts per count_mode, v records when rec_v1 is set.
+void forward_block(const sg_snn_cfg &c, const ScanArgs &a, const uint16_t *xb,
+                   const uint16_t *w1, const uint16_t *w2, const Layout &l,
+                   unsigned char *ws, const float *in, float *out,
+                   int count_mode, bool record, hipStream_t s) {
     const int M = c.checkpoint * c.batch;
+    float *a1 = (float *)(ws + l.a1), *s1 = (float *)(ws + l.s1);
+    float *a2 = (float *)(ws + l.a2), *cnt = (float *)(ws + l.counts);
     hipLaunchKernelGGL(gemm_nt, dim3((c.n_hid + 63) / 64, (M + 63) / 64),
-                       dim3(256), 0, s, x_block, w1, a1, M, c.n_hid, c.n_in);
+                       dim3(256), 0, s, xb, w1, a1, M, c.n_hid, c.n_in);
+    hipLaunchKernelGGL(scan1_fwd, dim3(blocks((long)c.batch * c.n_hid)),
+                       dim3(256), 0, s, a, a1, in, out, s1,
+                       record ? (float *)(ws + l.rv1) : nullptr);
+    hipLaunchKernelGGL(out_current, dim3((M + 3) / 4), dim3(256),
+                       (size_t)c.n_hid * kMaxOut * 2, s, s1, w2, a2, M,
+                       c.n_hid, c.n_out);
+    hipLaunchKernelGGL(scan2_fwd, dim3(blocks((long)c.batch * c.n_out)),
+                       dim3(256), 0, s, a, a2, in, out, cnt, count_mode,
+                       record ? (float *)(ws + l.rv2) : nullptr);
 }
 
 }  // namespace
@@ -486,19 +570,14 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     unsigned char *ws = (unsigned char *)workspace;
     float *state = (float *)(ws + l.state);
     float *cnt = (float *)(ws + l.counts);
-    float *a1 = (float *)(ws + l.a1);
-    const long slot = (long)c.batch * (2 * c.n_hid + 2 * c.n_out);
+    const long slot = slot_floats(c.batch, c.n_hid, c.n_out);
     const long xblock = (long)c.checkpoint * c.batch * c.n_in;
-    const size_t lds = (size_t)c.n_out * c.n_hid * 4;
     const ScanArgs a = scan_args(c);
     fill(state, slot, 0.0f, s);
-    for (int blk = 0; blk < c.steps / c.checkpoint; ++blk) {
-        input_current(c, x + blk * xblock, w1, a1, s);
-        hipLaunchKernelGGL(scan_fwd, dim3(c.batch), dim3(kScanThreads), lds, s,
-                           a, a1, w2, state + blk * slot,
-                           state + (blk + 1) * slot, cnt, blk == 0 ? 1 : 2,
-                           nullptr, nullptr, nullptr, nullptr);
-    }
+    for (int blk = 0; blk < c.steps / c.checkpoint; ++blk)
+        forward_block(c, a, x + blk * xblock, w1, w2, l, ws,
+                      state + blk * slot, state + (blk + 1) * slot,
+                      blk == 0 ? 1 : 2, false, s);
     hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), c.batch * 4, s, cnt,
                        labels, c.batch, c.n_out, loss, counts);
     const hipError_t e = hipGetLastError();
@@ -520,36 +599,41 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     float *state = (float *)(ws + l.state);
     float *cnt = (float *)(ws + l.counts);
     float *g = (float *)(ws + l.grad);
-    float *a1 = (float *)(ws + l.a1);
-    float *rv1 = (float *)(ws + l.rv1), *rs1 = (float *)(ws + l.rs1);
-    float *rv2 = (float *)(ws + l.rv2), *rs2 = (float *)(ws + l.rs2);
+    float *s1 = (float *)(ws + l.s1);
+    float *rv1 = (float *)(ws + l.rv1), *rv2 = (float *)(ws + l.rv2);
     float *carry1 = (float *)(ws + l.carry1), *carry2 = (float *)(ws + l.carry2);
     float *lam = (float *)(ws + l.lam), *g2 = (float *)(ws + l.g2);
+    float *part2 = (float *)(ws + l.part2), *part1 = (float *)(ws + l.part1);
     const int H = c.n_hid, O = c.n_out, B = c.batch;
     const int M = c.checkpoint * B;
-    const long slot = (long)B * (2 * H + 2 * O);
+    const long slot = slot_floats(B, H, O);
     const long xblock = (long)M * c.n_in;
-    const size_t lds = (size_t)O * H * 4;
     const ScanArgs a = scan_args(c);
     hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, s,
                        cnt, labels, B, O, grad_scale, g);
-    fill(dw1, (long)H * c.n_in, 0.0f, s);
-    fill(dw2, (long)O * H, 0.0f, s);
+    const int chunks = (M + kDw2Rows - 1) / kDw2Rows;
+    fill(part1, (long)kTnSplit * H * c.n_in, 0.0f, s);
+    fill(part2, (long)chunks * O * H, 0.0f, s);
     fill(carry1, (long)B * H, 0.0f, s);
     fill(carry2, (long)B * O, 0.0f, s);
     for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
         const uint16_t *xb = x + blk * xblock;
-        input_current(c, xb, w1, a1, s);
-        hipLaunchKernelGGL(scan_fwd, dim3(B), dim3(kScanThreads), lds, s, a, a1,
-                           w2, state + blk * slot, (float *)nullptr, cnt, 0,
-                           rv1, rs1, rv2, rs2);
-        hipLaunchKernelGGL(scan_bwd, dim3(B), dim3(kScanThreads), lds, s, a, w2,
-                           rv1, rv2, g, carry1, carry2, lam, g2);
-        hipLaunchKernelGGL(gemm_tn, dim3((c.n_in + 63) / 64, (H + 63) / 64),
-                           dim3(256), 0, s, lam, xb, dw1, M, H, c.n_in);
-        hipLaunchKernelGGL(dw2_kernel, dim3((O * H + 255) / 256), dim3(256), 0,
-                           s, g2, rs1, dw2, M, H, O);
+        forward_block(c, a, xb, w1, w2, l, ws, state + blk * slot, nullptr, 0,
+                      true, s);
+        hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)), dim3(256), 0,
+                           s, a, rv2, g, carry2, g2);
+        hipLaunchKernelGGL(scan1_bwd, dim3(blocks((long)B * H)), dim3(256), 0,
+                           s, a, w2, rv1, g2, carry1, lam);
+        hipLaunchKernelGGL(gemm_tn, dim3((c.n_in + 63) / 64, (H + 63) / 64,
+                                         kTnSplit),
+                           dim3(256), 0, s, lam, xb, part1, M, H, c.n_in);
+        hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
+                           dim3(256), 0, s, g2, s1, part2, M, H, O);
     }
+    hipLaunchKernelGGL(add_parts, dim3(1024), dim3(256), 0, s, part1, dw1,
+                       (long)H * c.n_in, kTnSplit);
+    hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 255) / 256), dim3(256), 0, s,
+                       part2, dw2, chunks, H, O);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(std::string("launch: ") + hipGetErrorString(e));
     return 0;

@@ -77,8 +77,10 @@ __global__ __launch_bounds__(64 * RT * KS) void ens_bf16_kernel(
     const Layer *__restrict__ layers, const Item *__restrict__ items,
     int parity, float leak, int rectify) {
     extern __shared__ __align__(16) unsigned char lds[];
+    // Fields are read through the pointer: a local copy of the descriptor
+    // indexed by the run-time parity would live in scratch memory.
     const Item it = items[blockIdx.x];
-    const Layer L = layers[it.layer];
+    const Layer &L = layers[it.layer];
     const int Kp = L.Kp;
     uint16_t *xs = (uint16_t *)lds;   // [3][Kp]
     const float *x = L.x[parity];
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(64 * WAVES) void ens_fp32_kernel(
     int parity, float leak, int rectify) {
     extern __shared__ __align__(16) unsigned char lds[];
     const Item it = items[blockIdx.x];
-    const Layer L = layers[it.layer];
+    const Layer &L = layers[it.layer];
     const int Kp = L.Kp;
     float *xs = (float *)lds;
     const float *x = L.x[parity];

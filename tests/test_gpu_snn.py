@@ -181,7 +181,7 @@ def test_sgd_step_lowers_the_loss():
 @pytest.mark.gpu
 def test_rejects_bad_configurations():
     import torch
-    cfg = adj.Config.make(batch=2, steps=10, n_in=8, n_hid=2048, n_out=4,
+    cfg = adj.Config.make(batch=2, steps=10, n_in=8, n_hid=16, n_out=17,
                           checkpoint=5)
     with pytest.raises(RuntimeError):
         adj.workspace_bytes(cfg)
