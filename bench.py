@@ -59,12 +59,16 @@ def step_bytes(ne, ni):
             ne * 8 + (ne + ne + ni) * 4)
 
 
-def measured_traffic():
-    """HBM bytes per launch of the fused kernel, from the committed rocprofv3
-    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter
-    reads for a plain copy of known size), or None."""
+ENS_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles",
+                                   "ens_bf16_ensemble_traffic.txt")
+
+
+def measured_traffic(path=TRAFFIC_SUMMARY):
+    """HBM bytes per launch of a workload's dominant kernel, from its
+    committed rocprofv3 summary (FETCH_SIZE and WRITE_SIZE, each scaled by
+    what the counter reads for a plain copy of known size), or None."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":")[1])
@@ -286,7 +290,10 @@ def bench_ensemble(a):
         "roofline": {"bound": "hbm", "kernel": "ens_bf16_kernel",
                      "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None, "avg_ms": kernel_ms},
+                     "traffic": measured_traffic(ENS_TRAFFIC_SUMMARY),
+                     "traffic_source": os.path.relpath(ENS_TRAFFIC_SUMMARY,
+                                                       ROOT),
+                     "avg_ms": kernel_ms},
         "mfma": {"useful_tflops": useful_tflops,
                  "issued_tflops": issued_tflops,
                  "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
