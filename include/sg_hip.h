@@ -26,9 +26,11 @@
 extern "C" {
 #endif
 
-/* Connection types (util/constants.h:71-78). */
+/* Connection types (util/constants.h:71-78).  SG_SPARSE is what a
+ * connection with "sparse": true becomes once its zero weights are dropped
+ * (state/weight_matrix.cpp:510-663, network/connection.cpp:219-222). */
 enum { SG_FULLY_CONNECTED = 0, SG_SUBSET = 1, SG_ONE_TO_ONE = 2,
-       SG_CONVERGENT = 3, SG_DIVERGENT = 4 };
+       SG_CONVERGENT = 3, SG_DIVERGENT = 4, SG_SPARSE = 5 };
 
 /* Source output types (util/constants.h:15-19). */
 enum { SG_OUT_FLOAT = 0, SG_OUT_BIT = 1, SG_OUT_INT = 2 };
@@ -41,7 +43,7 @@ enum { SG_ADD = 0, SG_SUB = 1, SG_MULT = 2, SG_DIV = 3, SG_POOL = 4,
 /* One connection as the kernels see it (engine/kernel/synapse_data.h:16-57
  * plus the ArborizedConfig / SubsetConfig it carries). */
 typedef struct sg_conn {
-    int type;            /* SG_FULLY_CONNECTED .. SG_DIVERGENT            */
+    int type;            /* SG_FULLY_CONNECTED .. SG_SPARSE               */
     int opcode;          /* aggregator for the base activator             */
     int out_type;        /* SG_OUT_* of the source layer                  */
     int delay;           /* connection delay                              */
@@ -81,6 +83,10 @@ typedef struct sg_conn {
      * it into the input; the neuron kernel then aggregates it into the
      * layer's init-config value (sg_iz_layer.noise == 2). */
     int store_sum;
+    /* SG_SPARSE: source index of visit k of slot s in device layout
+     * (k_per_dest = the largest nonzero count), -1 past the slot's count
+     * (synapse_kernel.h:305-366 loops over nonzero_counts[to]). */
+    const int32_t *sparse_from;
 } sg_conn;
 
 /* Fields of the interleaved per-synapse record of a tiled Izhikevich

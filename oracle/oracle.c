@@ -30,7 +30,8 @@
 
 /* Connection types and opcodes of this library (names as in the reference's
  * JSON vocabulary; numbering is this file's own). */
-enum { ORC_FC = 0, ORC_SUBSET, ORC_ONE_TO_ONE, ORC_CONVERGENT, ORC_DIVERGENT };
+enum { ORC_FC = 0, ORC_SUBSET, ORC_ONE_TO_ONE, ORC_CONVERGENT, ORC_DIVERGENT,
+       ORC_SPARSE };
 enum {
     ORC_ADD = 0, ORC_SUB, ORC_MULT, ORC_DIV, ORC_POOL, ORC_REWARD, ORC_GAP,
     ORC_ADD_HEAVISIDE, ORC_SUB_HEAVISIDE, ORC_MULT_HEAVISIDE
@@ -49,6 +50,10 @@ typedef struct {
     int field[2], stride[2], spacing[2], offset[2];
     int sub[8];
     long num_weights;
+    /* ORC_SPARSE: source index of entry k of destination `to` at
+     * [to * (num_weights / to_size) + k], -1 past the destination's
+     * nonzero count */
+    const int *sparse_from;
 } orc_geom;
 
 static int wrap_index(int x, int n) { return (x % n + n) % n; }
@@ -80,6 +85,8 @@ static long open_dest(long nd, int *dest, long *first, int to, long nv) {
  *   DIVERGENT        synapse_kernel.h:506-581 (start by C truncating
  *                    division, end = start + spacing*(field-stride)/stride;
  *                    weight offset to*(num_weights/to_size))
+ *   SPARSE           synapse_kernel.h:305-366 (each destination's compacted
+ *                    entries in order; weight to*max_nonzero + k)
  * Returns the number of visits; *ndest receives the number of destinations.
  * first[] gets ndest+1 entries when non-NULL.  Returns -1 for a geometry the
  * reference's loops would divide by zero on.
@@ -182,6 +189,17 @@ long orc_enumerate(const orc_geom *g, int *dest, long *first, int *src,
                         nv = emit(nv, src, widx, r * fc + c, base + k);
                     }
             }
+        break;
+    }
+    case ORC_SPARSE: {
+        const long per_dest = to_size ? g->num_weights / to_size : 0;
+        for (int to = 0; to < to_size; ++to) {
+            nd = open_dest(nd, dest, first, to, nv);
+            for (long k = 0; k < per_dest; ++k) {
+                const int s = g->sparse_from[to * per_dest + k];
+                if (s >= 0) nv = emit(nv, src, widx, s, to * per_dest + k);
+            }
+        }
         break;
     }
     default:

@@ -38,7 +38,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-FC, SUBSET, ONE_TO_ONE, CONVERGENT, DIVERGENT = range(5)
+FC, SUBSET, ONE_TO_ONE, CONVERGENT, DIVERGENT, SPARSE = range(6)
 TYPES = {"fully connected": FC, "subset": SUBSET, "one to one": ONE_TO_ONE,
          "convergent": CONVERGENT, "divergent": DIVERGENT}
 OPCODES = {"add": 0, "sub": 1, "mult": 2, "div": 3, "pool": 4, "reward": 5,
@@ -60,7 +60,8 @@ class Geom(ctypes.Structure):
                 ("convolutional", ctypes.c_int), ("wrap", ctypes.c_int),
                 ("field", ctypes.c_int * 2), ("stride", ctypes.c_int * 2),
                 ("spacing", ctypes.c_int * 2), ("offset", ctypes.c_int * 2),
-                ("sub", ctypes.c_int * 8), ("num_weights", ctypes.c_long)]
+                ("sub", ctypes.c_int * 8), ("num_weights", ctypes.c_long),
+                ("sparse_from", ctypes.c_void_p)]
 
 
 _lib = None
@@ -173,6 +174,9 @@ class Conn:
         self.delay = int(d.get("delay", "0"))
         self.max_weight = stof(d.get("max weight", "1.0"))
         self.convolutional = _bool(d.get("convolutional"), False)
+        self.sparse = _bool(d.get("sparse"), False)
+        if self.sparse and (self.convolutional or self.type == SUBSET):
+            raise ValueError("oracle: sparse " + self.name)
         g = Geom()
         g.type = self.type
         g.from_rows, g.from_cols = self.src.rows, self.src.cols
@@ -221,6 +225,50 @@ class Conn:
                           _p(self.first), _p(self.srcs), _p(self.widx),
                           ctypes.byref(nd))
         self.ndest = nd.value
+
+    def sparsify(self, weights, delays):
+        """WeightMatrix::sparsify (state/weight_matrix.cpp:510-663) and
+        adjust_sparse_indices (:253-343): each destination keeps its
+        visited synapses with a nonzero weight, in order, packed into rows
+        of max_nonzero entries (an all-zero matrix keeps the first entry of
+        every row); wrapped sources are already in the layer, unwrapped
+        ones out of bounds were never visited.  The connection becomes
+        SPARSE.  Returns the compacted weights and delays."""
+        to_n = self.dst.n
+        cols = self.num_weights // to_n
+        rows = []
+        for d in range(self.ndest):
+            to = int(self.dest[d])
+            ent = [(int(self.widx[v]) - to * cols, int(self.srcs[v]))
+                   for v in range(self.first[d], self.first[d + 1])]
+            rows.append((to, sorted(ent)))
+        keep = {to: [(k, s) for k, s in ent if weights[to * cols + k] != 0.0]
+                for to, ent in rows}
+        maxnz = max([len(v) for v in keep.values()] + [0])
+        if maxnz == 0:
+            maxnz = 1
+            keep = {to: [(0, s) for k, s in ent if k == 0]
+                    for to, ent in rows}
+        snw = maxnz * to_n
+        w = np.zeros(snw, np.float32)
+        frm = np.full(snw, -1, np.int32)
+        dl = None if delays is None else np.zeros(snw, np.int32)
+        dense_delays = None if delays is None else \
+            np.asarray(delays, np.int32).reshape(-1)
+        for to, ent in keep.items():
+            for j, (k, s) in enumerate(ent):
+                w[to * maxnz + j] = weights[to * cols + k]
+                frm[to * maxnz + j] = s
+                if dl is not None:
+                    dl[to * maxnz + j] = dense_delays[to * cols + k]
+        self.type = SPARSE
+        self.num_weights = snw
+        self.sparse_from = frm
+        self.geom.type = SPARSE
+        self.geom.num_weights = snw
+        self.geom.sparse_from = frm.ctypes.data
+        self._enumerate()
+        return w, dl
 
     def visit_args(self):
         return [ctypes.c_long(self.ndest), _p(self.dest), _p(self.first),
@@ -415,10 +463,13 @@ class Network:
             _init_layer(layer)
         for c in self.conns:
             c.weights = _init_weights(c, case.weights.get(c.name))
+            delays = case.delays.get(c.name)
+            if c.sparse:
+                c.weights, delays = c.sparsify(c.weights, delays)
             c.iz = c.dst.model == "izhikevich"
             c.direct = c.cfg.get("direct") == "true"
             if c.iz:
-                _init_iz_matrix(c, case.delays.get(c.name))
+                _init_iz_matrix(c, delays)
                 if c.plastic and (c.convolutional or
                                   c.opcode not in (0, 1)):
                     raise ValueError("the reference rejects this updater: " +

@@ -44,9 +44,13 @@ _lib = None
 
 
 def library():
+    """libsg_hip.so with the adjoint entry points typed.  PyTorch is
+    imported first: its bundled HIP runtime has the same soname as the one
+    the library links, and only the first one loaded serves the process."""
     global _lib
     if _lib is not None:
         return _lib
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIBRARY)
     P = ctypes.c_void_p
     cp = ctypes.POINTER(Config)

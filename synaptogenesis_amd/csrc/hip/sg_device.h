@@ -106,6 +106,8 @@ __device__ __forceinline__ int slot_to_index(const sg_conn &c, int slot) {
 //   CONVERGENT  synapse_kernel.h:384-421 (k_index also counts skipped
 //               out-of-bounds kernel cells)
 //   DIVERGENT   synapse_kernel.h:506-550 (C truncating division)
+//   SPARSE      synapse_kernel.h:311-322 (the compacted list; k runs to
+//               the largest count, entries past a slot's own count are -1)
 // weight_index is the device layout index (see tiled_index), or k for a
 // shared convolutional kernel.
 // Per-destination synapse arrays are tiled by 64 destinations: the k-th
@@ -153,6 +155,12 @@ __device__ __forceinline__ void for_each_synapse(const sg_conn &c, int slot,
                 f(k, fr * c.from_cols + col,
                   c.convolutional ? k : tiled_index(slot, k, K));
             }
+        }
+    } else if (TYPE == SG_SPARSE) {
+        for (int k = 0; k < K; ++k) {
+            const long wi = tiled_index(slot, k, K);
+            const int from = c.sparse_from[wi];
+            if (from >= 0) f(k, from, wi);
         }
     } else if (TYPE == SG_DIVERGENT) {
         const int to_row = slot / c.to_cols, to_col = slot % c.to_cols;
@@ -222,6 +230,10 @@ __host__ __device__ inline SynapseCursor cursor(const sg_conn &c, int slot) {
         q.ncols = c.col_field;
         q.count = c.row_field * c.col_field;
         q.rstep = c.row_spacing; q.cstep = c.col_spacing;
+    } else if (TYPE == SG_SPARSE) {
+        q.count = c.k_per_dest;
+        q.row0 = slot; q.col0 = 0; q.ncols = c.k_per_dest;
+        q.rstep = 1; q.cstep = 1;
     } else {  // SG_DIVERGENT
         const int to_row = slot / c.to_cols, to_col = slot % c.to_cols;
         q.row0 = (to_row + c.row_spacing * (-c.row_offset - c.row_field +
@@ -246,6 +258,8 @@ __device__ __forceinline__ int source_at(const sg_conn &c,
                                          const SynapseCursor &q, int k) {
     if (TYPE == SG_FULLY_CONNECTED) return k;
     if (TYPE == SG_ONE_TO_ONE) return q.row0 * c.from_cols + q.col0;
+    if (TYPE == SG_SPARSE)
+        return c.sparse_from[tiled_index(q.row0, k, c.k_per_dest)];
     const int kr = k / q.ncols, kc = k - kr * q.ncols;
     int r = q.row0 + kr * q.rstep, col = q.col0 + kc * q.cstep;
     if (TYPE == SG_SUBSET) return r * c.from_cols + col;
@@ -267,9 +281,11 @@ struct SynapseIter {
     int r, c, c0, rstep, cstep, rows, cols;
     bool wrap;
     long wi, wstep;
+    const int32_t *sf;   // SG_SPARSE: the tiled source indices
 
     __device__ __forceinline__ void init(const sg_conn &cn, int slot) {
         const SynapseCursor q = cursor<TYPE>(cn, slot);
+        sf = cn.sparse_from;
         k = 0; kc = 0;
         count = q.count; ncols = q.ncols;
         rows = cn.from_rows; cols = cn.from_cols;
@@ -299,6 +315,7 @@ struct SynapseIter {
     }
     // Source index of the current visit, or -1 where the loop skips it.
     __device__ __forceinline__ int source() const {
+        if (TYPE == SG_SPARSE) return sf[wi];
         if ((TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) && !wrap &&
             (r < 0 || r >= rows || c < 0 || c >= cols))
             return -1;

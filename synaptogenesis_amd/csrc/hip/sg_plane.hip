@@ -149,6 +149,7 @@ hipError_t launch_plane_base(const sg_conn &c, hipStream_t st) {
         case SG_SUBSET: return launch_plane_t<SG_SUBSET>(c, st);
         case SG_CONVERGENT: return launch_plane_t<SG_CONVERGENT>(c, st);
         case SG_DIVERGENT: return launch_plane_t<SG_DIVERGENT>(c, st);
+        case SG_SPARSE: return launch_plane_t<SG_SPARSE>(c, st);
     }
     return hipErrorInvalidValue;
 }

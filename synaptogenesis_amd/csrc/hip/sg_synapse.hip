@@ -318,6 +318,8 @@ __global__ __launch_bounds__(kBlock) void iz_gap_kernel(sg_conn c,
                               __VA_ARGS__; } break;                        \
         case SG_DIVERGENT: { constexpr int T = SG_DIVERGENT;               \
                              __VA_ARGS__; } break;                         \
+        case SG_SPARSE: { constexpr int T = SG_SPARSE;                     \
+                          __VA_ARGS__; } break;                            \
         default: return hipErrorInvalidValue;                              \
     }
 
@@ -344,7 +346,7 @@ hipError_t launch_iz_tile(const sg_conn &c, const sg_iz_syn &s,
 
 static inline bool tiled_type(int t) {
     return t == SG_FULLY_CONNECTED || t == SG_SUBSET || t == SG_CONVERGENT ||
-           t == SG_DIVERGENT;
+           t == SG_DIVERGENT || t == SG_SPARSE;
 }
 
 }  // namespace sg
@@ -362,6 +364,7 @@ extern "C" int sg_hip_activate_base(const sg_conn *c, void *stream) {
         case SG_ONE_TO_ONE: return launch_base_t<SG_ONE_TO_ONE>(*c, s);
         case SG_CONVERGENT: return launch_base_t<SG_CONVERGENT>(*c, s);
         case SG_DIVERGENT: return launch_base_t<SG_DIVERGENT>(*c, s);
+        case SG_SPARSE: return launch_base_t<SG_SPARSE>(*c, s);
     }
     return hipErrorInvalidValue;
 }

@@ -295,6 +295,7 @@ static hipError_t launch_tile_base_t(const sg_conn &c, hipStream_t st) {
         case SG_SUBSET: SG_TB(SG_SUBSET); break;
         case SG_CONVERGENT: SG_TB(SG_CONVERGENT); break;
         case SG_DIVERGENT: SG_TB(SG_DIVERGENT); break;
+        case SG_SPARSE: SG_TB(SG_SPARSE); break;
         default: return hipErrorInvalidValue;
     }
 #undef SG_TB
@@ -373,6 +374,7 @@ static hipError_t launch_iz_tile_t(const sg_conn &c, const sg_iz_syn &s,
         case SG_SUBSET: SG_IZT(SG_SUBSET); break;
         case SG_CONVERGENT: SG_IZT(SG_CONVERGENT); break;
         case SG_DIVERGENT: SG_IZT(SG_DIVERGENT); break;
+        case SG_SPARSE: SG_IZT(SG_SPARSE); break;
         default: return hipErrorInvalidValue;
     }
 #undef SG_IZ

@@ -415,6 +415,7 @@ Props *Engine::run(const Props &args) {
             d.plane = cs.d_plane;
         }
         d.store_sum = store_sum.count(&cs) ? 1 : 0;
+        d.sparse_from = cs.d_sparse_from;
         return d;
     };
     auto iz_syn = [&](ConnState &cs) {

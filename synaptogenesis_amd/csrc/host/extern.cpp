@@ -340,8 +340,14 @@ ARRAY get_all_devices(void) {
     return ARRAY{n + 1, INT_POINTER, p, true};
 }
 
-void set_suppress_output(bool val) { g_suppress = val; }
-void set_warnings(bool val) { g_warnings = val; }
+void set_suppress_output(bool val) {
+    g_suppress = val;
+    sg::set_host_messages(g_warnings, g_suppress);
+}
+void set_warnings(bool val) {
+    g_warnings = val;
+    sg::set_host_messages(g_warnings, g_suppress);
+}
 void set_debug(bool val) { g_debug = val; }
 
 void interrupt_engine(void) { Engine::interrupt(); }

@@ -54,6 +54,7 @@ int Connection::matrix_rows() const {
 
 int Connection::matrix_columns() const {
     if (type == SG_SUBSET) return subset.from_size();
+    if (type == SG_SPARSE) return num_weights / to->size;
     if (type == SG_CONVERGENT || type == SG_DIVERGENT) return arbor.total();
     if (type == SG_ONE_TO_ONE) return 1;
     return from->size;
@@ -202,8 +203,7 @@ void Network::add_connection(const Props &cc) {
     c->max_weight = cc.get_float("max weight", 1.0f);
     c->type = connection_type_from(cc.get("type", "fully connected"));
     c->opcode = opcode_from(cc.get("opcode", "add"));
-    c->sparse = cc.get_bool("sparse", false) ||
-                cc.get_bool("randomized projection", false);
+    c->sparse = cc.get_bool("sparse", false);
     c->convolutional = cc.get_bool("convolutional", false);
     c->recurrent = cc.get_bool("recurrent", false);
     if (cc.get("dendrite", "root") != "root")
@@ -211,9 +211,14 @@ void Network::add_connection(const Props &cc) {
     if (c->delay < 0)
         throw Error("Attempted to construct ConnectionConfig with negative "
                     "delay!");
-    if (c->sparse)
-        throw Error("Sparse and randomized-projection connections are not "
-                    "supported");
+    if (cc.get_bool("randomized projection", false))
+        throw Error("Randomized-projection connections are not supported");
+    if (c->sparse && c->convolutional)   // connection_config.cpp:234-236
+        throw Error("Convolutional connections cannot be sparse!");
+    if (c->sparse && c->type == SG_SUBSET)
+        throw Error("Sparse SUBSET connections are not supported (the "
+                    "sparse matrix indexes subset rows as layer "
+                    "destinations)");
     if (c->convolutional && c->type != SG_CONVERGENT &&
         c->type != SG_DIVERGENT)
         throw Error("Only convergent/divergent connections can be "
@@ -258,6 +263,8 @@ void Network::add_connection(const Props &cc) {
             break;
         }
     }
+    c->dense_type = c->type;
+    c->dense_num_weights = c->num_weights;
     c->id = std::hash<std::string>()(c->str());
     from->outputs.push_back(c.get());
     to->inputs.push_back(c.get());

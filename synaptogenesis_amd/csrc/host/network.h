@@ -67,6 +67,10 @@ struct Connection {
     ArborConfig arbor;
     SubsetConfig subset;
     int num_weights = 0;
+    // A sparse connection becomes SG_SPARSE with fewer weights when a state
+    // is built (network/connection.cpp:219-222); the dense type and size
+    // are kept so that a later state starts from them again.
+    int dense_type = SG_FULLY_CONNECTED, dense_num_weights = 0;
     size_t id = 0;
 
     std::string param(const std::string &k, const std::string &def) const {

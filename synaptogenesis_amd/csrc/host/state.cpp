@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -14,6 +15,18 @@
 #include "callbacks.h"
 
 namespace sg {
+
+static bool g_host_warnings = true, g_host_suppress = false;
+
+void set_host_messages(bool warnings, bool suppress) {
+    g_host_warnings = warnings;
+    g_host_suppress = suppress;
+}
+
+void warn(const std::string &msg) {
+    if (g_host_warnings && !g_host_suppress)
+        fprintf(stderr, "%s\n", msg.c_str());
+}
 
 void check_hip(int code, const char *what) {
     if (code != 0)
@@ -79,6 +92,10 @@ State::State(Network *net) : net_(net) {
     for (auto *l : net->layers()) build_layer(l);
     for (auto *l : net->layers())
         for (auto *c : l->inputs) build_connection(c);
+    // Sparse wrapped indices are remapped only now, after every delay was
+    // initialised from the pre-wrap ones (state.cpp:108-115).
+    for (auto &cs : conns_)
+        if (cs->conn->type == SG_SPARSE) adjust_sparse_indices(*cs);
     // Index arrays needed only while weights and delays are initialised are
     // dropped once every connection is built, so they are not part of a
     // state file (state.cpp:108-119, weight_matrix.cpp:345-353).
@@ -104,6 +121,7 @@ State::~State() {
         free_dev(cs->d_tss[0]);
         free_dev(cs->d_tss[1]);
         free_dev(cs->d_plane);
+        free_dev(cs->d_sparse_from);
     }
     free_dev(scratch_);
     free_dev(d_tables_);
@@ -190,6 +208,10 @@ ConnState *State::conn_state(Connection *c) const {
 // Connections
 
 void State::build_connection(Connection *c) {
+    if (c->sparse) {
+        c->type = c->dense_type;
+        c->num_weights = c->dense_num_weights;
+    }
     std::unique_ptr<ConnState> cs(new ConnState());
     cs->conn = c;
     cs->from = layer_state(c->from);
@@ -249,14 +271,128 @@ void State::build_connection(Connection *c) {
     }
 
     init_weights(s);
+    if (c->sparse) sparsify(s);
     if (s.iz) {
+        const int n = c->num_weights;
         for (const char *k : {"short trace", "long trace", "presyn trace",
                               "short term R", "short term w",
                               "weight derivatives"})
-            s.arrays[k] = make_array(k, Elem::F32, nw);
+            s.arrays[k] = make_array(k, Elem::F32, n);
         s.arrays["time since spike"] =
-            make_array("time since spike", Elem::I32, nw);
+            make_array("time since spike", Elem::I32, n);
         process_iz_matrix(s);
+    }
+}
+
+// WeightMatrix::sparsify (weight_matrix.cpp:510-663): zero weights become
+// unused; each destination row keeps its used entries in order, packed to
+// the front of a row of max_nonzero entries (weights padded with 0, index
+// arrays with -1, distances and delays with 0), nonzero_counts per row.
+// The connection turns SG_SPARSE (connection.cpp:219-222).
+void State::sparsify(ConnState &cs) {
+    Connection &c = *cs.conn;
+    compute_indices(cs);
+    const int rows = c.to->size;
+    const int cols = c.num_weights / rows;
+    const float *w = cs.arr("weights").f();
+    int32_t *used = cs.arr("used").i();
+    int maxnz = 0;
+    for (int r = 0; r < rows; ++r) {
+        int nz = 0;
+        for (int k = 0; k < cols; ++k) {
+            const long i = (long)r * cols + k;
+            if (w[i] == 0.0f) used[i] = 0;
+            nz += used[i];
+        }
+        maxnz = std::max(maxnz, nz);
+    }
+    if (maxnz == 0) {
+        warn("Warning in weight config for " + c.str() + ":\n" +
+             "    Attempted to sparsify empty matrix!");
+        maxnz = 1;
+        for (int r = 0; r < rows; ++r) used[(long)r * cols] = 1;
+    }
+    const long snw = (long)maxnz * rows;
+    Array counts = make_array("nonzero counts", Elem::I32, rows, 0);
+    int32_t *cnt = counts.i();
+    auto compact = [&](const char *key, uint32_t pad) {
+        if (!cs.has(key)) return;
+        Array &a = cs.arr(key);
+        Array out = make_array(key, a.elem, snw, pad);
+        for (int r = 0; r < rows; ++r) {
+            int n = 0;
+            for (int k = 0; k < cols; ++k) {
+                const long i = (long)r * cols + k;
+                if (used[i]) out.host[(long)r * maxnz + n++] = a.host[i];
+            }
+        }
+        a = std::move(out);
+    };
+    for (int r = 0; r < rows; ++r) {
+        int n = 0;
+        for (int k = 0; k < cols; ++k) n += used[(long)r * cols + k];
+        cnt[r] = n;
+    }
+    compact("weights", 0u);
+    for (const char *k : {"from row indices", "from column indices",
+                          "from indices", "to row indices",
+                          "to column indices", "to indices"})
+        compact(k, (uint32_t)-1);
+    compact("distances", 0u);
+    compact("delays", 0u);
+    compact("used", 0u);
+    cs.arrays["nonzero counts"] = std::move(counts);
+    c.num_weights = (int)snw;
+    c.type = SG_SPARSE;
+    cs.num_dest = rows;
+    cs.k_per_dest = maxnz;
+}
+
+// WeightMatrix::adjust_sparse_indices (weight_matrix.cpp:253-343): wrapped
+// arborized sources are remapped into the layer; without wrap, sources out
+// of bounds become unused and each row's index entries shift left over
+// them (the weights stay where they are, as in the reference).
+void State::adjust_sparse_indices(ConnState &cs) {
+    Connection &c = *cs.conn;
+    const int rows = c.to->size, maxnz = cs.k_per_dest;
+    const int fr_n = c.from->rows, fc_n = c.from->cols;
+    const bool wrap = c.arbor.wrap;
+    int32_t *fr = cs.arr("from row indices").i();
+    int32_t *fc = cs.arr("from column indices").i();
+    int32_t *fi = cs.arr("from indices").i();
+    int32_t *tr = cs.arr("to row indices").i();
+    int32_t *tc = cs.arr("to column indices").i();
+    int32_t *ti = cs.arr("to indices").i();
+    int32_t *used = cs.arr("used").i();
+    int32_t *cnt = cs.arr("nonzero counts").i();
+    for (int r = 0; r < rows; ++r)
+        for (int k = 0; k < maxnz; ++k) {
+            const long i = (long)r * maxnz + k;
+            if (!used[i]) continue;
+            if (wrap) {
+                fr[i] = (fr[i] % fr_n + fr_n) % fr_n;
+                fc[i] = (fc[i] % fc_n + fc_n) % fc_n;
+                fi[i] = fr[i] * fc_n + fc[i];
+            } else if (fr[i] < 0 || fr[i] >= fr_n || fc[i] < 0 ||
+                       fc[i] >= fc_n) {
+                used[i] = 0;
+            }
+        }
+    if (wrap) return;
+    for (int r = 0; r < rows; ++r) {
+        const long off = (long)r * maxnz;
+        long cur = off, dst = off;
+        for (int k = 0; k < cnt[r]; ++k, ++cur) {
+            if (!used[cur]) continue;
+            if (cur != dst) {
+                fr[dst] = fr[cur]; fc[dst] = fc[cur]; fi[dst] = fi[cur];
+                tr[dst] = tr[cur]; tc[dst] = tc[cur]; ti[dst] = ti[cur];
+                used[dst] = 1;
+                used[cur] = 0;
+            }
+            ++dst;
+        }
+        cnt[r] = (int)(dst - off);
     }
 }
 
@@ -325,6 +461,30 @@ static void enumerate_synapses(const Connection &c, int num_dest, F &&f) {
     }
 }
 
+// The same enumeration for a connection's state: SG_SPARSE walks each
+// row's compacted list (synapse_kernel.h:311-322), any other type its
+// geometry.
+template <class F>
+static void enumerate_synapses(const ConnState &cs, F &&f) {
+    const Connection &c = *cs.conn;
+    if (c.type != SG_SPARSE) {
+        enumerate_synapses(c, cs.num_dest, f);
+        return;
+    }
+    const int K = cs.k_per_dest;
+    auto ints = [&](const char *k) {
+        return reinterpret_cast<const int32_t *>(cs.arrays.at(k).host.data());
+    };
+    const int32_t *cnt = ints("nonzero counts");
+    const int32_t *fr = ints("from row indices");
+    const int32_t *fc = ints("from column indices");
+    for (int slot = 0; slot < cs.num_dest; ++slot)
+        for (int k = 0; k < cnt[slot]; ++k) {
+            const long wi = (long)slot * K + k;
+            f(slot, slot, k, fr[wi], fc[wi], fr[wi], fc[wi], wi);
+        }
+}
+
 // get_indices (weight_matrix.cpp:1039-1195): per host weight index, the
 // source/destination rows, columns and indices (pre-wrap for arborized
 // connections) and a used flag; unvisited entries keep -1 / 0.
@@ -346,8 +506,8 @@ void State::compute_indices(ConnState &cs) {
     int32_t *used = cs.arr("used").i();
     const bool arbor = c.type == SG_CONVERGENT || c.type == SG_DIVERGENT;
     const int ncols = c.from->cols;
-    enumerate_synapses(c, cs.num_dest, [&](int, int to, int, int r, int col,
-                                           int pr, int pc, long wi) {
+    enumerate_synapses(cs, [&](int, int to, int, int r, int col, int pr,
+                               int pc, long wi) {
         const int rr = arbor ? pr : r, cc = arbor ? pc : col;
         fr[wi] = rr;
         fc[wi] = cc;
@@ -638,7 +798,8 @@ static bool needs_device(const std::string &key) {
     static const char *host_only[] = {
         "delays", "distances", "used", "from row indices",
         "from column indices", "from indices", "to row indices",
-        "to column indices", "to indices", "time since spike"};
+        "to column indices", "to indices", "time since spike",
+        "nonzero counts"};
     for (auto *k : host_only)
         if (key == k) return false;
     return true;
@@ -719,6 +880,23 @@ void State::upload_conn_derived(ConnState &cs) {
     const bool conductance = cs.iz && !cs.direct && cs.iz_kind != SG_IZ_GAP;
     // Base activators reading a BIT source get a spike-plane scratch buffer
     // (one bit per source neuron).
+    if (c.type == SG_SPARSE) {
+        // source index per (slot, k) in the synapse layout, -1 past each
+        // slot's nonzero count
+        const int K = cs.k_per_dest;
+        std::vector<int32_t> idx(tiled_elems(cs.num_dest, K), -1);
+        const int32_t *cnt = cs.arr("nonzero counts").i();
+        const int32_t *fi = cs.arr("from indices").i();
+        for (int slot = 0; slot < cs.num_dest; ++slot)
+            for (int k = 0; k < cnt[slot]; ++k)
+                idx[(((long)slot >> 6) * K + k) * 64 + (slot & 63)] =
+                    fi[(long)slot * K + k];
+        if (cs.d_sparse_from) check_hip(hipFree(cs.d_sparse_from), "hipFree");
+        check_hip(hipMalloc((void **)&cs.d_sparse_from, idx.size() * 4),
+                  "hipMalloc sparse");
+        check_hip(hipMemcpy(cs.d_sparse_from, idx.data(), idx.size() * 4,
+                            hipMemcpyHostToDevice), "hipMemcpy sparse");
+    }
     if ((!cs.iz || cs.direct) && cs.from->out_type == SG_OUT_BIT &&
         !cs.d_plane)
         check_hip(hipMalloc((void **)&cs.d_plane,
@@ -744,8 +922,8 @@ void State::upload_conn_derived(ConnState &cs) {
     // (up to) 32 visits in the reference's serial (destination-major) order,
     // newest first, for the time-since-spike pass.
     std::vector<int32_t> visits(from_n, 0);
-    enumerate_synapses(c, cs.num_dest, [&](int, int, int, int r, int col,
-                                           int, int, long) {
+    enumerate_synapses(cs, [&](int, int, int, int r, int col, int, int,
+                               long) {
         ++visits[r * c.from->cols + col];
     });
     const bool tiled = conn_tiled(cs);
@@ -758,8 +936,8 @@ void State::upload_conn_derived(ConnState &cs) {
     std::vector<uint8_t> rev((size_t)from_n * 32, 0);
     std::vector<int32_t> seen(from_n, 0);
     const int32_t *dl = cs.arr("delays").i();
-    enumerate_synapses(c, cs.num_dest, [&](int slot, int, int k, int r,
-                                           int col, int, int, long wi) {
+    enumerate_synapses(cs, [&](int slot, int, int k, int r, int col, int,
+                               int, long wi) {
         const int f = r * c.from->cols + col;
         const int j = ++seen[f];           // 1-based position of this visit
         const int back = visits[f] - j;    // visits of f after this one

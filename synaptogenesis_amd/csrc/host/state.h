@@ -85,6 +85,7 @@ struct ConnState {
     uint8_t *d_visits = nullptr;      // min(32, visits per source)
     int32_t *d_tss[2] = {nullptr, nullptr};
     uint64_t *d_plane = nullptr;      // spike plane of a BIT source (base)
+    int32_t *d_sparse_from = nullptr; // SG_SPARSE source indices, tiled
     int tss_cur = 0;
     bool pending_update = false;
     bool fused_gol = false;   // handled by sg_hip_gol_step
@@ -93,6 +94,11 @@ struct ConnState {
     Array &arr(const std::string &k) { return arrays.at(k); }
     bool has(const std::string &k) const { return arrays.count(k) > 0; }
 };
+
+// Warnings of state construction (the reference's LOG_WARNING), shown
+// unless set_warnings(false) or set_suppress_output(true) turned them off.
+void set_host_messages(bool warnings, bool suppress);
+void warn(const std::string &msg);
 
 class State {
   public:
@@ -131,6 +137,8 @@ class State {
     void init_weights(ConnState &cs);
     void process_iz_matrix(ConnState &cs);
     void compute_indices(ConnState &cs);
+    void sparsify(ConnState &cs);
+    void adjust_sparse_indices(ConnState &cs);
     void compute_distances(ConnState &cs, float from_spacing,
                            float to_spacing, float x_offset, float y_offset);
     void upload_conn_derived(ConnState &cs);

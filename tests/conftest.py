@@ -3,6 +3,16 @@ import sys
 
 import pytest
 
+# PyTorch ships its own HIP runtime with the same soname
+# (libamdhip64.so.7) as the one the library links; whichever is loaded
+# first serves the whole process, and PyTorch finds no GPU on a runtime
+# other than its own.  Loading it before any test opens the library lets
+# the ensemble, simulation and autograd tests share one process.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))

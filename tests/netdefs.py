@@ -538,6 +538,97 @@ def mixed_clusters(steps=40, workload=117, name="mixed_clusters"):
                               ("par", "I", "voltage")])
 
 
+def _thin(w, st, stream, keep):
+    """Zero a seeded share (1 - keep) of the weights."""
+    mask = st.uniform(w.size, stream=stream) < np.float32(keep)
+    return np.where(mask, w, np.float32(0.0)).astype(np.float32)
+
+
+def sparse_base(steps=20, workload=119, name="sparse_base"):
+    """Sparse base activators (state/weight_matrix.cpp:510-663,
+    synapse_kernel.h:305-366): fully connected, convergent with wrap and
+    without, divergent and one-to-one connections with "sparse": true and
+    about a third of their weights zero, from a relay and a game-of-life
+    source (FLOAT and BIT), plus an all-zero matrix (kept as one entry per
+    destination, with the reference's warning)."""
+    st = Stream(workload)
+    L = [layer("in", "relay", 6, 6), layer("g", "game_of_life", 6, 6),
+         layer("h", "rate_encoding", 1, 20), layer("c", "relay", 6, 6),
+         layer("d", "relay", 6, 6)]
+    C = [conn("in_h", "in", "h", "fully connected", sparse=True),
+         conn("g_h", "g", "h", "fully connected", "sub", sparse=True),
+         conn("in_c", "in", "c", "convergent", sparse=True,
+              arbor={"field size": 3, "wrap": True}),
+         conn("g_c", "g", "c", "convergent", sparse=True,
+              arbor={"field size": 3, "stride": 1}),
+         conn("in_d", "in", "d", "divergent", sparse=True,
+              arbor={"field size": 3, "stride": 1}),
+         conn("g_d", "g", "d", "one to one", sparse=True),
+         conn("h_in", "h", "in", "fully connected", sparse=True)]
+    life = conn("life", "g", "g", "convergent", convolutional=True,
+                arbor={"field size": 3, "wrap": True},
+                weight={"type": "flat", "weight": 1, "diagonal": False})
+    weights = {}
+    for i, c in enumerate(C):
+        w = st.normal(num_weights(c, L), stream=10 + i, std=0.6)
+        weights[c["name"]] = (np.zeros_like(w) if c["name"] == "h_in"
+                              else _thin(w, st, 30 + i, 0.65))
+    gol0 = (st.bernoulli(36, 0.5, stream=5).astype(np.uint32) << 31)
+    inputs = {("sparse", "in"): lambda t: st.normal(36, step=t + 1,
+                                                    stream=2)}
+    return Case(name, net("sparse", L, C + [life]), steps, weights=weights,
+                inputs=inputs, init={("sparse", "g", "output"): gol0},
+                capture=[("sparse", l, "output") for l in ("h", "c", "d")] +
+                        [("sparse", l, "input") for l in ("h", "c", "d")],
+                final_conn=[(c["name"], "weights") for c in C])
+
+
+def sparse_iz(steps=50, workload=120, name="sparse_iz"):
+    """Sparse Izhikevich conductance connections: a plastic E->E
+    convergent wrap (STDP + short-term plasticity, per-synapse delays from
+    the dense delay callback, compacted with the weights), a fully
+    connected E->I (short-term w starts at 0: the type is SPARSE now) and
+    a direct divergent I->E; about 40 % of the weights zero."""
+    st = Stream(workload)
+    L = [layer("E", "izhikevich", 10, 12, params="regular"),
+         layer("I", "izhikevich", 5, 6, params="fast")]
+    C = [conn("EE", "E", "E", "convergent", plastic=True, sparse=True,
+              arbor={"field size": 5, "stride": 1, "wrap": True},
+              learning_rate=0.05),
+         conn("EI", "E", "I", "fully connected", sparse=True, plastic=True),
+         conn("IE", "I", "E", "divergent", "sub", sparse=True,
+              arbor={"field size": 4, "stride": 2, "wrap": True},
+              direct=True, myelinated=True)]
+    weights = {}
+    for i, c in enumerate(C):
+        w = st.uniform(num_weights(c, L), stream=10 + i, low=0.0, high=0.8)
+        weights[c["name"]] = _thin(w, st, 30 + i, 0.6)
+    delays = {c["name"]: st.integers(num_weights(c, L), 0, 31,
+                                     stream=20 + i)
+              for i, c in enumerate(C[:2])}
+    init = {}
+    for lname, n in (("E", 120), ("I", 30)):
+        init[("sparse", lname, "voltage")] = np.full(n, -65.0, np.float32)
+        init[("sparse", lname, "recovery")] = np.full(
+            n, np.float32(0.2) * np.float32(-65.0), np.float32)
+    inputs = {("sparse", "E"): lambda t: st.normal(120, step=t + 1, stream=3,
+                                                   mean=3.0, std=5.0)}
+    keys = ["weights", "short term R", "short term w", "weight derivatives",
+            "presyn trace", "time since spike", "short trace", "long trace"]
+    return Case(name, net("sparse", L, C), steps, weights=weights,
+                delays=delays, init=init, inputs=inputs,
+                capture=[("sparse", "E", "output"), ("sparse", "I", "output"),
+                         ("sparse", "I", "voltage")],
+                final_neuron=[("sparse", l, k) for l in ("E", "I")
+                              for k in ("voltage", "recovery", "ampa",
+                                        "gabaa", "time since spike")],
+                final_conn=[(c, k) for c in ("EE", "EI") for k in keys])
+
+
+def sparse_cases():
+    return [sparse_base(), sparse_iz()]
+
+
 def cluster_cases():
     return [seq_chain(), seq_chain(name="ff_chain", cluster="feedforward",
                                    workload=118),
@@ -552,4 +643,4 @@ def all_cases():
             iz_sheet(24, 32, 80, "iz_sheet_hot", noise=(2.0, 5.0),
                      workload=107),
             iz_ops(), rate(), iz_init(), history(), gol_rule(),
-            relay_ramp()] + ragged_cases() + cluster_cases()
+            relay_ramp()] + ragged_cases() + cluster_cases() + sparse_cases()

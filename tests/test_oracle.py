@@ -275,3 +275,41 @@ def test_tolerant_compare_keeps_spike_words_exact():
     flipped = dict(captures)
     flipped[("par", "E", "output")] = e
     assert golden_check.compare(c, flipped, finals, fx, **kw)
+
+
+def test_planted_sparse_mistakes_are_caught(monkeypatch):
+    """Keeping zero-weight synapses in a sparse connection changes which
+    visits update the per-source time since spike; dropping the last
+    nonzero entry of every destination changes the sums.  Each fails the
+    reference's fixture."""
+    real = stepper.Conn.sparsify
+    c = CASES["sparse_iz"]
+
+    def keep_zeros(self, weights, delays):
+        w = np.where(np.asarray(weights) == 0.0, np.float32(1e-30), weights)
+        out_w, out_d = real(self, w.astype(np.float32), delays)
+        return np.where(out_w == np.float32(1e-30), np.float32(0.0),
+                        out_w).astype(np.float32), out_d
+
+    monkeypatch.setattr(stepper.Conn, "sparsify", keep_zeros)
+    captures, finals = stepper.run(c)
+    assert golden_check.compare(c, captures, finals,
+                                golden_check.load("sparse_iz"))
+
+    def drop_last(self, weights, delays):
+        w, d = real(self, weights, delays)
+        K = self.num_weights // self.dst.n
+        frm = self.sparse_from.reshape(-1, K)
+        for row in frm:
+            live = np.flatnonzero(row >= 0)
+            if live.size > 1:
+                row[live[-1]] = -1
+        self._enumerate()
+        return w, d
+
+    monkeypatch.setattr(stepper.Conn, "sparsify", drop_last)
+    b = CASES["sparse_base"]
+    captures, finals = stepper.run(b)
+    assert golden_check.compare(b, captures, finals,
+                                golden_check.load("sparse_base"))
+
