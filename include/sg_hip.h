@@ -289,6 +289,12 @@ int sg_hip_device_count(void);
 /* Last error string for a code. */
 const char *sg_hip_error_string(int code);
 
+/* calc_internal (engine/kernel/kernel.cpp:161-187): one dendritic node's
+ * register folded into its parent's, dst[i] = aggregate(opcode, dst[i],
+ * src[i]), and reset to the node's init value, src[i] = trail. */
+int sg_hip_dendrite(float *dst, float *src, int n, int opcode, float trail,
+                    void *stream);
+
 #ifdef __cplusplus
 }
 #endif

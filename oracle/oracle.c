@@ -237,6 +237,16 @@ static float aggregate(int op, float prior, float input) {
     return prior;
 }
 
+/* calc_internal_SERIAL (engine/kernel/kernel.cpp:161-171): a dendritic
+ * node's register folded into its parent's, then reset to the node's init
+ * value. */
+void orc_dendrite(long n, float *dst, float *src, int opcode, float trail) {
+    for (long i = 0; i < n; ++i) {
+        dst[i] = aggregate(opcode, dst[i], src[i]);
+        src[i] = trail;
+    }
+}
+
 /*
  * activate_base (engine/kernel/kernel.cpp:217, synapse_kernel.h:887-912):
  * per destination sum = 0.0f; sum += extract(src) * w in visit order; then

@@ -131,8 +131,32 @@ class Layer:
                                       "reference's unseeded generator")
         self.history = 1
         self.vars = {}
-        self.input = np.zeros(self.n, np.float32)
+        # dendritic tree (network/layer.cpp:135-152): nodes in creation
+        # order, register = position; connections join as leaves later
+        self.nodes = [{"name": "root", "opcode": 0, "init": 0.0, "reg": 0,
+                       "children": [], "conns": []}]
+        self._add_dendrites(self.nodes[0], d.get("dendrites", []))
+        self.registers = len(self.nodes)
+        self.input = np.zeros(self.registers * self.n, np.float32)
         self.is_input = False
+
+    def _add_dendrites(self, parent, configs):
+        for dc in configs:
+            if _bool(dc.get("second order"), False):
+                raise NotImplementedError("oracle: second-order dendrites")
+            node = {"name": dc["name"],
+                    "opcode": OPCODES[dc.get("opcode", "add")],
+                    "init": stof(dc.get("init", "0.0")),
+                    "reg": len(self.nodes), "children": [], "conns": []}
+            self.nodes.append(node)
+            parent["children"].append(node)
+            self._add_dendrites(node, dc.get("children", []))
+
+    def node(self, name):
+        for nd in self.nodes:
+            if nd["name"] == name:
+                return nd
+        raise ValueError("Cannot find Dendritic Node " + name)
 
 
 def arborized(cfg):
@@ -175,6 +199,9 @@ class Conn:
         self.max_weight = stof(d.get("max weight", "1.0"))
         self.convolutional = _bool(d.get("convolutional"), False)
         self.sparse = _bool(d.get("sparse"), False)
+        node = self.dst.node(d.get("dendrite", "root"))
+        node["conns"].append(self)
+        self.reg = node["reg"]
         if self.sparse and (self.convolutional or self.type == SUBSET):
             raise ValueError("oracle: sparse " + self.name)
         g = Geom()
@@ -496,16 +523,19 @@ class Network:
         src_out = words[reg * src.n:(reg + 1) * src.n]
         dst = c.dst
         va = c.visit_args()
+        # the input register of the connection's dendritic node
+        # (synapse_data.cpp:9-64)
+        reg_in = dst.input[c.reg * dst.n:(c.reg + 1) * dst.n]
         if not c.iz or c.direct:
             lib.orc_activate_base(*va, _p(c.weights), _p(src_out),
                                   src.out_type, c.delay, c.opcode,
-                                  _p(dst.input))
+                                  _p(reg_in))
             return
         a = c.arrays
         v = dst.vars
         if c.opcode == OPCODES["gap"]:
             lib.orc_iz_gap(*va, _p(c.weights), _p(v["voltage"]),
-                           _p(dst.input))
+                           _p(reg_in))
             return
         kind, short, long_ = {
             0: (0, "ampa", "nmda"), 1: (1, "gabaa", "gabab"),
@@ -551,15 +581,36 @@ class Network:
                           c.src.out_type, ctypes.c_float(c.learning_rate),
                           ctypes.c_float(c.max_weight), int(c.stp_flag))
 
+    def start_run(self):
+        """Internal dendritic registers start each run at their node's init
+        value (the engine's cluster nodes set them when they are built,
+        cluster_node.cpp:118-129)."""
+        for layer in self.order:
+            for nd in layer.nodes[1:]:
+                r = nd["reg"]
+                layer.input[r * layer.n:(r + 1) * layer.n] = nd["init"]
+
     def _activate_layer(self, layer):
         """A layer's activate instructions: clear (non-input layers), then
-        its input connections in order (cluster_node.cpp:40-67,
+        its dendritic tree depth first -- each internal node's subtree,
+        then that node's register folded into its parent's -- and each
+        node's own connections in order (cluster_node.cpp:40-67, 115-212;
         instruction.cpp:3-30)."""
+        n = layer.n
         if not layer.is_input:
-            layer.input[:] = 0.0
-        for c in self.conns:
-            if c.dst is layer:
+            layer.input[:n] = 0.0
+        lib = load()
+
+        def dfs(node):
+            for ch in node["children"]:
+                dfs(ch)
+                lib.orc_dendrite(
+                    ctypes.c_long(n), _p(layer.input[node["reg"] * n:]),
+                    _p(layer.input[ch["reg"] * n:]), ch["opcode"],
+                    ctypes.c_float(ch["init"]))
+            for c in node["conns"]:
                 self._activate(c)
+        dfs(layer.nodes[0])
 
     def step(self, inputs, t, learning=True):
         """engine.cpp:231-264: every cluster's pre-input work, the input,
@@ -576,7 +627,8 @@ class Network:
                 if not layer.is_input:
                     self._activate_layer(layer)
         for (s, l), fn in inputs.items():
-            self.layers[(s, l)].input[:] = np.asarray(fn(t), np.float32)
+            layer = self.layers[(s, l)]
+            layer.input[:layer.n] = np.asarray(fn(t), np.float32)
         for i, (ctype, members) in enumerate(self.structures):
             if ctype == "parallel":
                 for layer in members:
@@ -618,6 +670,8 @@ def run(case, steps=None):
     net = Network(case)
     steps = case.steps if steps is None else steps
     captures = {k: [] for k in case.capture}
+    if steps > 0:
+        net.start_run()
     for t in range(steps):
         net.step(case.inputs, t, case.learning)
         for k in case.capture:

@@ -369,6 +369,23 @@ extern "C" int sg_hip_activate_base(const sg_conn *c, void *stream) {
     return hipErrorInvalidValue;
 }
 
+// calc_internal (engine/kernel/kernel.cpp:161-187).
+__global__ void dendrite_kernel(float *dst, float *src, int n, int op,
+                                float trail) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = aggregate(op, dst[i], src[i]);
+    src[i] = trail;
+}
+
+extern "C" int sg_hip_dendrite(float *dst, float *src, int n, int opcode,
+                               float trail, void *stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(dendrite_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, dst, src, n, opcode, trail);
+    return hipGetLastError();
+}
+
 extern "C" int sg_hip_tss_step(int32_t *next, const int32_t *cur,
                                const uint8_t *visits,
                                const uint8_t *rev_delays,

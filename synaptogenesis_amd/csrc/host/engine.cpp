@@ -399,7 +399,7 @@ Props *Engine::run(const Props &args) {
         const int per_word = cs.from->out_type == SG_OUT_BIT ? 32 : 1;
         d.src = cs.from->output.du() +
                 (size_t)(c.delay / per_word) * c.from->size;
-        d.dst = cs.to->input.df();
+        d.dst = cs.to->input.df() + (size_t)c.reg * c.to->size;
         d.weights = cs.arr("weights").df();
         // Plastic base-activator weights stay finite: the updater clamps
         // them to [1e-4, max weight].
@@ -536,7 +536,8 @@ Props *Engine::run(const Props &args) {
         if (io_types.count(ls) || ls->layer->config.has_child("init config") ||
             in_seq.count(ls->layer))
             continue;
-        if (ls->layer->inputs.size() != 1) continue;
+        if (ls->layer->inputs.size() != 1 || ls->layer->registers > 1)
+            continue;
         Connection *c = ls->layer->inputs[0];
         const ArborConfig &a = c->arbor;
         if (c->from != ls->layer || c->type != SG_CONVERGENT ||
@@ -604,7 +605,9 @@ Props *Engine::run(const Props &args) {
     // input too, so their layers keep the separate pass.
     for (auto &lsp : state_->layers()) {
         LayerState *ls = lsp.get();
-        if (ls->model != Model::IZHIKEVICH || io_types.count(ls)) continue;
+        if (ls->model != Model::IZHIKEVICH || io_types.count(ls) ||
+            ls->layer->registers > 1)
+            continue;
         InitFold f;
         if (Props *ic = ls->layer->config.get_child("init config")) {
             const std::string type = ic->get("type", "normal");
@@ -667,11 +670,63 @@ Props *Engine::run(const Props &args) {
                                         lsp->d_plane, stream),
                       "pack plane");
 
+    // Activation order of each layer: its dendritic tree depth first --
+    // every internal node's subtree, then that node's register folded into
+    // its parent's (cluster_node.cpp:115-212, kernel.cpp:161-187) -- and
+    // then the node's own connections; without dendrites, the connections
+    // in creation order.  Internal registers start each run at their
+    // node's init value.
+    struct Act {
+        ConnState *cs;
+        const Dendrite *child, *parent;
+    };
+    std::function<bool(const ConnState &)> is_base_act;
+    std::map<LayerState *, std::vector<Act>> plan;
+    bool any_dendrites = false;
+    for (auto &lsp : state_->layers()) {
+        LayerState *ls = lsp.get();
+        std::vector<Act> &v = plan[ls];
+        std::function<void(const Dendrite *)> dfs = [&](const Dendrite *d) {
+            for (const Dendrite *ch : d->children) {
+                dfs(ch);
+                v.push_back(Act{nullptr, ch, d});
+            }
+            for (Connection *c : d->conns)
+                v.push_back(Act{state_->conn_state(c), nullptr, nullptr});
+        };
+        dfs(ls->layer->dendrites[0].get());
+        const int n = ls->layer->size;
+        for (size_t i = 1; i < ls->layer->dendrites.size(); ++i) {
+            any_dendrites = true;
+            const Dendrite &d = *ls->layer->dendrites[i];
+            check_hip(sg_hip_set_f32(ls->input.df() + (size_t)d.reg * n,
+                                     d.init, n, 1, stream),
+                      "dendrite init");
+        }
+    }
+    auto activate_layer = [&](LayerState *ls, bool split_base) {
+        const int n = ls->layer->size;
+        for (const Act &a : plan[ls]) {
+            if (a.cs) {
+                activate(*a.cs, split_base && is_base_act(*a.cs) ? side
+                                                                : stream);
+                continue;
+            }
+            float *in = ls->input.df();
+            check_hip(sg_hip_dendrite(in + (size_t)a.parent->reg * n,
+                                      in + (size_t)a.child->reg * n, n,
+                                      a.child->opcode, a.child->init,
+                                      stream),
+                      "dendrite");
+        }
+    };
+
     // Split the pre-input activators over two streams only when both kinds
-    // are there.
+    // are there (and no dendritic tree orders a layer's activators).
     auto is_base = [&](const ConnState &cs) {
         return (!cs.iz || cs.direct) && !cs.fused_gol;
     };
+    is_base_act = is_base;
     bool any_base = false, any_other = false;
     for (auto &lsp : state_->layers()) {
         if (is_input(lsp.get()) || gol_fast.count(lsp.get()) ||
@@ -683,7 +738,7 @@ Props *Engine::run(const Props &args) {
             (is_base(cs) ? any_base : any_other) = true;
         }
     }
-    const bool split_ok = any_base && any_other;
+    const bool split_ok = any_base && any_other && !any_dendrites;
 
     // ---- main loop --------------------------------------------------------
     auto &layers = state_->layers();
@@ -712,10 +767,7 @@ Props *Engine::run(const Props &args) {
             LayerState *ls = lsp.get();
             if (is_input(ls) || gol_fast.count(ls) || in_seq.count(ls->layer))
                 continue;
-            for (auto *c : ls->layer->inputs) {
-                ConnState &cs = *state_->conn_state(c);
-                activate(cs, split && is_base(cs) ? side : stream);
-            }
+            activate_layer(ls, split);
         }
         if (split) {
             check_hip(hipEventRecord(ev_join, side), "hipEventRecord");
@@ -885,8 +937,7 @@ Props *Engine::run(const Props &args) {
                     const auto ik = in_keys.find(ls);
                     init_layer(ls, ik == in_keys.end() ||
                                    !ik->second.count("input"), it);
-                    for (auto *c : ls->layer->inputs)
-                        activate(*state_->conn_state(c), stream);
+                    activate_layer(ls, false);
                 }
                 continue;
             }
@@ -899,8 +950,7 @@ Props *Engine::run(const Props &args) {
                 } else if (!init_fold.count(ls)) {
                     init_layer(ls, true, it);
                 }
-                for (auto *c : ls->layer->inputs)
-                    activate(*state_->conn_state(c), stream);
+                activate_layer(ls, false);
                 update_neurons({ls});
             }
         }

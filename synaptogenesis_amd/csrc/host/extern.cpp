@@ -187,6 +187,9 @@ NETWORK create_network(PROPS properties) {
         std::unique_ptr<NetHandle> n(new NetHandle());
         n->net.reset(new Network(*P(properties)));
         return track(n.release());
+    } catch (const std::exception &ex) {
+        if (!g_suppress) fprintf(stderr, "syngen: %s\n", ex.what());
+        return nullptr;
     } catch (...) {
         return nullptr;
     }

@@ -4,6 +4,9 @@
 
 namespace sg {
 
+void add_dendrites(Layer &l, Dendrite *parent,
+                   const std::vector<Props *> &configs);
+
 int connection_type_from(const std::string &s) {   // util/constants.h:88-94
     if (s == "fully connected") return SG_FULLY_CONNECTED;
     if (s == "subset") return SG_SUBSET;
@@ -151,14 +154,48 @@ void Network::add_structure(const Props &sc) {
         l->id = std::hash<std::string>()(st->name + "/" + l->name);
         if (l->rows <= 0 || l->cols <= 0)
             throw Error("Invalid dimensions for layer " + l->name);
-        if (!lc->get_child_array("dendrites").empty())
-            throw Error("Dendritic trees other than the root node are not "
-                        "supported (layer " + l->name + ")");
+        l->dendrites.emplace_back(new Dendrite());
+        l->dendrites[0]->name = "root";
+        add_dendrites(*l, l->dendrites[0].get(),
+                      lc->get_child_array("dendrites"));
+        l->registers = (int)l->dendrites.size();
         if (st->layer(l->name))
             throw Error("Repeated layer name : \"" + l->name + "\"");
         st->layers.push_back(std::move(l));
     }
     structures_.push_back(std::move(st));
+}
+
+// Layer::add_dendrites (layer.cpp:135-152): children in config order,
+// each followed by its own subtree; a new node takes the next register.
+void add_dendrites(Layer &l, Dendrite *parent,
+                   const std::vector<Props *> &configs) {
+    for (Props *dc : configs) {
+        if (!dc->has("name"))
+            throw Error("Attempted to dendrite without name to layer!");
+        if (dc->get_bool("second order", false))
+            throw Error("Second-order dendritic nodes are not supported "
+                        "(layer " + l.name + ")");
+        const std::string name = dc->get("name");
+        if (l.dendrite(name))
+            throw Error("Error in dendritic node of " + l.str() + "\n"
+                        "  Duplicate dendritic node name: " + name + "!");
+        std::unique_ptr<Dendrite> d(new Dendrite());
+        d->name = name;
+        d->opcode = opcode_from(dc->get("opcode", "add"));
+        d->init = dc->get_float("init", 0.0f);
+        d->reg = (int)l.dendrites.size();
+        Dendrite *child = d.get();
+        parent->children.push_back(child);
+        l.dendrites.push_back(std::move(d));
+        add_dendrites(l, child, dc->get_child_array("children"));
+    }
+}
+
+Dendrite *Layer::dendrite(const std::string &n) const {
+    for (auto &d : dendrites)
+        if (d->name == n) return d.get();
+    return nullptr;
 }
 
 void Network::add_connection(const Props &cc) {
@@ -206,8 +243,6 @@ void Network::add_connection(const Props &cc) {
     c->sparse = cc.get_bool("sparse", false);
     c->convolutional = cc.get_bool("convolutional", false);
     c->recurrent = cc.get_bool("recurrent", false);
-    if (cc.get("dendrite", "root") != "root")
-        throw Error("Dendrites other than the root node are not supported");
     if (c->delay < 0)
         throw Error("Attempted to construct ConnectionConfig with negative "
                     "delay!");
@@ -265,6 +300,13 @@ void Network::add_connection(const Props &cc) {
     }
     c->dense_type = c->type;
     c->dense_num_weights = c->num_weights;
+    const std::string dname = cc.get("dendrite", "root");
+    Dendrite *node = to->dendrite(dname);
+    if (!node)
+        throw Error("Error in " + to->str() + ":\n"
+                    "  Cannot find Dendritic Node " + dname + "!");
+    node->conns.push_back(c.get());
+    c->reg = node->reg;
     c->id = std::hash<std::string>()(c->str());
     from->outputs.push_back(c.get());
     to->inputs.push_back(c.get());

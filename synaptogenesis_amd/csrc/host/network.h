@@ -38,6 +38,21 @@ struct SubsetConfig {  // network/connection_config.cpp:112-134
     }
 };
 
+// One node of a layer's dendritic tree (network/dendritic_node.h:26-66,
+// layer.cpp:135-152).  Internal nodes own an input register, numbered in
+// the order the nodes are created (root 0); connections are leaves of the
+// node they name and activate into its register.  A node's children run
+// internal nodes first (created with the layer) and then its connections
+// (created afterwards), as in the reference's child list.
+struct Dendrite {
+    std::string name;
+    int opcode = SG_ADD;
+    float init = 0.0f;
+    int reg = 0;
+    std::vector<Dendrite *> children;
+    std::vector<Connection *> conns;
+};
+
 struct Layer {
     Structure *structure = nullptr;
     Props config;
@@ -46,10 +61,13 @@ struct Layer {
     bool plastic = false;
     size_t id = 0;
     std::vector<Connection *> inputs, outputs;
+    std::vector<std::unique_ptr<Dendrite>> dendrites;   // [0] is the root
+    int registers = 1;                                  // input registers
 
     std::string param(const std::string &k, const std::string &def) const {
         return config.get(k, def);
     }
+    Dendrite *dendrite(const std::string &name) const;
     int max_out_delay() const;
     std::string str() const;
 };
@@ -64,6 +82,7 @@ struct Connection {
     float max_weight = 1.0f;
     bool plastic = true, convolutional = false, sparse = false;
     bool recurrent = false;
+    int reg = 0;          // input register of its dendritic node
     ArborConfig arbor;
     SubsetConfig subset;
     int num_weights = 0;

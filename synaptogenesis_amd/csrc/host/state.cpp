@@ -135,7 +135,9 @@ void State::build_layer(Layer *l) {
     const int per_word = ls->out_type == SG_OUT_BIT ? 32 : 1;
     ls->history = 1 + l->max_out_delay() / per_word;
     const int n = l->size;
-    ls->input = make_array("input", Elem::F32, n);
+    // one input register per dendritic node with children of its own
+    // (attributes.cpp:23-28); register 0 is the neurons' input
+    ls->input = make_array("input", Elem::F32, (size_t)l->registers * n);
     ls->output = make_array("output", Elem::U32, (size_t)ls->history * n);
 
     if (ls->model == Model::IZHIKEVICH) {

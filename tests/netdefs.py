@@ -625,6 +625,43 @@ def sparse_iz(steps=50, workload=120, name="sparse_iz"):
                 final_conn=[(c, k) for c in ("EE", "EI") for k in keys])
 
 
+def dendrites(steps=20, workload=121, name="dendrites"):
+    """Dendritic trees (network/dendritic_node.h:26-66,
+    cluster_node.cpp:115-212, kernel.cpp:161-187): a relay layer whose
+    tree has a multiplicative node (init 1) over two connections, a
+    subtractive node holding a nested additive node (init 0.5) and a
+    connection of its own, and a connection on the root; a rate layer with
+    a pooling node.  Every internal register is folded into its parent's
+    after its subtree and reset to its init value."""
+    st = Stream(workload)
+    tree = [{"name": "m", "opcode": "mult", "init": "1.0"},
+            {"name": "s", "opcode": "sub",
+             "children": [{"name": "a", "opcode": "add", "init": "0.5"}]}]
+    L = [layer("in", "relay", 6, 6), layer("g", "relay", 6, 6),
+         dict(layer("d", "relay", 6, 6), dendrites=tree),
+         dict(layer("r", "rate_encoding", 1, 12),
+              dendrites=[{"name": "p", "opcode": "pool", "init": "-1.0"}])]
+    C = [conn("m1", "in", "d", "convergent", dendrite="m",
+              arbor={"field size": 3, "wrap": True}),
+         conn("m2", "g", "d", "one to one", dendrite="m"),
+         conn("a1", "in", "d", "fully connected", dendrite="a"),
+         conn("s1", "g", "d", "convergent", dendrite="s",
+              arbor={"field size": 3}),
+         conn("root", "in", "d", "one to one"),
+         conn("p1", "d", "r", "fully connected", dendrite="p"),
+         conn("p2", "in", "r", "fully connected", dendrite="p"),
+         conn("r1", "g", "r", "fully connected")]
+    weights = {c["name"]: st.normal(num_weights(c, L), stream=10 + i,
+                                    std=0.5)
+               for i, c in enumerate(C)}
+    inputs = {("dend", "in"): lambda t: st.normal(36, step=t + 1, stream=2),
+              ("dend", "g"): lambda t: st.uniform(36, step=t + 1, stream=3)}
+    return Case(name, net("dend", L, C), steps, weights=weights,
+                inputs=inputs,
+                capture=[("dend", "d", "output"), ("dend", "r", "output")],
+                final_neuron=[("dend", "d", "input"), ("dend", "r", "input")])
+
+
 def sparse_cases():
     return [sparse_base(), sparse_iz()]
 
@@ -643,4 +680,5 @@ def all_cases():
             iz_sheet(24, 32, 80, "iz_sheet_hot", noise=(2.0, 5.0),
                      workload=107),
             iz_ops(), rate(), iz_init(), history(), gol_rule(),
-            relay_ramp()] + ragged_cases() + cluster_cases() + sparse_cases()
+            relay_ramp()] + ragged_cases() + cluster_cases() + sparse_cases() + \
+        [dendrites()]

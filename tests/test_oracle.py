@@ -86,6 +86,11 @@ PLANTED = [
     ("birth range collapsed to its lower end",
      ": (input >= birth_min && input <= birth_max);",
      ": (input == birth_min);", "gol_rule"),
+    ("dendritic register not reset to its node's init value",
+     "        src[i] = trail;", "        (void)trail;", "dendrites"),
+    ("dendritic register folded with its parent's opcode",
+     "        dst[i] = aggregate(opcode, dst[i], src[i]);",
+     "        dst[i] = aggregate(ORC_ADD, dst[i], src[i]);", "dendrites"),
     ("Izhikevich settle loop stops on either variable",
      "} while (fabsf(delta_v) > 0.001 && fabsf(delta_r) > 0.001);",
      "} while (fabsf(delta_v) > 0.001 || fabsf(delta_r) > 0.001);",
