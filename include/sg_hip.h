@@ -188,6 +188,13 @@ typedef struct sg_iz_layer {
  * extractor.cpp:8-15, aggregator.cpp:15-48). */
 int sg_hip_activate_base(const sg_conn *c, void *stream);
 
+/* activate_base_second_order (synapse_kernel.h:918-935, kernel.cpp:220): a
+ * second-order slave connection folds each synapse's activity into the
+ * host's weight copy, sow[w] = aggregate(opcode, sow[w], extract(source) *
+ * weights[w]); sow has the slave's (= the host's) device layout. */
+int sg_hip_activate_second_order(const sg_conn *slave, float *sow,
+                                 void *stream);
+
 /* Time since spike of every source after one activation of an Izhikevich
  * conductance connection (izhikevich_attributes.cpp:258-262, applied once
  * per visit in the reference's serial order).  A source visited m times

@@ -269,6 +269,21 @@ void orc_activate_base(long ndest, const int *dest, const long *first,
     }
 }
 
+/* activate_base_second_order (engine/kernel/synapse_kernel.h:918-935):
+ * every visited synapse of a second-order slave folds its activity into the
+ * host's weight copy, sow[w] = aggregate(sow[w], extract(src) * weight[w]). */
+void orc_activate_second_order(long ndest, const long *first, const int *src,
+                               const long *widx, const float *weights,
+                               const uint32_t *src_out, int out_type,
+                               int delay, int opcode, float *sow) {
+    for (long d = 0; d < ndest; ++d)
+        for (long v = first[d]; v < first[d + 1]; ++v) {
+            float val = extract(src_out[src[v]], out_type, delay) *
+                        weights[widx[v]];
+            sow[widx[v]] = aggregate(opcode, sow[widx[v]], val);
+        }
+}
+
 /*
  * Izhikevich conductance activators activate_iz_{add,sub,mult,reward}
  * (state/impl/izhikevich_attributes.cpp:218-347).  Per visit: the spike bit

@@ -142,11 +142,11 @@ class Layer:
 
     def _add_dendrites(self, parent, configs):
         for dc in configs:
-            if _bool(dc.get("second order"), False):
-                raise NotImplementedError("oracle: second-order dendrites")
             node = {"name": dc["name"],
                     "opcode": OPCODES[dc.get("opcode", "add")],
                     "init": stof(dc.get("init", "0.0")),
+                    "second order": _bool(dc.get("second order"), False),
+                    "host": None,
                     "reg": len(self.nodes), "children": [], "conns": []}
             self.nodes.append(node)
             parent["children"].append(node)
@@ -200,8 +200,15 @@ class Conn:
         self.convolutional = _bool(d.get("convolutional"), False)
         self.sparse = _bool(d.get("sparse"), False)
         node = self.dst.node(d.get("dendrite", "root"))
-        node["conns"].append(self)
         self.reg = node["reg"]
+        # second order (dendritic_node.cpp:118-139): the node's first
+        # connection hosts, the later ones fold into its weight copy
+        self.so_host = node.get("second order", False) and \
+            node["host"] is None
+        if self.so_host:
+            node["host"] = self
+        else:
+            node["conns"].append(self)
         if self.sparse and (self.convolutional or self.type == SUBSET):
             raise ValueError("oracle: sparse " + self.name)
         g = Geom()
@@ -515,19 +522,27 @@ class Network:
 
     # -- one step ------------------------------------------------------------
 
-    def _activate(self, c):
-        lib = load()
+    @staticmethod
+    def _src_out(c):
+        """The source's output register the connection reads at its delay
+        (synapse_data.cpp:43-51)."""
         src = c.src
         reg = c.delay // 32 if src.out_type == OUT_BIT else c.delay
-        words = src.vars["output"]
-        src_out = words[reg * src.n:(reg + 1) * src.n]
+        return src.vars["output"][reg * src.n:(reg + 1) * src.n]
+
+    def _activate(self, c, weights=None):
+        lib = load()
+        src = c.src
+        src_out = self._src_out(c)
+        if weights is None:
+            weights = c.weights
         dst = c.dst
         va = c.visit_args()
         # the input register of the connection's dendritic node
         # (synapse_data.cpp:9-64)
         reg_in = dst.input[c.reg * dst.n:(c.reg + 1) * dst.n]
         if not c.iz or c.direct:
-            lib.orc_activate_base(*va, _p(c.weights), _p(src_out),
+            lib.orc_activate_base(*va, _p(weights), _p(src_out),
                                   src.out_type, c.delay, c.opcode,
                                   _p(reg_in))
             return
@@ -608,6 +623,19 @@ class Network:
                     ctypes.c_long(n), _p(layer.input[node["reg"] * n:]),
                     _p(layer.input[ch["reg"] * n:]), ch["opcode"],
                     ctypes.c_float(ch["init"]))
+            if node.get("second order"):
+                host = node["host"]
+                if host is None:
+                    return
+                sow = host.weights.copy()
+                host.arrays["second order weights"] = sow
+                for c in node["conns"]:
+                    lib.orc_activate_second_order(
+                        ctypes.c_long(c.ndest), _p(c.first), _p(c.srcs),
+                        _p(c.widx), _p(c.weights), _p(self._src_out(c)),
+                        c.src.out_type, c.delay, c.opcode, _p(sow))
+                self._activate(host, weights=sow)
+                return
             for c in node["conns"]:
                 self._activate(c)
         dfs(layer.nodes[0])

@@ -369,6 +369,63 @@ extern "C" int sg_hip_activate_base(const sg_conn *c, void *stream) {
     return hipErrorInvalidValue;
 }
 
+// activate_base_second_order (synapse_kernel.h:918-935): one lane per
+// destination slot walks its visits; every synapse of the slave updates the
+// same index of the host's weight copy.
+template <int TYPE, int OUT>
+__global__ void second_order_kernel(sg_conn c, float *sow) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= c.num_dest) return;
+    SynapseIter<TYPE> it;
+    it.init(c, slot);
+    for (int k = 0; k < it.count; ++k) {
+        const int from = it.source();
+        if (from >= 0) {
+            const float val = extract<OUT>(c.src[from], c.delay) * c.weights[it.wi];
+            sow[it.wi] = aggregate(c.opcode, sow[it.wi], val);
+        }
+        it.advance();
+    }
+}
+
+template <int TYPE>
+static hipError_t launch_second_order_t(const sg_conn &c, float *sow,
+                                        hipStream_t s) {
+    dim3 g((c.num_dest + 255) / 256), b(256);
+    switch (c.out_type) {
+        case SG_OUT_BIT:
+            hipLaunchKernelGGL((second_order_kernel<TYPE, SG_OUT_BIT>), g, b,
+                               0, s, c, sow);
+            break;
+        case SG_OUT_INT:
+            hipLaunchKernelGGL((second_order_kernel<TYPE, SG_OUT_INT>), g, b,
+                               0, s, c, sow);
+            break;
+        default:
+            hipLaunchKernelGGL((second_order_kernel<TYPE, SG_OUT_FLOAT>), g,
+                               b, 0, s, c, sow);
+    }
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_activate_second_order(const sg_conn *c, float *sow,
+                                            void *stream) {
+    if (c->num_dest <= 0) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    switch (c->type) {
+        case SG_FULLY_CONNECTED:
+            return launch_second_order_t<SG_FULLY_CONNECTED>(*c, sow, s);
+        case SG_SUBSET: return launch_second_order_t<SG_SUBSET>(*c, sow, s);
+        case SG_ONE_TO_ONE:
+            return launch_second_order_t<SG_ONE_TO_ONE>(*c, sow, s);
+        case SG_CONVERGENT:
+            return launch_second_order_t<SG_CONVERGENT>(*c, sow, s);
+        case SG_DIVERGENT:
+            return launch_second_order_t<SG_DIVERGENT>(*c, sow, s);
+    }
+    return hipErrorInvalidValue;
+}
+
 // calc_internal (engine/kernel/kernel.cpp:161-187).
 __global__ void dendrite_kernel(float *dst, float *src, int n, int op,
                                 float trail) {

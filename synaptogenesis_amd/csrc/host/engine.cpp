@@ -416,6 +416,12 @@ Props *Engine::run(const Props &args) {
         }
         d.store_sum = store_sum.count(&cs) ? 1 : 0;
         d.sparse_from = cs.d_sparse_from;
+        // A second-order host activates with its slaves' copy of its
+        // weights (synapse_data.cpp:20-23), which may hold any value.
+        if (c.so_host) {
+            d.weights = cs.arr("second order weights").df();
+            d.skip_silent = 0;
+        }
         return d;
     };
     auto iz_syn = [&](ConnState &cs) {
@@ -676,8 +682,12 @@ Props *Engine::run(const Props &args) {
     // then the node's own connections; without dendrites, the connections
     // in creation order.  Internal registers start each run at their
     // node's init value.
+    // kinds: a connection's activation, a register fold, a second-order
+    // node's weight copy, a slave's fold into it, the host's activation
+    enum { ACT_CONN, ACT_FOLD, ACT_SO_COPY, ACT_SO_SLAVE, ACT_SO_HOST };
     struct Act {
-        ConnState *cs;
+        int kind;
+        ConnState *cs, *host;
         const Dendrite *child, *parent;
     };
     std::function<bool(const ConnState &)> is_base_act;
@@ -689,10 +699,24 @@ Props *Engine::run(const Props &args) {
         std::function<void(const Dendrite *)> dfs = [&](const Dendrite *d) {
             for (const Dendrite *ch : d->children) {
                 dfs(ch);
-                v.push_back(Act{nullptr, ch, d});
+                v.push_back(Act{ACT_FOLD, nullptr, nullptr, ch, d});
+            }
+            if (d->second_order) {
+                // cluster_node.cpp:118-121, 185-211: copy the host's
+                // weights, let every slave fold into the copy, then
+                // activate the host with it
+                if (!d->host) return;
+                ConnState *host = state_->conn_state(d->host);
+                v.push_back(Act{ACT_SO_COPY, host, host, nullptr, nullptr});
+                for (Connection *c : d->conns)
+                    v.push_back(Act{ACT_SO_SLAVE, state_->conn_state(c), host,
+                                    nullptr, nullptr});
+                v.push_back(Act{ACT_SO_HOST, host, host, nullptr, nullptr});
+                return;
             }
             for (Connection *c : d->conns)
-                v.push_back(Act{state_->conn_state(c), nullptr, nullptr});
+                v.push_back(Act{ACT_CONN, state_->conn_state(c), nullptr,
+                                nullptr, nullptr});
         };
         dfs(ls->layer->dendrites[0].get());
         const int n = ls->layer->size;
@@ -707,9 +731,25 @@ Props *Engine::run(const Props &args) {
     auto activate_layer = [&](LayerState *ls, bool split_base) {
         const int n = ls->layer->size;
         for (const Act &a : plan[ls]) {
-            if (a.cs) {
+            if (a.kind == ACT_CONN || a.kind == ACT_SO_HOST) {
                 activate(*a.cs, split_base && is_base_act(*a.cs) ? side
                                                                 : stream);
+                continue;
+            }
+            if (a.kind == ACT_SO_COPY) {
+                const Array &w = a.host->arr("weights");
+                check_hip(hipMemcpyAsync(a.host->arr("second order weights").dev,
+                                         w.dev, w.dev_elems * 4,
+                                         hipMemcpyDeviceToDevice, stream),
+                          "second order copy");
+                continue;
+            }
+            if (a.kind == ACT_SO_SLAVE) {
+                const sg_conn d = conn_desc(*a.cs);
+                check_hip(sg_hip_activate_second_order(
+                              &d, a.host->arr("second order weights").df(),
+                              stream),
+                          "second order");
                 continue;
             }
             float *in = ls->input.df();

@@ -46,7 +46,7 @@ std::string Layer::str() const {
 std::string Connection::str() const {               // connection.cpp:251-259
     return "[Connection: " + from->name + " (" + from->structure->name +
            ") -> " + to->name + " (" + to->structure->name + ") " +
-           "{Node: root} name=" + name + "]";
+           "{Node: " + node + "} name=" + name + "]";
 }
 
 int Connection::matrix_rows() const {
@@ -173,10 +173,11 @@ void add_dendrites(Layer &l, Dendrite *parent,
     for (Props *dc : configs) {
         if (!dc->has("name"))
             throw Error("Attempted to dendrite without name to layer!");
-        if (dc->get_bool("second order", false))
-            throw Error("Second-order dendritic nodes are not supported "
-                        "(layer " + l.name + ")");
         const std::string name = dc->get("name");
+        if (parent->second_order)      // dendritic_node.cpp:81-86
+            throw Error("Error in dendritic node " + name + " of " + l.str() +
+                        "\n  Second order dendritic nodes cannot have "
+                        "internal children!");
         if (l.dendrite(name))
             throw Error("Error in dendritic node of " + l.str() + "\n"
                         "  Duplicate dendritic node name: " + name + "!");
@@ -184,6 +185,7 @@ void add_dendrites(Layer &l, Dendrite *parent,
         d->name = name;
         d->opcode = opcode_from(dc->get("opcode", "add"));
         d->init = dc->get_float("init", 0.0f);
+        d->second_order = dc->get_bool("second order", false);
         d->reg = (int)l.dendrites.size();
         Dendrite *child = d.get();
         parent->children.push_back(child);
@@ -305,8 +307,29 @@ void Network::add_connection(const Props &cc) {
     if (!node)
         throw Error("Error in " + to->str() + ":\n"
                     "  Cannot find Dendritic Node " + dname + "!");
-    node->conns.push_back(c.get());
+    c->node = node->name;
     c->reg = node->reg;
+    // connection.cpp:141-190: second-order roles and their restrictions
+    if (node->second_order) {
+        c->so_host = node->host == nullptr;
+        c->so_slave = !c->so_host;
+        if (c->plastic)
+            throw Error("Error in " + c->str() + ":\n  Plastic second "
+                        "order connections are not supported!");
+        if (c->sparse)
+            throw Error("Error in " + c->str() + ":\n  Sparse second "
+                        "order connections are not supported!");
+        if (c->convolutional)
+            throw Error("Convolutional second-order connections are not "
+                        "supported (" + c->str() + ")");
+        if (c->so_slave &&
+            (c->type != node->host->type ||
+             c->num_weights != node->host->num_weights))
+            throw Error("Error in " + c->str() + ":\n  Second order "
+                        "connection does not match host connection!");
+    }
+    if (c->so_host) node->host = c.get();
+    else node->conns.push_back(c.get());
     c->id = std::hash<std::string>()(c->str());
     from->outputs.push_back(c.get());
     to->inputs.push_back(c.get());

@@ -49,6 +49,11 @@ struct Dendrite {
     int opcode = SG_ADD;
     float init = 0.0f;
     int reg = 0;
+    // second order (dendritic_node.h:53-59): the first connection is the
+    // host, the later ones (its leaves) fold their synaptic activities into
+    // a copy of the host's weights before the host activates
+    bool second_order = false;
+    Connection *host = nullptr;
     std::vector<Dendrite *> children;
     std::vector<Connection *> conns;
 };
@@ -83,6 +88,8 @@ struct Connection {
     bool plastic = true, convolutional = false, sparse = false;
     bool recurrent = false;
     int reg = 0;          // input register of its dendritic node
+    std::string node = "root";                 // its dendritic node
+    bool so_host = false, so_slave = false;    // second-order roles
     ArborConfig arbor;
     SubsetConfig subset;
     int num_weights = 0;

@@ -274,6 +274,13 @@ void State::build_connection(Connection *c) {
 
     init_weights(s);
     if (c->sparse) sparsify(s);
+    if ((c->so_host || c->so_slave) && s.iz && !s.direct)
+        throw Error("Second-order connections into an Izhikevich "
+                    "conductance path are not supported (" + c->str() + ")");
+    // the host's weight copy for its slaves (weight_matrix.cpp:78-80)
+    if (c->so_host)
+        s.arrays["second order weights"] =
+            make_array("second order weights", Elem::F32, c->num_weights);
     if (s.iz) {
         const int n = c->num_weights;
         for (const char *k : {"short trace", "long trace", "presyn trace",

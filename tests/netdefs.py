@@ -662,6 +662,45 @@ def dendrites(steps=20, workload=121, name="dendrites"):
                 final_neuron=[("dend", "d", "input"), ("dend", "r", "input")])
 
 
+def second_order(steps=20, workload=122, name="second_order"):
+    """Second-order dendritic nodes (dendritic_node.h:53-59,
+    cluster_node.cpp:115-211, synapse_kernel.h:918-935): on a relay layer,
+    a node whose host is a convergent connection and whose two slaves
+    (multiplying, then adding) fold their synaptic activities into the
+    host's weight copy each step; a second node with a fully connected host
+    and one subtracting slave from a game-of-life source (BIT); plus a
+    plain connection on the root."""
+    st = Stream(workload)
+    tree = [{"name": "so", "opcode": "add", "second order": "true"},
+            {"name": "sf", "opcode": "sub", "second order": "true"}]
+    L = [layer("in", "relay", 6, 6), layer("g", "game_of_life", 6, 6),
+         layer("q", "relay", 6, 6),
+         dict(layer("d", "relay", 6, 6), dendrites=tree)]
+    C = [conn("host", "in", "d", "convergent", dendrite="so",
+              arbor={"field size": 3, "wrap": True}),
+         conn("slave_m", "q", "d", "convergent", "mult", dendrite="so",
+              arbor={"field size": 3, "wrap": True}),
+         conn("slave_a", "g", "d", "convergent", dendrite="so",
+              arbor={"field size": 3, "wrap": True}),
+         conn("fhost", "q", "d", "fully connected", dendrite="sf"),
+         conn("fslave", "g", "d", "fully connected", "sub", dendrite="sf"),
+         conn("root", "in", "d", "one to one")]
+    life = conn("life", "g", "g", "convergent", convolutional=True,
+                arbor={"field size": 3, "wrap": True},
+                weight={"type": "flat", "weight": 1, "diagonal": False})
+    weights = {c["name"]: st.normal(num_weights(c, L), stream=10 + i,
+                                    std=0.5)
+               for i, c in enumerate(C)}
+    gol0 = (st.bernoulli(36, 0.5, stream=5).astype(np.uint32) << 31)
+    inputs = {("so", "in"): lambda t: st.normal(36, step=t + 1, stream=2),
+              ("so", "q"): lambda t: st.uniform(36, step=t + 1, stream=3)}
+    return Case(name, net("so", L, C + [life]), steps, weights=weights,
+                inputs=inputs, init={("so", "g", "output"): gol0},
+                capture=[("so", "d", "output")],
+                final_neuron=[("so", "d", "input")],
+                final_conn=[(c, "weights") for c in ("host", "fhost")])
+
+
 def sparse_cases():
     return [sparse_base(), sparse_iz()]
 
@@ -681,4 +720,4 @@ def all_cases():
                      workload=107),
             iz_ops(), rate(), iz_init(), history(), gol_rule(),
             relay_ramp()] + ragged_cases() + cluster_cases() + sparse_cases() + \
-        [dendrites()]
+        [dendrites(), second_order()]

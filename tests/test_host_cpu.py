@@ -249,7 +249,7 @@ def _records(path):
 
 @pytest.mark.parametrize("name", ["iz_ops", "conn_zoo", "iz_sheet", "gol_small",
                                   "rate", "seq_iz", "sparse_base",
-                                  "sparse_iz"])
+                                  "sparse_iz", "dendrites", "second_order"])
 def test_state_file_records_match_reference(lib, ref_lib, tmp_path, name):
     """A freshly built state saves the same records, in the same order and
     sizes, as the reference's: index arrays used only during construction

@@ -91,6 +91,9 @@ PLANTED = [
     ("dendritic register folded with its parent's opcode",
      "        dst[i] = aggregate(opcode, dst[i], src[i]);",
      "        dst[i] = aggregate(ORC_ADD, dst[i], src[i]);", "dendrites"),
+    ("second-order slaves leave the host's weight copy alone",
+     "            sow[widx[v]] = aggregate(opcode, sow[widx[v]], val);",
+     "            (void)val;", "second_order"),
     ("Izhikevich settle loop stops on either variable",
      "} while (fabsf(delta_v) > 0.001 && fabsf(delta_r) > 0.001);",
      "} while (fabsf(delta_v) > 0.001 || fabsf(delta_r) > 0.001);",
