@@ -461,12 +461,15 @@ def adjoint_cpu_leg(cfg, x, w1, w2, labels):
         oloss, d1, d2 = snn.adjoint(X, W1, W2, L, ocfg)
         dt = time.perf_counter() - t0
     _, ocounts, _ = snn.forward(X, W1, W2, L, ocfg)
+    pairs = ((g1.astype(np.float64), d1), (g2.astype(np.float64), d2))
     cos = [float((g * d).sum() / np.linalg.norm(g) / np.linalg.norm(d))
-           for g, d in ((g1, d1), (g2, d2))]
+           for g, d in pairs]
+    rel = [float(np.linalg.norm(g - d) / np.linalg.norm(d)) for g, d in pairs]
     errors = {"loss_abs_error": abs(float(loss.detach()) - oloss),
               "counts_identical": float(np.mean(counts.cpu().numpy() ==
                                                 ocounts)),
-              "dw1_cosine": cos[0], "dw2_cosine": cos[1]}
+              "dw1_cosine": cos[0], "dw2_cosine": cos[1],
+              "dw1_rel_l2": rel[0], "dw2_rel_l2": rel[1]}
     cpu = {"value": 1.0 / dt, "unit": "iterations/s", "cores": 1,
            "kind": "port",
            "sample": "one full forward + adjoint iteration of the FP64 "

@@ -82,32 +82,49 @@ __device__ __forceinline__ u32x4 load8(const uint16_t *p, int rows, int cols,
 }
 
 // C[M][N] = A[M][K] . B[N][K]^T, BF16 operands, FP32 result.  64 x 64
-// tile per block, four wavefronts of 32 x 32 (2 x 2 MFMA 16x16x32 tiles).
+// tile per block, four wavefronts of 32 x 32 (2 x 2 MFMA 16x16x32 tiles),
+// 64 columns of K per barrier pair; the next slice's global loads are in
+// flight while the current one is multiplied.
 __global__ __launch_bounds__(256) void gemm_nt(const uint16_t *__restrict__ A,
                                                const uint16_t *__restrict__ B,
                                                float *__restrict__ C, int M,
                                                int N, int K) {
-    __shared__ __align__(16) uint16_t As[64][40];
-    __shared__ __align__(16) uint16_t Bs[64][40];
+    __shared__ __align__(16) uint16_t As[64][72];
+    __shared__ __align__(16) uint16_t Bs[64][72];
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    const int t = threadIdx.x, lr = t >> 2, lc = (t & 3) * 8;
+    const int t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
+    // chunk i of this thread: row (t + 256 i) / 8, columns ((t + 256 i) % 8) * 8
+    const int r0 = t >> 3, r1 = (t + 256) >> 3, kc = (t & 7) * 8;
+    u32x4 a0 = load8(A, M, K, m0 + r0, kc), a1 = load8(A, M, K, m0 + r1, kc);
+    u32x4 b0 = load8(B, N, K, n0 + r0, kc), b1 = load8(B, N, K, n0 + r1, kc);
     f32x4 acc[2][2] = {};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        *(u32x4 *)&As[lr][lc] = load8(A, M, K, m0 + lr, k0 + lc);
-        *(u32x4 *)&Bs[lr][lc] = load8(B, N, K, n0 + lr, k0 + lc);
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        *(u32x4 *)&As[r0][kc] = a0;
+        *(u32x4 *)&As[r1][kc] = a1;
+        *(u32x4 *)&Bs[r0][kc] = b0;
+        *(u32x4 *)&Bs[r1][kc] = b1;
         __syncthreads();
-        bf16x8 a[2], b[2];
-        for (int i = 0; i < 2; ++i)
-            a[i] = *(const bf16x8 *)&As[wm + i * 16 + li][lk];
-        for (int j = 0; j < 2; ++j)
-            b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + li][lk];
-        for (int i = 0; i < 2; ++i)
+        if (k0 + 64 < K) {
+            a0 = load8(A, M, K, m0 + r0, k0 + 64 + kc);
+            a1 = load8(A, M, K, m0 + r1, k0 + 64 + kc);
+            b0 = load8(B, N, K, n0 + r0, k0 + 64 + kc);
+            b1 = load8(B, N, K, n0 + r1, k0 + 64 + kc);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            bf16x8 a[2], b[2];
+            for (int i = 0; i < 2; ++i)
+                a[i] = *(const bf16x8 *)&As[wm + i * 16 + li][h * 32 + lk];
             for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                    a[i], b[j], acc[i][j], 0, 0, 0);
+                b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + li][h * 32 + lk];
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
     for (int i = 0; i < 2; ++i)
@@ -132,35 +149,65 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
     const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
     const int per = (M + gridDim.z - 1) / gridDim.z;
     const int rb = blockIdx.z * per, re = min(M, rb + per);
-    const int t = threadIdx.x, lr = t >> 3, lc = (t & 7) * 8;
+    const int t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
     const int wn = (w >> 1) * 32, wk = (w & 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
+    // staging: thread (row pair rp, 4 columns nc) loads rows rp, rp + 1 of
+    // its 4 columns of L and X (16 B and 8 B vector loads, coalesced across
+    // the 16 threads of a row) and stores each column's two rows as one
+    // 32-bit word of the transposed tiles
+    const int rp = (t >> 4) * 2, nc = (t & 15) * 4;
+    const bool lvec = (N & 3) == 0 && ((uintptr_t)L & 15) == 0;
+    const bool xvec = (K & 3) == 0 && ((uintptr_t)X & 7) == 0;
     f32x4 acc[2][2] = {};
     for (int r0 = rb; r0 < re; r0 += 32) {
-        const int r = r0 + lr;
-        for (int e = 0; e < 8; ++e) {
-            const int n = n0 + lc + e;
-            const float v = (r < re && n < N) ? L[(long)r * N + n] : 0.0f;
-            const uint32_t hi = bf16_rne(v);
-            const float rem = v - bf16_value(hi);
-            const uint32_t mid = bf16_rne(rem);
-            const uint32_t lo = bf16_rne(rem - bf16_value(mid));
-            Ah[0][lc + e][lr] = (uint16_t)hi;
-            Ah[1][lc + e][lr] = (uint16_t)mid;
-            Ah[2][lc + e][lr] = (uint16_t)lo;
+        float lv[2][4];
+        uint16_t xv[2][4];
+        for (int h = 0; h < 2; ++h) {
+            const int r = r0 + rp + h;
+            const bool in = r < re;
+            const float *lrow = L + (long)r * N + n0 + nc;
+            if (in && lvec && n0 + nc + 4 <= N) {
+                const f32x4 v = *(const f32x4 *)lrow;
+                for (int e = 0; e < 4; ++e) lv[h][e] = v[e];
+            } else {
+                for (int e = 0; e < 4; ++e)
+                    lv[h][e] = (in && n0 + nc + e < N) ? lrow[e] : 0.0f;
+            }
+            const uint16_t *xrow = X + (long)r * K + k0 + nc;
+            if (in && xvec && k0 + nc + 4 <= K) {
+                uint2 v = *(const uint2 *)xrow;
+                xv[h][0] = (uint16_t)v.x; xv[h][1] = (uint16_t)(v.x >> 16);
+                xv[h][2] = (uint16_t)v.y; xv[h][3] = (uint16_t)(v.y >> 16);
+            } else {
+                for (int e = 0; e < 4; ++e)
+                    xv[h][e] = (in && k0 + nc + e < K) ? xrow[e] : (uint16_t)0;
+            }
         }
-        const u32x4 xv = load8(X, re, K, r, k0 + lc);
-        uint16_t xe[8];
-        memcpy(xe, &xv, 16);
-        for (int e = 0; e < 8; ++e) Bs[lc + e][lr] = xe[e];
+        for (int e = 0; e < 4; ++e) {
+            uint32_t term[2][3];
+            for (int h = 0; h < 2; ++h) {
+                const float v = lv[h][e];
+                const uint32_t hi = bf16_rne(v);
+                const float rem = v - bf16_value(hi);
+                const uint32_t mid = bf16_rne(rem);
+                term[h][0] = hi;
+                term[h][1] = mid;
+                term[h][2] = bf16_rne(rem - bf16_value(mid));
+            }
+            for (int q = 0; q < 3; ++q)
+                *(uint32_t *)&Ah[q][nc + e][rp] = term[0][q] | (term[1][q] << 16);
+            *(uint32_t *)&Bs[nc + e][rp] =
+                (uint32_t)xv[0][e] | ((uint32_t)xv[1][e] << 16);
+        }
         __syncthreads();
         bf16x8 b[2];
         for (int j = 0; j < 2; ++j)
             b[j] = *(const bf16x8 *)&Bs[wk + j * 16 + li][lk];
-        for (int s = 0; s < 3; ++s)
+        for (int q = 0; q < 3; ++q)
             for (int i = 0; i < 2; ++i) {
-                const bf16x8 a = *(const bf16x8 *)&Ah[s][wn + i * 16 + li][lk];
+                const bf16x8 a = *(const bf16x8 *)&Ah[q][wn + i * 16 + li][lk];
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         a, b[j], acc[i][j], 0, 0, 0);
@@ -229,20 +276,22 @@ __global__ void scan1_fwd(ScanArgs a, const float *__restrict__ a1,
     }
 }
 
-// A2[row][o] = sum over u of W2[o][u] s1[row][u], one wavefront per row.
-// W2 is staged in LDS transposed, [u][16] BF16 (outputs padded to 16), so
-// a lane reads all outputs of its unit with two 16 B loads; lane partial
-// sums over u = lane + 64 j (j ascending), then a fixed butterfly across
-// the wavefront.
+// W2 transposed to [u][16] BF16 (outputs padded to 16 with zeros): a lane
+// reads all outputs of its unit with two 16 B loads.
+__global__ void transpose_w2(const uint16_t *__restrict__ w2,
+                             uint16_t *__restrict__ w2t, int H, int O) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * kMaxOut) return;
+    const int u = i / kMaxOut, o = i % kMaxOut;
+    w2t[i] = o < O ? w2[o * H + u] : (uint16_t)0;
+}
+
+// A2[row][o] = sum over u of W2[o][u] s1[row][u], one wavefront per row:
+// lane partial sums over u = lane + 64 j (j ascending), then a fixed
+// butterfly across the wavefront.
 __global__ __launch_bounds__(256) void out_current(
-    const float *__restrict__ s1, const uint16_t *__restrict__ w2,
+    const float *__restrict__ s1, const uint16_t *__restrict__ w2t,
     float *__restrict__ a2, int M, int H, int O) {
-    extern __shared__ __align__(16) uint16_t w2t[];   // [H][16]
-    for (int i = threadIdx.x; i < H * kMaxOut; i += blockDim.x) {
-        const int u = i % H, o = i / H;
-        w2t[u * kMaxOut + o] = o < O ? w2[o * H + u] : (uint16_t)0;
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= M) return;
@@ -458,7 +507,7 @@ size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 // Workspace carve-up, in bytes from the base.
 struct Layout {
     size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, lam,
-        g2, part2, part1, total;
+        g2, part2, part1, w2t, total;
 };
 
 Layout layout(const sg_snn_cfg &c) {
@@ -482,6 +531,7 @@ Layout layout(const sg_snn_cfg &c) {
     l.g2 = take(M * O);
     l.part2 = take((M + kDw2Rows - 1) / kDw2Rows * O * H);
     l.part1 = take((size_t)kTnSplit * H * c.n_in);
+    l.w2t = take((H * kMaxOut + 1) / 2);
     l.total = off;
     return l;
 }
@@ -493,8 +543,6 @@ int check(const sg_snn_cfg *c) {
         return fail("sizes must be positive");
     if (c->n_hid > 8192) return fail("n_hid above 8192");
     if (c->n_out > kMaxOut) return fail("n_out above 16");
-    if (c->n_hid > 2048)
-        return fail("n_hid above 2048 (W2 is staged in LDS)");
     if (c->batch > 16384) return fail("batch above 16384");
     if (c->checkpoint < 1 || c->steps % c->checkpoint != 0)
         return fail("checkpoint must divide steps");
@@ -537,9 +585,9 @@ void forward_block(const sg_snn_cfg &c, const ScanArgs &a, const uint16_t *xb,
     hipLaunchKernelGGL(scan1_fwd, dim3(blocks((long)c.batch * c.n_hid)),
                        dim3(256), 0, s, a, a1, in, out, s1,
                        record ? (float *)(ws + l.rv1) : nullptr);
-    hipLaunchKernelGGL(out_current, dim3((M + 3) / 4), dim3(256),
-                       (size_t)c.n_hid * kMaxOut * 2, s, s1, w2, a2, M,
-                       c.n_hid, c.n_out);
+    hipLaunchKernelGGL(out_current, dim3((M + 3) / 4), dim3(256), 0, s, s1,
+                       (const uint16_t *)(ws + l.w2t), a2, M, c.n_hid,
+                       c.n_out);
     hipLaunchKernelGGL(scan2_fwd, dim3(blocks((long)c.batch * c.n_out)),
                        dim3(256), 0, s, a, a2, in, out, cnt, count_mode,
                        record ? (float *)(ws + l.rv2) : nullptr);
@@ -574,6 +622,9 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     const long xblock = (long)c.checkpoint * c.batch * c.n_in;
     const ScanArgs a = scan_args(c);
     fill(state, slot, 0.0f, s);
+    hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)c.n_hid * kMaxOut)),
+                       dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), c.n_hid,
+                       c.n_out);
     for (int blk = 0; blk < c.steps / c.checkpoint; ++blk)
         forward_block(c, a, x + blk * xblock, w1, w2, l, ws,
                       state + blk * slot, state + (blk + 1) * slot,
@@ -611,6 +662,8 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     const ScanArgs a = scan_args(c);
     hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, s,
                        cnt, labels, B, O, grad_scale, g);
+    hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)H * kMaxOut)),
+                       dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), H, O);
     const int chunks = (M + kDw2Rows - 1) / kDw2Rows;
     fill(part1, (long)kTnSplit * H * c.n_in, 0.0f, s);
     fill(part2, (long)chunks * O * H, 0.0f, s);
