@@ -762,9 +762,12 @@ Props *Engine::run(const Props &args) {
     };
 
     // Split the pre-input activators over two streams only when both kinds
-    // are there (and no dendritic tree orders a layer's activators).
+    // are there (and no dendritic tree orders a layer's activators).  The
+    // activators that aggregate into a layer's input -- base activators and
+    // gap junctions -- all go to the second stream, in order.
     auto is_base = [&](const ConnState &cs) {
-        return (!cs.iz || cs.direct) && !cs.fused_gol;
+        return (!cs.iz || cs.direct || cs.iz_kind == SG_IZ_GAP) &&
+               !cs.fused_gol;
     };
     is_base_act = is_base;
     bool any_base = false, any_other = false;
@@ -778,7 +781,9 @@ Props *Engine::run(const Props &args) {
             (is_base(cs) ? any_base : any_other) = true;
         }
     }
-    const bool split_ok = any_base && any_other && !any_dendrites;
+    // "concurrent activators": "false" keeps every activator on one stream.
+    const bool split_ok = any_base && any_other && !any_dendrites &&
+                          args.get_bool("concurrent activators", true);
 
     // ---- main loop --------------------------------------------------------
     auto &layers = state_->layers();
@@ -1092,6 +1097,9 @@ Props *Engine::run(const Props &args) {
     r->set("engine buffer bytes", std::to_string(buf_bytes));
     r->set_child("args", args);
     r->set("interrupted", interrupted ? "true" : "false");
+    // 2: the input-writing activators ran on a second stream beside the
+    // conductance passes; 1: every activator on one stream.
+    r->set("activator streams", split_ok && !kt.on ? "2" : "1");
     Props mem;
     mem.set("device", std::to_string(device));
     mem.set("bytes", std::to_string(state_->network_bytes()));
