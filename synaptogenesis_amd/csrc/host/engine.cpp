@@ -344,11 +344,12 @@ Props *Engine::run(const Props &args) {
     hipStream_t side;
     check_hip(hipStreamCreate(&side), "hipStreamCreate");
     StreamGuard sg2_{side};
+    // The fork and join events order two streams of one device: no
+    // system-scope fence (that is for the host and other devices).
     hipEvent_t ev_fork, ev_join;
-    check_hip(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming),
-              "hipEventCreate");
-    check_hip(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming),
-              "hipEventCreate");
+    const unsigned ev_flags = hipEventDisableTiming | hipEventDisableSystemFence;
+    check_hip(hipEventCreateWithFlags(&ev_fork, ev_flags), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&ev_join, ev_flags), "hipEventCreate");
     struct EventGuard {
         hipEvent_t a, b;
         ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
