@@ -100,6 +100,19 @@ typedef struct sg_conn {
 enum { SG_IZ_W = 0, SG_IZ_R = 1, SG_IZ_SW = 2, SG_IZ_PRE = 3, SG_IZ_DW = 4,
        SG_IZ_ST = 5, SG_IZ_LT = 6, SG_IZ_FIELDS = 7 };
 
+/* The compact record: the same layout with four fields.  The presynaptic
+ * trace and the short-term plasticity pair (R, w) of a synapse change only
+ * in the updater, and there only from their own values and the source's
+ * spike (izhikevich_attributes.cpp:444-508); the updater skips synapses of
+ * weight below 1e-4 and clamps every weight it writes to at least 1e-4.  So
+ * every synapse of weight >= 1e-4 from one source carries the same three
+ * values if they were equal at the start, and they are kept once per source
+ * (sg_iz_syn.src_stp).  A synapse of weight below 1e-4 never changes its
+ * weight, derivative, trace or (R, w): its DW field holds its constant
+ * product R * w instead, and the host keeps its other values. */
+enum { SG_IZC_W = 0, SG_IZC_DW = 1, SG_IZC_ST = 2, SG_IZC_LT = 3,
+       SG_IZC_FIELDS = 4 };
+
 /* Per-synapse and per-source state of an Izhikevich conductance connection
  * (state/impl/izhikevich_attributes.h:45-66).  Tiled connection types use
  * `record` (interleaved, above); one-to-one connections use the separate
@@ -116,6 +129,10 @@ typedef struct sg_iz_syn {
     /* time since spike per source after the last activation (read by the
      * updater) */
     const int32_t *tss_cur;
+    /* NULL: `record` has SG_IZ_FIELDS fields.  Otherwise `record` is the
+     * compact record (SG_IZC_FIELDS) and this holds, per source, {R, w,
+     * presynaptic trace, unused} after the source's last update. */
+    const float *src_stp;
     float baseline_conductance;
     float learning_rate;
     float max_weight;
@@ -149,6 +166,12 @@ typedef struct sg_tss_out {
     const uint8_t *visits;
     const uint8_t *rev_delays;
     int word;                     /* history word read: delay / 32 */
+    /* A compact record's per-source {R, w, presynaptic trace} to step with
+     * the new time since spike this step (the connection learns), or
+     * NULL; stp_sub selects update_iz_sub's constants, stp_flag the
+     * connection's short-term plasticity (as sg_hip_iz_source_update). */
+    float *stp;
+    int stp_sub, stp_flag;
 } sg_tss_out;
 
 enum { SG_MAX_TSS_OUT = 2 };
@@ -226,6 +249,15 @@ int sg_hip_iz_update_activate(const sg_conn *c, const sg_iz_syn *s,
                               const float *src_table, const float *dst_table,
                               void *stream);
 
+/* One updater step of a compact record's per-source state: for every
+ * source f with src_spike = (tss[f] == 0), the presynaptic trace (update_iz_add
+ * only) and, with stp_flag, R and w, exactly as update_iz_add / update_iz_sub
+ * step them for each synapse (izhikevich_attributes.cpp:444-508).  stp holds
+ * 4 floats per source; tss is the time since spike after this step's
+ * activation. */
+int sg_hip_iz_source_update(float *stp, const int32_t *tss, int n, int sub,
+                            int stp_flag, void *stream);
+
 /* Neuron models (izhikevich_attributes.cpp:102-213,
  * game_of_life_attributes.cpp:14-33, rate_encoding_attributes.cpp:14-23,
  * relay_attributes.cpp:10-18) including the history shift
@@ -281,10 +313,12 @@ int sg_hip_tile32(const void *src, void *dst, int slots, int K, int nfields,
                   int field, int to_tiled, void *stream);
 
 /* Layout change between one [slots][K] host-layout array and field
- * `field` of an Izhikevich record (layout above); to_record != 0: host ->
- * record, else record -> host.  Lanes past `slots` are written as 0. */
+ * `field` of an Izhikevich record of `nfields` fields (SG_IZ_FIELDS, or
+ * SG_IZC_FIELDS for the compact record; layout above with nfields in place
+ * of SG_IZ_FIELDS); to_record != 0: host -> record, else record -> host.
+ * Lanes past `slots` are written as 0. */
 int sg_hip_record32(const void *src, void *dst, int slots, int K, int field,
-                    int to_record, void *stream);
+                    int nfields, int to_record, void *stream);
 
 /* Plain copy of n 32-bit words (a known-size reference for the hardware
  * byte counters). */

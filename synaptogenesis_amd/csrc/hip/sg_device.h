@@ -90,6 +90,28 @@ __device__ __forceinline__ int tss_next(int cur, int m, const uint8_t *rev,
     return t;
 }
 
+// The presynaptic trace (update_iz_add only) and the short-term plasticity
+// pair (R, w) of a synapse after one updater step: they depend only on
+// their own values and the source's spike (izhikevich_attributes.cpp:
+// 444-508).  The tile kernels step them per synapse; a compact record
+// steps them once per source.
+template <int SUB>
+__device__ __forceinline__ void iz_source_step(bool src_spike, int stp_flag,
+                                               float &pre, float &r,
+                                               float &sw) {
+    if (!SUB) pre = src_spike ? (float)0.004 : (float)((double)pre * 0.933);
+    if (stp_flag) {
+        const double U = SUB ? 0.2 : 0.5;
+        const float D = SUB ? 700.0f : 800.0f;
+        const double F = SUB ? 20.0 : 1000.0;
+        const float fs = (float)src_spike;
+        const float r0 = r, sw0 = sw;
+        r = r0 + (((1.0f - r0) / D) - (r0 * sw0 * fs));
+        sw = (float)((double)sw0 + (((U - (double)sw0) / F) +
+                                    (U * (double)(1.0f - sw0) * (double)fs)));
+    }
+}
+
 // Destination index of slot `slot` (SUBSET slots cover the to-rectangle).
 __device__ __forceinline__ int slot_to_index(const sg_conn &c, int slot) {
     if (c.type != SG_SUBSET) return slot;

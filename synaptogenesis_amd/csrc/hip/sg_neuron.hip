@@ -78,6 +78,20 @@ __device__ __forceinline__ float init_value(int dist, float a, float b,
 // the words before the shift) and the plane of the new spike bits.  Every
 // lane of the wavefront calls it (the plane is a wavefront ballot);
 // nid >= l.size only takes part in the ballot.
+// One updater step of a compact record's per-source {R, w, pre, -}.
+__device__ __forceinline__ void source_step(float *stp, long f, bool spike,
+                                            int sub, int stp_flag) {
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    f32x4 v = reinterpret_cast<const f32x4 *>(stp)[f];
+    float r = v[0], sw = v[1], pre = v[2];
+    if (sub) iz_source_step<1>(spike, stp_flag, pre, r, sw);
+    else iz_source_step<0>(spike, stp_flag, pre, r, sw);
+    v[0] = r;
+    v[1] = sw;
+    v[2] = pre;
+    reinterpret_cast<f32x4 *>(stp)[f] = v;
+}
+
 __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
     const bool valid = nid < l.size;
     uint32_t spike = 0u;
@@ -140,9 +154,11 @@ __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
 
         for (int t = 0; t < l.ntss; ++t) {
             const sg_tss_out &o = l.tss[t];
-            o.next[nid] = tss_next(o.cur[nid], o.visits[nid],
-                                   o.rev_delays + (long)nid * 32,
-                                   l.output[(long)o.word * l.size + nid]);
+            const int tn = tss_next(o.cur[nid], o.visits[nid],
+                                    o.rev_delays + (long)nid * 32,
+                                    l.output[(long)o.word * l.size + nid]);
+            o.next[nid] = tn;
+            if (o.stp) source_step(o.stp, nid, tn == 0, o.stp_sub, o.stp_flag);
         }
 
         spike = (double)voltage >= 30.0 ? 1u : 0u;
@@ -171,6 +187,12 @@ struct IzBatch {
     int first[SG_MAX_IZ_BATCH + 1];
     sg_iz_layer l[SG_MAX_IZ_BATCH];
 };
+
+__global__ __launch_bounds__(256) void source_update_kernel(
+        float *stp, const int32_t *tss, int n, int sub, int stp_flag) {
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f < n) source_step(stp, f, tss[f] == 0, sub, stp_flag);
+}
 
 __global__ __launch_bounds__(kNeuronBlock) void iz_neuron_kernel(IzBatch b) {
     int j = 0;
@@ -278,7 +300,7 @@ __global__ __launch_bounds__(kNeuronBlock) void tile32_kernel(
 
 __global__ __launch_bounds__(kNeuronBlock) void record32_kernel(
         const uint32_t *src, uint32_t *dst, int slots, int K, int field,
-        int to_record) {
+        int nfields, int to_record) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long tiles = (slots + 63) / 64;
     if (i >= tiles * 64 * K) return;
@@ -289,7 +311,7 @@ __global__ __launch_bounds__(kNeuronBlock) void record32_kernel(
     const long tile = tk / K;
     const long slot = tile * 64 + lane;
     const long r = ((((long)(k >> 3) * tiles + tile) * 8 + (k & 7)) *
-                        SG_IZ_FIELDS + field) * 64 + lane;
+                        nfields + field) * 64 + lane;
     if (slot >= slots) {
         if (to_record) dst[r] = 0u;
         return;
@@ -409,14 +431,26 @@ extern "C" int sg_hip_tile32(const void *src, void *dst, int slots, int K,
 }
 
 extern "C" int sg_hip_record32(const void *src, void *dst, int slots,
-                               int K, int field, int to_record,
+                               int K, int field, int nfields, int to_record,
                                void *stream) {
     if (slots <= 0 || K <= 0) return 0;
-    if (field < 0 || field >= SG_IZ_FIELDS) return (int)hipErrorInvalidValue;
+    if ((nfields != SG_IZ_FIELDS && nfields != SG_IZC_FIELDS) || field < 0 ||
+        field >= nfields)
+        return (int)hipErrorInvalidValue;
     const long n = (long)((slots + 63) / 64) * 64 * K;
     hipLaunchKernelGGL(record32_kernel, dim3(nblocks(n)), dim3(kNeuronBlock),
                        0, (hipStream_t)stream, (const uint32_t *)src,
-                       (uint32_t *)dst, slots, K, field, to_record);
+                       (uint32_t *)dst, slots, K, field, nfields, to_record);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_iz_source_update(float *stp, const int32_t *tss, int n,
+                                       int sub, int stp_flag, void *stream) {
+    if (n <= 0) return 0;
+    if (!stp || !tss || ((uintptr_t)stp & 15) != 0)
+        return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(source_update_kernel, dim3((n + 255) / 256), dim3(256),
+                       0, (hipStream_t)stream, stp, tss, n, sub, stp_flag);
     return hipGetLastError();
 }
 

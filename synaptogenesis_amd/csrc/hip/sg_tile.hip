@@ -1,278 +1,8 @@
-// Tile kernels for every per-destination connection type (FULLY_CONNECTED,
-// SUBSET, CONVERGENT, DIVERGENT): the base activator and the three
-// Izhikevich synapse passes (update, activate, fused update+activate).
-//
-// A block owns a tile of 64 destinations.  Its W wavefronts (W = 3 or 4
-// for the Izhikevich passes, chosen per launch so that the whole grid is
-// resident at once where possible) split the tile's synapse list k-wise in
-// chunks of 8 * W (8 visits per wavefront per chunk): every synapse's
-// independent work -- loads, STDP / short-term plasticity, trace decay,
-// stores -- runs with W wavefronts in flight per tile instead of one.  The
-// only order-dependent part, the FP32 sum over a destination's synapses, is
-// taken out of the per-synapse pass: each synapse writes the value it adds
-// to a double-buffered LDS chunk, and after a barrier one wavefront per sum
-// adds the chunk's values in k order.  The result is the reference's
-// sequential sum, bit for bit.
-//
-// Per-synapse arrays are in the 64-destination tiled layout (tiled_index),
-// so each wavefront load is one contiguous 256-byte line and each tile
-// streams one contiguous K * 256-byte block per array.  The Izhikevich
-// record is read and written once per step and is far larger than the
-// caches, so its loads and stores are non-temporal: the stream does not
-// evict the small, reused state (neuron arrays, spike words, other
-// connections' weights) from L2 and the Infinity Cache.
-#include <hip/hip_runtime.h>
+// Tile kernels, full records: the base activator's launcher and the
+// Izhikevich passes over records with every per-synapse field; compact
+// records go to sg_tile_compact.hip.
+#include "sg_tile_kernels.h"
 
-#include "sg_device.h"
-
-namespace sg {
-
-constexpr int kPerWave = 8;      // visits per wavefront per chunk
-constexpr int kSub = 8;
-
-template <int TYPE>
-__device__ __forceinline__ int visit_count(const sg_conn &c) {
-    return cursor<TYPE>(c, 0).count;
-}
-
-// ---------------------------------------------------------------------------
-// Base activator (synapse_kernel.h:887-912).  WAVES wavefronts share a tile
-// and CHUNK visits per barrier; small destination layers use more
-// wavefronts per tile to keep the GPU busy.
-template <int TYPE>
-__device__ __forceinline__ void ordered_sum_n(const float (*vals)[64],
-                                              SynapseIter<TYPE> &walk, int n,
-                                              int lane, float &sum) {
-    for (int j = 0; j < n; ++j) {
-        const bool valid = walk.source() >= 0;
-        walk.advance();
-        if (valid) sum += vals[j][lane];
-    }
-}
-
-template <int TYPE, int OUT, int WAVES, int CHUNK>
-__global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
-    constexpr int PER = CHUNK / WAVES;
-    __shared__ float vals[2][CHUNK][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = blockIdx.x * 64 + lane;
-    const bool live = slot < c.num_dest;
-    const int count = visit_count<TYPE>(c);
-    const int K = c.k_per_dest;
-    const bool skip = OUT == SG_OUT_BIT && c.skip_silent;
-    SynapseIter<TYPE> walk;
-    walk.init(c, live ? slot : 0);
-    float sum = 0.0f;
-    for (int k0 = 0, chunk = 0; k0 < count; k0 += CHUNK, ++chunk) {
-        const int buf = chunk & 1;
-        const int kw = k0 + wave * PER;
-        if (live && kw < count) {
-            SynapseIter<TYPE> it;
-            it.seek(c, slot, kw);
-            const float *wrow = c.convolutional ? c.weights + kw
-                                                : c.weights +
-                                                      tiled_index(slot, kw, K);
-            const int wstep = c.convolutional ? 1 : 64;
-            uint32_t x[PER];
-            int from[PER];
-#pragma unroll
-            for (int b = 0; b < PER; ++b) {
-                from[b] = it.k < count ? it.source() : -1;
-                if (it.k < count) it.advance();
-                if (from[b] >= 0) x[b] = c.src[from[b]];
-            }
-            if (skip) {
-#pragma unroll
-                for (int b = 0; b < PER; ++b)
-                    if (from[b] >= 0) {
-                        const bool on = (x[b] << (c.delay & 31)) >> 31;
-                        vals[buf][wave * PER + b][lane] =
-                            on ? wrow[b * wstep] : 0.0f;
-                    }
-            } else {
-                float w[PER];
-#pragma unroll
-                for (int b = 0; b < PER; ++b)
-                    if (from[b] >= 0) w[b] = wrow[b * wstep];
-#pragma unroll
-                for (int b = 0; b < PER; ++b)
-                    if (from[b] >= 0)
-                        vals[buf][wave * PER + b][lane] =
-                            extract<OUT>(x[b], c.delay) * w[b];
-            }
-        }
-        __syncthreads();
-        if (live && wave == 0)
-            ordered_sum_n(vals[buf], walk, min(CHUNK, count - k0), lane, sum);
-    }
-    if (live && wave == 0) {
-        const int to = slot_to_index(c, slot);
-        put_sum(c, to, sum);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Izhikevich synapse passes.  UPD: STDP / iSTDP + short-term plasticity
-// (izhikevich_attributes.cpp:444-508) of the previous step (fused) or of
-// this step (UPD only).  ACT: conductance activator
-// (izhikevich_attributes.cpp:257-284).
-// Non-temporal access to field f of the record lane q points at.
-__device__ __forceinline__ float rec_ld(const float *q, int f) {
-    return __builtin_nontemporal_load(&q[f * 64]);
-}
-__device__ __forceinline__ void rec_st(float v, float *q, int f) {
-    __builtin_nontemporal_store(v, &q[f * 64]);
-}
-
-template <int SUB>
-__device__ __forceinline__ void tile_update(const sg_iz_syn &s, float dest_spike,
-                                            float dest_exc_trace, int dest_tss,
-                                            const float *src_table,
-                                            const float *dst_table, int src_tss,
-                                            float &w, float &r, float &sw,
-                                            float &pre, float &dw) {
-    const bool src_spike = src_tss == 0;
-    float src_trace, dest_trace;
-    if (!SUB) {
-        pre = src_spike ? (float)0.004 : (float)((double)pre * 0.933);
-        src_trace = pre;
-        dest_trace = dest_exc_trace;
-    } else {
-        src_trace = (dest_spike > 0.0f && src_tss > 0 && src_tss < 32)
-            ? src_table[src_tss] : 0.0f;
-        dest_trace = (src_spike && dest_tss > 0 && dest_tss < 32)
-            ? dst_table[dest_tss] : 0.0f;
-    }
-    const double rhs = (((double)dw - 0.000001) * 0.0001 +
-                        (double)(dest_spike * src_trace)) -
-                       (double)((float)src_spike * dest_trace);
-    dw = (float)((double)dw - rhs);
-    const float nw = w + s.learning_rate * dw;
-    w = fmaxf(0.0001f, fminf(s.max_weight, nw));
-    if (s.stp_flag) {
-        const double U = SUB ? 0.2 : 0.5;
-        const float D = SUB ? 700.0f : 800.0f;
-        const double F = SUB ? 20.0 : 1000.0;
-        const float fs = (float)src_spike;
-        const float r0 = r, sw0 = sw;
-        r = r0 + (((1.0f - r0) / D) - (r0 * sw0 * fs));
-        sw = (float)((double)sw0 + (((U - (double)sw0) / F) +
-                                    (U * (double)(1.0f - sw0) * (double)fs)));
-    }
-}
-
-template <int TYPE, int KIND, int SUB, int UPD, int ACT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
-        sg_conn c, sg_iz_syn s, sg_iz_dest d, const float *src_table,
-        const float *dst_table) {
-    constexpr bool kLong = KIND == SG_IZ_ADD || KIND == SG_IZ_SUB;
-    constexpr int F = SG_IZ_FIELDS;
-    constexpr int CHUNK = WAVES * kPerWave;
-    __shared__ float vals[2][2][CHUNK][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int slot = blockIdx.x * 64 + lane;
-    const bool live = slot < c.num_dest;
-    const int to = live ? slot_to_index(c, slot) : 0;
-    const int K = c.k_per_dest;
-    float dest_spike = 0.0f, dest_trace = 0.0f;
-    int dest_tss = 0;
-    if (UPD && live) {
-        dest_spike = extract<SG_OUT_BIT>(d.out_word0[to], 0);
-        dest_trace = d.post_trace[to];
-        dest_tss = d.time_since_spike[to];
-    }
-    const int count = visit_count<TYPE>(c);
-    SynapseIter<TYPE> walk;
-    walk.init(c, live ? slot : 0);
-    float sum = 0.0f;
-    for (int k0 = 0, chunk = 0; k0 < count; k0 += CHUNK, ++chunk) {
-        const int buf = chunk & 1;
-        const int kw = k0 + wave * kPerWave;
-        if (live && kw < count) {
-            SynapseIter<TYPE> it;
-            it.seek(c, slot, kw);
-            // Record of visit kw (a multiple of 8) of this lane's
-            // destination: group kw / 8 of tile blockIdx.x; visit kw + b is
-            // b * F * 64 further, its delay byte b * 64 further.
-            float *rec = s.record +
-                ((((long)(kw >> 3) * gridDim.x + blockIdx.x) * 8) * F) * 64 +
-                lane;
-            const uint8_t *dly = s.delays +
-                (((long)(kw >> 3) * gridDim.x + blockIdx.x) * 8) * 64 + lane;
-            for (int h = 0; h < kPerWave; h += kSub) {
-            float w[kSub], r[kSub], sw[kSub], pre[kSub], dw[kSub], st[kSub],
-                lt[kSub];
-            uint32_t x[kSub];
-            int from[kSub], tss[kSub], dl[kSub];
-#pragma unroll
-            for (int b = 0; b < kSub; ++b) {
-                from[b] = it.k < count ? it.source() : -1;
-                if (it.k < count) it.advance();
-                if (from[b] < 0) continue;
-                const float *q = rec + (h + b) * F * 64;
-                w[b] = rec_ld(q, SG_IZ_W);
-                r[b] = rec_ld(q, SG_IZ_R);
-                sw[b] = rec_ld(q, SG_IZ_SW);
-                if (UPD) {
-                    dw[b] = rec_ld(q, SG_IZ_DW);
-                    if (!SUB) pre[b] = rec_ld(q, SG_IZ_PRE);
-                    tss[b] = s.tss_cur[from[b]];
-                }
-                if (ACT) {
-                    dl[b] = dly[(h + b) * 64];
-                    x[b] = c.src[from[b]];
-                    st[b] = rec_ld(q, SG_IZ_ST);
-                    if (kLong) lt[b] = rec_ld(q, SG_IZ_LT);
-                }
-            }
-#pragma unroll
-            for (int b = 0; b < kSub; ++b) {
-                if (from[b] < 0) continue;
-                float *q = rec + (h + b) * F * 64;
-                if (UPD && w[b] >= 0.0001f) {
-                    tile_update<SUB>(s, dest_spike, dest_trace, dest_tss,
-                                     src_table, dst_table, tss[b], w[b], r[b],
-                                     sw[b], pre[b], dw[b]);
-                    rec_st(w[b], q, SG_IZ_W);
-                    rec_st(dw[b], q, SG_IZ_DW);
-                    if (!SUB) rec_st(pre[b], q, SG_IZ_PRE);
-                    if (s.stp_flag) {
-                        rec_st(r[b], q, SG_IZ_R);
-                        rec_st(sw[b], q, SG_IZ_SW);
-                    }
-                }
-                if (ACT) {
-                    const float spike = extract<SG_OUT_BIT>(x[b], dl[b]);
-                    const float weight = w[b] * (r[b] * sw[b]);
-                    const float inc = spike * weight * s.baseline_conductance;
-                    const float st1 = st[b] + inc;
-                    vals[buf][0][wave * kPerWave + h + b][lane] = st1;
-                    rec_st(st1 * d.short_tau, q, SG_IZ_ST);
-                    if (kLong) {
-                        const float lt1 = lt[b] + inc;
-                        vals[buf][1][wave * kPerWave + h + b][lane] = lt1;
-                        rec_st(lt1 * d.long_tau, q, SG_IZ_LT);
-                    }
-                }
-            }
-            }
-        }
-        if (ACT) {
-            __syncthreads();
-            if (live && (wave == 0 || (kLong && wave == 1)))
-                ordered_sum_n(vals[buf][wave], walk, min(CHUNK, count - k0),
-                              lane, sum);
-        }
-    }
-    if (ACT && live) {
-        if (wave == 0) d.short_cond[to] += sum;
-        else if (kLong && wave == 1) d.long_cond[to] += sum;
-    }
-}
-
-}  // namespace sg
-
-// Launchers used by sg_synapse.hip for the tiled connection types.
 namespace sg {
 
 template <int WAVES, int CHUNK>
@@ -309,88 +39,24 @@ hipError_t launch_tile_base(const sg_conn &c, hipStream_t st) {
     return launch_tile_base_t<4, 32>(c, st);
 }
 
-// Number of compute units of the current device (cached per device).
-static int cu_count() {
-    static int cached[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (!cached[dev]) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount,
-                                  dev) != hipSuccess || n <= 0)
-            n = 256;
-        cached[dev] = n;
-    }
-    return cached[dev];
-}
-
-// Wavefronts per tile for a fused Izhikevich launch: the count that lets
-// the whole grid be resident in the fewest rounds of blocks (a grid that
-// overflows the GPU by a fraction runs its last round mostly idle), ties
-// going to more wavefronts per tile.
-template <class KA, class KB>
-static int pick_waves(KA *four, KB *three, int tiles) {
-    int nb4 = 0, nb3 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb4, reinterpret_cast<const void *>(four), 256, 0) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb3, reinterpret_cast<const void *>(three), 192, 0) != hipSuccess ||
-        nb4 <= 0 || nb3 <= 0)
-        return 4;
-    const long cus = cu_count();
-    const long r4 = (tiles + nb4 * cus - 1) / (nb4 * cus);
-    const long r3 = (tiles + nb3 * cus - 1) / (nb3 * cus);
-    return r3 < r4 ? 3 : 4;
-}
-
-template <int UPD, int ACT>
-static hipError_t launch_iz_tile_t(const sg_conn &c, const sg_iz_syn &s,
-                                   const sg_iz_dest &d, int kind, int sub,
-                                   const float *st_, const float *dt_,
-                                   hipStream_t st) {
-    const int tiles = (c.num_dest + 63) / 64;
-    dim3 g(tiles);
-#define SG_IZ(T, K, S)                                                       \
-    do {                                                                     \
-        auto *k4 = iz_tile_kernel<T, K, S, UPD, ACT, 4>;                     \
-        auto *k3 = iz_tile_kernel<T, K, S, UPD, ACT, 3>;                     \
-        if (pick_waves(k4, k3, tiles) == 3)                                  \
-            hipLaunchKernelGGL(k3, g, dim3(192), 0, st, c, s, d, st_, dt_);  \
-        else                                                                 \
-            hipLaunchKernelGGL(k4, g, dim3(256), 0, st, c, s, d, st_, dt_);  \
-    } while (0)
-#define SG_IZK(T, S)                                                         \
-    switch (kind) {                                                          \
-        case SG_IZ_ADD: SG_IZ(T, SG_IZ_ADD, S); break;                       \
-        case SG_IZ_SUB: SG_IZ(T, SG_IZ_SUB, S); break;                       \
-        case SG_IZ_MULT: SG_IZ(T, SG_IZ_MULT, S); break;                     \
-        case SG_IZ_REWARD: SG_IZ(T, SG_IZ_REWARD, S); break;                 \
-        default: return hipErrorInvalidValue;                                \
-    }
-#define SG_IZT(T)                                                            \
-    if (sub) { SG_IZK(T, 1) } else { SG_IZK(T, 0) }
-    switch (c.type) {
-        case SG_FULLY_CONNECTED: SG_IZT(SG_FULLY_CONNECTED); break;
-        case SG_SUBSET: SG_IZT(SG_SUBSET); break;
-        case SG_CONVERGENT: SG_IZT(SG_CONVERGENT); break;
-        case SG_DIVERGENT: SG_IZT(SG_DIVERGENT); break;
-        case SG_SPARSE: SG_IZT(SG_SPARSE); break;
-        default: return hipErrorInvalidValue;
-    }
-#undef SG_IZ
-#undef SG_IZK
-#undef SG_IZT
-    return hipGetLastError();
-}
+hipError_t launch_iz_tile_compact(const sg_conn &c, const sg_iz_syn &s,
+                                  const sg_iz_dest &d, int kind, int sub,
+                                  int upd, int act, const float *st_,
+                                  const float *dt_, hipStream_t st);
 
 hipError_t launch_iz_tile(const sg_conn &c, const sg_iz_syn &s,
                           const sg_iz_dest &d, int kind, int sub, int upd,
                           int act, const float *st_, const float *dt_,
                           hipStream_t st) {
-    if (upd && act) return launch_iz_tile_t<1, 1>(c, s, d, kind, sub, st_, dt_, st);
-    if (upd) return launch_iz_tile_t<1, 0>(c, s, d, sub ? SG_IZ_SUB : SG_IZ_ADD,
-                                           sub, st_, dt_, st);
-    return launch_iz_tile_t<0, 1>(c, s, d, kind, 0, st_, dt_, st);
+    if (s.src_stp)
+        return launch_iz_tile_compact(c, s, d, kind, sub, upd, act, st_, dt_,
+                                      st);
+    if (upd && act)
+        return launch_iz_tile_t<1, 1, 0>(c, s, d, kind, sub, st_, dt_, st);
+    if (upd)
+        return launch_iz_tile_t<1, 0, 0>(c, s, d, sub ? SG_IZ_SUB : SG_IZ_ADD,
+                                         sub, st_, dt_, st);
+    return launch_iz_tile_t<0, 1, 0>(c, s, d, kind, 0, st_, dt_, st);
 }
 
 }  // namespace sg

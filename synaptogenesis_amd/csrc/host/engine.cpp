@@ -223,7 +223,9 @@ Props *Engine::run(const Props &args) {
         for (auto &l : st.layers) in_seq.insert(l.get());
     }
 
-    state_->to_device();
+    // "compact records": "false" keeps every per-synapse field in the
+    // Izhikevich records uploaded by this run.
+    state_->to_device(args.get_bool("compact records", true));
 
     const bool learning = args.get_bool("learning flag", true);
     const bool suppress = args.get_bool("suppress output", false);
@@ -438,6 +440,7 @@ Props *Engine::run(const Props &args) {
         s.weight_derivatives = cs.arr("weight derivatives").df();
         s.delays = cs.d_delays;
         s.tss_cur = cs.d_tss[cs.tss_cur];
+        s.src_stp = cs.compact ? cs.d_src_stp : nullptr;
         s.baseline_conductance = cs.baseline_conductance;
         s.learning_rate = cs.learning_rate;
         s.max_weight = cs.conn->max_weight;
@@ -517,7 +520,18 @@ Props *Engine::run(const Props &args) {
                           "iz activate");
             });
         }
-        if (!tss_folded.count(&cs)) cs.tss_cur = 1 - cs.tss_cur;
+        if (!tss_folded.count(&cs)) {
+            cs.tss_cur = 1 - cs.tss_cur;
+            // A compact record's per-source state takes this step's
+            // updater step here (the neuron kernel does it for folded
+            // connections).
+            if (cs.compact && learning && cs.update_kind >= 0)
+                check_hip(sg_hip_iz_source_update(
+                              cs.d_src_stp, cs.d_tss[cs.tss_cur],
+                              cs.conn->from->size, cs.update_kind == 1,
+                              cs.stp_flag, st),
+                          "source update");
+        }
     };
     auto update = [&](ConnState &cs) {
         sg_conn d = conn_desc(cs);
@@ -905,6 +919,11 @@ Props *Engine::run(const Props &args) {
                             o.visits = cs->d_visits;
                             o.rev_delays = cs->d_rev_delays;
                             o.word = cs->conn->delay / 32;
+                            o.stp = (cs->compact && learning &&
+                                     cs->update_kind >= 0)
+                                ? cs->d_src_stp : nullptr;
+                            o.stp_sub = cs->update_kind == 1;
+                            o.stp_flag = cs->stp_flag;
                         }
                         auto fi = init_fold.find(ls);
                         if (fi != init_fold.end()) {
@@ -1101,6 +1120,10 @@ Props *Engine::run(const Props &args) {
     // 2: the input-writing activators ran on a second stream beside the
     // conductance passes; 1: every activator on one stream.
     r->set("activator streams", split_ok && !kt.on ? "2" : "1");
+    // Izhikevich connections whose records are compact (sg_hip.h SG_IZC_*).
+    int compact = 0;
+    for (auto &csp : state_->conns()) compact += csp->compact ? 1 : 0;
+    r->set("compact records", std::to_string(compact));
     Props mem;
     mem.set("device", std::to_string(device));
     mem.set("bytes", std::to_string(state_->network_bytes()));

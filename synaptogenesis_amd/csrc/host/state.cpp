@@ -115,6 +115,7 @@ State::~State() {
     for (auto &cs : conns_) {
         for (auto &kv : cs->arrays) free_dev(kv.second.dev);
         free_dev(cs->d_record);
+        free_dev(cs->d_src_stp);
         free_dev(cs->d_delays);
         free_dev(cs->d_rev_delays);
         free_dev(cs->d_visits);
@@ -982,7 +983,63 @@ static bool record_packed(const ConnState &cs) {
            conn_tiled(cs);
 }
 
-void State::to_device() {
+// Compact record fields: the arrays they hold (DW: see compact_dw).
+static const char *kCompactFields[SG_IZC_FIELDS] = {
+    "weights", "weight derivatives", "short trace", "long trace"};
+
+static bool active_weight(float w) { return w >= 0.0001f; }
+
+// Per source {R, w, presynaptic trace, 0} for a compact record: true when
+// every visited synapse of weight >= 1e-4 from one source carries the same
+// three values (bit for bit).  Sources without such a synapse keep the
+// first visited synapse's values (no active synapse reads them).
+static bool compact_sources(const ConnState &cs, std::vector<float> &out) {
+    const Connection &c = *cs.conn;
+    const int n = c.from->size;
+    out.assign((size_t)n * 4, 0.0f);
+    std::vector<uint8_t> seen(n, 0);
+    const float *w = const_cast<ConnState &>(cs).arr("weights").f();
+    const float *R = const_cast<ConnState &>(cs).arr("short term R").f();
+    const float *W = const_cast<ConnState &>(cs).arr("short term w").f();
+    const float *P = const_cast<ConnState &>(cs).arr("presyn trace").f();
+    bool ok = true;
+    enumerate_synapses(cs, [&](int, int, int, int r, int col, int, int,
+                               long wi) {
+        if (!ok) return;
+        const int f = r * c.from->cols + col;
+        const bool act = active_weight(w[wi]);
+        float *o = &out[(size_t)f * 4];
+        if (seen[f] < 2 && (act || !seen[f])) {
+            o[0] = R[wi];
+            o[1] = W[wi];
+            o[2] = P[wi];
+            seen[f] = act ? 2 : 1;
+            return;
+        }
+        if (act && (fbits(o[0]) != fbits(R[wi]) || fbits(o[1]) != fbits(W[wi]) ||
+                    fbits(o[2]) != fbits(P[wi])))
+            ok = false;
+    });
+    return ok;
+}
+
+// DW field of a compact record in host layout: the derivative of an active
+// synapse, R * w of a frozen one (weight below 1e-4: the updater never
+// touches it, and its activation weighs w * (R * w_stp)).
+static std::vector<float> compact_dw(const ConnState &cs) {
+    ConnState &m = const_cast<ConnState &>(cs);
+    const float *w = m.arr("weights").f();
+    const float *R = m.arr("short term R").f();
+    const float *W = m.arr("short term w").f();
+    const float *D = m.arr("weight derivatives").f();
+    std::vector<float> out(D, D + m.arr("weight derivatives").size());
+    enumerate_synapses(cs, [&](int, int, int, int, int, int, int, long wi) {
+        if (!active_weight(w[wi])) out[wi] = R[wi] * W[wi];
+    });
+    return out;
+}
+
+void State::to_device(bool allow_compact) {
     ensure_device();
     if (!on_host_) return;
     for (auto &ls : layers_) {
@@ -996,22 +1053,44 @@ void State::to_device() {
     }
     for (auto &cs : conns_) {
         cs->packed = record_packed(*cs);
+        cs->compact = false;
         if (cs->packed) {
+            std::vector<float> src;
+            cs->compact = allow_compact && compact_sources(*cs, src);
+            const int nf = cs->compact ? SG_IZC_FIELDS : SG_IZ_FIELDS;
             const size_t n = tiled_elems(cs->num_dest,
-                                         (cs->k_per_dest + 7) / 8 * 8) *
-                             SG_IZ_FIELDS;
+                                         (cs->k_per_dest + 7) / 8 * 8) * nf;
+            if (cs->d_record && cs->record_fields != nf) {
+                check_hip(hipFree(cs->d_record), "hipFree record");
+                cs->d_record = nullptr;
+            }
             if (!cs->d_record)
                 check_hip(hipMalloc((void **)&cs->d_record, n * 4),
                           "hipMalloc record");
-            for (int f = 0; f < SG_IZ_FIELDS; ++f) {
-                Array &a = cs->arr(kRecordFields[f]);
+            cs->record_fields = nf;
+            std::vector<float> dw;
+            if (cs->compact) dw = compact_dw(*cs);
+            for (int f = 0; f < nf; ++f) {
+                Array &a = cs->arr(cs->compact ? kCompactFields[f]
+                                               : kRecordFields[f]);
+                const void *h = (cs->compact && f == SG_IZC_DW)
+                    ? (const void *)dw.data() : (const void *)a.host.data();
                 void *tmp = scratch(a.bytes());
-                check_hip(hipMemcpy(tmp, a.host.data(), a.bytes(),
-                                    hipMemcpyHostToDevice), "hipMemcpy H2D");
+                check_hip(hipMemcpy(tmp, h, a.bytes(), hipMemcpyHostToDevice),
+                          "hipMemcpy H2D");
                 check_hip(sg_hip_record32(tmp, cs->d_record, cs->num_dest,
-                                          cs->k_per_dest, f, 1, nullptr),
+                                          cs->k_per_dest, f, nf, 1, nullptr),
                           "record upload");
                 check_hip(hipDeviceSynchronize(), "tile sync");
+            }
+            if (cs->compact) {
+                if (!cs->d_src_stp)
+                    check_hip(hipMalloc((void **)&cs->d_src_stp,
+                                        src.size() * 4),
+                              "hipMalloc source stp");
+                check_hip(hipMemcpy(cs->d_src_stp, src.data(), src.size() * 4,
+                                    hipMemcpyHostToDevice),
+                          "hipMemcpy source stp");
             }
         }
         for (auto &kv : cs->arrays) {
@@ -1035,6 +1114,43 @@ void State::to_device() {
     on_host_ = false;
 }
 
+// A compact record back to the host arrays: weights and traces from the
+// record; for every visited synapse of weight >= 1e-4 its derivative from
+// the DW field and R, w and the presynaptic trace from its source.  A
+// frozen synapse's derivative, R, w and trace never changed: the host
+// copies still hold them.
+void State::download_compact(ConnState &cs) {
+    std::vector<float> dw(cs.arr("weight derivatives").size());
+    for (int f = 0; f < SG_IZC_FIELDS; ++f) {
+        Array &a = cs.arr(kCompactFields[f]);
+        void *dst = f == SG_IZC_DW ? (void *)dw.data() : (void *)a.host.data();
+        void *tmp = scratch(a.bytes());
+        check_hip(sg_hip_record32(cs.d_record, tmp, cs.num_dest, cs.k_per_dest,
+                                  f, SG_IZC_FIELDS, 0, nullptr),
+                  "record download");
+        check_hip(hipMemcpy(dst, tmp, a.bytes(), hipMemcpyDeviceToHost),
+                  "hipMemcpy D2H");
+    }
+    const Connection &c = *cs.conn;
+    std::vector<float> src((size_t)c.from->size * 4);
+    check_hip(hipMemcpy(src.data(), cs.d_src_stp, src.size() * 4,
+                        hipMemcpyDeviceToHost), "hipMemcpy source stp");
+    const float *w = cs.arr("weights").f();
+    float *R = cs.arr("short term R").f();
+    float *W = cs.arr("short term w").f();
+    float *P = cs.arr("presyn trace").f();
+    float *D = cs.arr("weight derivatives").f();
+    enumerate_synapses(cs, [&](int, int, int, int r, int col, int, int,
+                               long wi) {
+        if (!active_weight(w[wi])) return;
+        const float *o = &src[(size_t)(r * c.from->cols + col) * 4];
+        R[wi] = o[0];
+        W[wi] = o[1];
+        P[wi] = o[2];
+        D[wi] = dw[wi];
+    });
+}
+
 void State::to_host() {
     if (on_host_) return;
     check_hip(hipDeviceSynchronize(), "download sync");
@@ -1044,12 +1160,15 @@ void State::to_host() {
         for (auto &kv : ls->vars) download(kv.second);
     }
     for (auto &cs : conns_) {
-        if (cs->packed) {
+        if (cs->packed && cs->compact) {
+            download_compact(*cs);
+        } else if (cs->packed) {
             for (int f = 0; f < SG_IZ_FIELDS; ++f) {
                 Array &a = cs->arr(kRecordFields[f]);
                 void *tmp = scratch(a.bytes());
                 check_hip(sg_hip_record32(cs->d_record, tmp, cs->num_dest,
-                                          cs->k_per_dest, f, 0, nullptr),
+                                          cs->k_per_dest, f, SG_IZ_FIELDS, 0,
+                                          nullptr),
                           "record download");
                 check_hip(hipMemcpy(a.host.data(), tmp, a.bytes(),
                                     hipMemcpyDeviceToHost), "hipMemcpy D2H");

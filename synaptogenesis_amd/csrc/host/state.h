@@ -80,6 +80,11 @@ struct ConnState {
     // device-only derived arrays
     float *d_record = nullptr;        // interleaved per-synapse record
     bool packed = false;              // iz arrays live in d_record
+    int record_fields = 0;            // fields d_record was allocated with
+    // Compact record (sg_hip.h SG_IZC_*): presynaptic trace and short-term
+    // plasticity kept per source in d_src_stp ({R, w, pre, -} per source).
+    bool compact = false;
+    float *d_src_stp = nullptr;
     uint8_t *d_delays = nullptr;      // delay & 31, synapse layout
     uint8_t *d_rev_delays = nullptr;  // [source][32] last visits, reversed
     uint8_t *d_visits = nullptr;      // min(32, visits per source)
@@ -111,7 +116,10 @@ class State {
     Array *weight_matrix(Connection *conn, const std::string &key);
 
     void to_host();
-    void to_device();
+    void download_compact(ConnState &cs);
+    // allow_compact: tiled Izhikevich conductance connections whose
+    // per-source values agree get the compact record.
+    void to_device(bool allow_compact = true);
     bool on_host() const { return on_host_; }
 
     std::vector<std::unique_ptr<LayerState>> &layers() { return layers_; }

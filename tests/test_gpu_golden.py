@@ -37,6 +37,33 @@ def test_golden_bit_exact(gpu_lib, name):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", [n for n in CASES if n not in TANH])
+def test_golden_bit_exact_full_records(gpu_lib, name):
+    """The same cases with every Izhikevich record kept whole: the compact
+    records (per-source presynaptic trace and short-term plasticity) that
+    the engine uses by default and the full records both match the
+    reference."""
+    case = CASES[name]
+    print("case", name, "full records", flush=True)
+    captures, finals, report = harness.run_case(
+        case, gpu_lib, {"compact records": "false"})
+    if case.steps:
+        assert report.get("compact records") == "0"
+    problems = golden_check.compare(case, captures, finals,
+                                    golden_check.load(name))
+    assert not problems, "\n".join(problems)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["iz_sheet", "iz_ops", "sparse_iz", "seq_iz"])
+def test_golden_cases_use_compact_records(gpu_lib, name):
+    case = CASES[name]
+    print("case", name, "compact", flush=True)
+    _, _, report = harness.run_case(case, gpu_lib)
+    assert int(report.get("compact records")) >= 1
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", TANH)
 def test_golden_rate_encoding(gpu_lib, name):
     """rate_encoding uses tanh; the reference calls glibc tanhf, the kernel
