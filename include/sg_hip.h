@@ -131,7 +131,9 @@ typedef struct sg_iz_syn {
     const int32_t *tss_cur;
     /* NULL: `record` has SG_IZ_FIELDS fields.  Otherwise `record` is the
      * compact record (SG_IZC_FIELDS) and this holds, per source, {R, w,
-     * presynaptic trace, unused} after the source's last update. */
+     * presynaptic trace, time since spike (int32 bits)} after the source's
+     * last update: the update reads the time since spike from here instead
+     * of tss_cur. */
     const float *src_stp;
     float baseline_conductance;
     float learning_rate;
@@ -252,8 +254,9 @@ int sg_hip_iz_update_activate(const sg_conn *c, const sg_iz_syn *s,
 /* One updater step of a compact record's per-source state: for every
  * source f with src_spike = (tss[f] == 0), the presynaptic trace (update_iz_add
  * only) and, with stp_flag, R and w, exactly as update_iz_add / update_iz_sub
- * step them for each synapse (izhikevich_attributes.cpp:444-508).  stp holds
- * 4 floats per source; tss is the time since spike after this step's
+ * step them for each synapse (izhikevich_attributes.cpp:444-508), and
+ * stores tss[f] beside them.  stp holds 4 values per source
+ * (sg_iz_syn.src_stp); tss is the time since spike after this step's
  * activation. */
 int sg_hip_iz_source_update(float *stp, const int32_t *tss, int n, int sub,
                             int stp_flag, void *stream);

@@ -78,17 +78,20 @@ __device__ __forceinline__ float init_value(int dist, float a, float b,
 // the words before the shift) and the plane of the new spike bits.  Every
 // lane of the wavefront calls it (the plane is a wavefront ballot);
 // nid >= l.size only takes part in the ballot.
-// One updater step of a compact record's per-source {R, w, pre, -}.
-__device__ __forceinline__ void source_step(float *stp, long f, bool spike,
+// One updater step of a compact record's per-source {R, w, pre, tss}:
+// tss is the time since spike the step read (the updater's next pass
+// reads it from here).
+__device__ __forceinline__ void source_step(float *stp, long f, int tss,
                                             int sub, int stp_flag) {
     typedef float f32x4 __attribute__((ext_vector_type(4)));
     f32x4 v = reinterpret_cast<const f32x4 *>(stp)[f];
     float r = v[0], sw = v[1], pre = v[2];
-    if (sub) iz_source_step<1>(spike, stp_flag, pre, r, sw);
-    else iz_source_step<0>(spike, stp_flag, pre, r, sw);
+    if (sub) iz_source_step<1>(tss == 0, stp_flag, pre, r, sw);
+    else iz_source_step<0>(tss == 0, stp_flag, pre, r, sw);
     v[0] = r;
     v[1] = sw;
     v[2] = pre;
+    v[3] = __int_as_float(tss);
     reinterpret_cast<f32x4 *>(stp)[f] = v;
 }
 
@@ -158,7 +161,7 @@ __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
                                     o.rev_delays + (long)nid * 32,
                                     l.output[(long)o.word * l.size + nid]);
             o.next[nid] = tn;
-            if (o.stp) source_step(o.stp, nid, tn == 0, o.stp_sub, o.stp_flag);
+            if (o.stp) source_step(o.stp, nid, tn, o.stp_sub, o.stp_flag);
         }
 
         spike = (double)voltage >= 30.0 ? 1u : 0u;
@@ -191,7 +194,7 @@ struct IzBatch {
 __global__ __launch_bounds__(256) void source_update_kernel(
         float *stp, const int32_t *tss, int n, int sub, int stp_flag) {
     const int f = blockIdx.x * 256 + threadIdx.x;
-    if (f < n) source_step(stp, f, tss[f] == 0, sub, stp_flag);
+    if (f < n) source_step(stp, f, tss[f], sub, stp_flag);
 }
 
 __global__ __launch_bounds__(kNeuronBlock) void iz_neuron_kernel(IzBatch b) {

@@ -160,8 +160,8 @@ __device__ __forceinline__ void tile_update(const sg_iz_syn &s, float dest_spike
     w = fmaxf(0.0001f, fminf(s.max_weight, nw));
 }
 
-// COMPACT: the record has the SG_IZC_* fields and pre, R and w come from
-// the per-source array s.src_stp.
+// COMPACT: the record has the SG_IZC_* fields; pre, R, w and the time
+// since spike come from the per-source array s.src_stp.
 template <int TYPE, int KIND, int SUB, int UPD, int ACT, int WAVES,
           int COMPACT>
 __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
@@ -223,14 +223,17 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
                     r[b] = v[0];
                     sw[b] = v[1];
                     pre[b] = v[2];
+                    if (UPD) tss[b] = __float_as_int(v[3]);
                 } else {
                     r[b] = rec_ld(q, SG_IZ_R);
                     sw[b] = rec_ld(q, SG_IZ_SW);
                 }
                 if (UPD) {
                     dw[b] = rec_ld(q, FDW);
-                    if (!COMPACT && !SUB) pre[b] = rec_ld(q, SG_IZ_PRE);
-                    tss[b] = s.tss_cur[from[b]];
+                    if (!COMPACT) {
+                        if (!SUB) pre[b] = rec_ld(q, SG_IZ_PRE);
+                        tss[b] = s.tss_cur[from[b]];
+                    }
                 }
                 if (ACT) {
                     dl[b] = dly[(h + b) * 64];
