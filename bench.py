@@ -47,6 +47,10 @@ TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles",
 # weight derivative (7 x 8 B) + R delay (4 B) = 60 B; per source neuron the
 # history word (4 B) and time-since-spike RW (8 B); per destination the
 # history word, post trace, time since spike (12 B) and ampa/nmda RW (16 B).
+# These are the reference's element types (the denominator stays fixed); the
+# kernel itself streams the compact record (weight, derivative, short and
+# long trace RW + a uint8 delay = 33 B per synapse), which the measured
+# `traffic` shows.
 def fused_bytes(ne):
     return ne * 225 * 60 + ne * 12 + ne * 28
 
@@ -579,6 +583,7 @@ def main():
     kernels = {k[:-3]: float(v) / float(timing[k[:-3] + " launches"])
                for k, v in timing.items() if k.endswith(" ms")}
 
+    traffic = measured_traffic() if a.workload == "sheet" else None
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(wl, a.cpu_steps or spec["cpu_steps"])
@@ -601,8 +606,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": spec["label"],
                      "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": (measured_traffic()
-                                 if a.workload == "sheet" else None),
+                     "traffic": traffic,
+                     # HBM bytes actually moved per launch over the
+                     # launch time: below `achieved` when the kernel's
+                     # layout moves fewer bytes than the reference's
+                     # element types count
+                     "traffic_GBps": (traffic / (kernel_ms * 1e-3) / 1e9
+                                      if traffic else None),
                      "traffic_source": (os.path.relpath(TRAFFIC_SUMMARY, ROOT)
                                         if a.workload == "sheet" else None),
                      "avg_ms": kernel_ms},

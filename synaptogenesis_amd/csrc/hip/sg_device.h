@@ -314,7 +314,11 @@ struct SynapseIter {
         rstep = q.rstep; cstep = q.cstep;
         wrap = (TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) && cn.wrap;
         r = q.row0; c = q.col0;
-        if (wrap) { r = wrap_index(r, rows); c = wrap_index(c, cols); }
+        if (wrap) {
+            r = wrap_index(r, rows); c = wrap_index(c, cols);
+            // steps taken modulo the layer: one subtraction wraps a step
+            rstep = wrap_index(rstep, rows); cstep = wrap_index(cstep, cols);
+        }
         c0 = c;
         if (TYPE == SG_ONE_TO_ONE) { wi = slot; wstep = 0; }
         else if ((TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) &&
@@ -330,8 +334,8 @@ struct SynapseIter {
         k = kk;
         kc = kk % ncols;
         const int kr = kk / ncols;
-        r = q.row0 + kr * rstep;
-        c = q.col0 + kc * cstep;
+        r = q.row0 + kr * q.rstep;
+        c = q.col0 + kc * q.cstep;
         if (wrap) { r = wrap_index(r, rows); c = wrap_index(c, cols); }
         wi += (long)kk * wstep;
     }
@@ -349,11 +353,11 @@ struct SynapseIter {
         if (++kc == ncols) {
             kc = 0;
             r += rstep;
-            if (wrap) while (r >= rows) r -= rows;
+            if (wrap && r >= rows) r -= rows;
             c = c0;
         } else {
             c += cstep;
-            if (wrap) while (c >= cols) c -= cols;
+            if (wrap && c >= cols) c -= cols;
         }
     }
 };

@@ -29,6 +29,8 @@
 // connections' weights) from L2 and the Infinity Cache.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sg_device.h"
 
 namespace sg {
@@ -188,6 +190,9 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
         dest_tss = d.time_since_spike[to];
     }
     const int count = visit_count<TYPE>(c);
+    // No visit is ever skipped: every k < count has a source.
+    const bool noskip = TYPE == SG_FULLY_CONNECTED || TYPE == SG_SUBSET ||
+        ((TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) && c.wrap);
     SynapseIter<TYPE> walk;
     walk.init(c, live ? slot : 0);
     float sum = 0.0f;
@@ -205,16 +210,25 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
                 lane;
             const uint8_t *dly = s.delays +
                 (((long)(kw >> 3) * gridDim.x + blockIdx.x) * 8) * 64 + lane;
-            for (int h = 0; h < kPerWave; h += kSub) {
+            // One batch of kSub visits.  CHECK: some visit may be skipped
+            // (past the count, or a source outside a non-wrapping layer);
+            // otherwise every visit is valid and no lane branches.
+            auto batch = [&](int h, auto check) {
+            constexpr bool CHECK = decltype(check)::value;
             float w[kSub], r[kSub], sw[kSub], pre[kSub], dw[kSub], st[kSub],
                 lt[kSub];
             uint32_t x[kSub];
             int from[kSub], tss[kSub], dl[kSub];
 #pragma unroll
             for (int b = 0; b < kSub; ++b) {
-                from[b] = it.k < count ? it.source() : -1;
-                if (it.k < count) it.advance();
-                if (from[b] < 0) continue;
+                if (CHECK) {
+                    from[b] = it.k < count ? it.source() : -1;
+                    if (it.k < count) it.advance();
+                    if (from[b] < 0) continue;
+                } else {
+                    from[b] = it.source();
+                    it.advance();
+                }
                 const float *q = rec + (h + b) * F * 64;
                 w[b] = rec_ld(q, FW);
                 if (COMPACT) {
@@ -244,17 +258,24 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
             }
 #pragma unroll
             for (int b = 0; b < kSub; ++b) {
-                if (from[b] < 0) continue;
+                if (CHECK && from[b] < 0) continue;
                 float *q = rec + (h + b) * F * 64;
                 // The updater leaves a synapse of weight below 1e-4 alone
                 // and never writes a weight below it: the test is the same
-                // before and after the update.
+                // before and after the update.  The update is computed for
+                // every lane and kept where it applies; a frozen synapse
+                // writes back the values it read.
                 const bool active = w[b] >= 0.0001f;
-                if (UPD && active) {
+                if (UPD) {
+                    float w1 = w[b], r1 = r[b], sw1 = sw[b], pre1 = pre[b],
+                          dw1 = dw[b];
                     tile_update<SUB, COMPACT>(s, dest_spike, dest_trace,
                                               dest_tss, src_table, dst_table,
-                                              tss[b], w[b], r[b], sw[b],
-                                              pre[b], dw[b]);
+                                              tss[b], w1, r1, sw1, pre1, dw1);
+                    if (active) {
+                        w[b] = w1; dw[b] = dw1; pre[b] = pre1;
+                        r[b] = r1; sw[b] = sw1;
+                    }
                     rec_st(w[b], q, FW);
                     rec_st(dw[b], q, FDW);
                     if (!COMPACT) {
@@ -283,13 +304,24 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
                     }
                 }
             }
+            };
+            for (int h = 0; h < kPerWave; h += kSub) {
+                if (noskip && kw + h + kSub <= count)
+                    batch(h, std::false_type{});
+                else
+                    batch(h, std::true_type{});
             }
         }
         if (ACT) {
             __syncthreads();
-            if (live && (wave == 0 || (kLong && wave == 1)))
-                ordered_sum_n(vals[buf][wave], walk, min(CHUNK, count - k0),
-                              lane, sum);
+            if (live && (wave == 0 || (kLong && wave == 1))) {
+                const int n = min(CHUNK, count - k0);
+                if (noskip) {
+                    for (int j = 0; j < n; ++j) sum += vals[buf][wave][j][lane];
+                } else {
+                    ordered_sum_n(vals[buf][wave], walk, n, lane, sum);
+                }
+            }
         }
     }
     if (ACT && live) {
