@@ -613,6 +613,8 @@ def main():
                      # element types count
                      "traffic_GBps": (traffic / (kernel_ms * 1e-3) / 1e9
                                       if traffic else None),
+                     "traffic_frac": (traffic / (kernel_ms * 1e-3) / 1e9 /
+                                      HBM_PEAK_GBPS if traffic else None),
                      "traffic_source": (os.path.relpath(TRAFFIC_SUMMARY, ROOT)
                                         if a.workload == "sheet" else None),
                      "avg_ms": kernel_ms},
