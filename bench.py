@@ -95,6 +95,16 @@ def _workloads():
                              "15x15 s2 direct, I->E div 9x9 s2 direct)",
                       data="synthetic (Philox-seeded weights U[0,0.5), "
                            "delays U{0..31}; device-drawn N(0,3) noise)"),
+        "sparse": dict(make=workloads.sparse_sheet,
+                       kernel="iz_update_activate",
+                       label="iz_update_activate (E->E sparse)",
+                       launch_bytes=None, step_bytes=None, cpu_steps=20,
+                       detail=" (E->E SPARSE from conv 15x15 s1 wrap STDP+STP,"
+                              " E->I conv 15x15 s2 direct, I->E div 9x9 s2 "
+                              "direct)",
+                       data="synthetic (Philox-seeded weights U[0,0.5) kept "
+                            "with probability 0.5, delays U{0..31}; "
+                            "device-drawn N(0,3) noise)"),
         # weights 4 B/synapse + source words + destination input RW
         "izfc": dict(make=workloads.izfc, kernel="activate_base",
                      label="activate_base (L1->L2 fully connected)",
@@ -579,7 +589,9 @@ def main():
     syn_per_s = wl["visits_per_step"] * a.steps / dt
     key = spec["kernel"]
     kernel_ms = float(timing[key + " ms"]) / float(timing[key + " launches"])
-    achieved = spec["launch_bytes"] / (kernel_ms * 1e-3) / 1e9
+    launch_bytes = wl.get("launch_bytes", spec["launch_bytes"])
+    step_bytes_ = wl.get("step_bytes", spec["step_bytes"])
+    achieved = launch_bytes / (kernel_ms * 1e-3) / 1e9
     kernels = {k[:-3]: float(v) / float(timing[k[:-3] + " launches"])
                for k, v in timing.items() if k.endswith(" ms")}
 
@@ -601,7 +613,7 @@ def main():
                    "synapse_visits_per_step": wl["visits_per_step"],
                    "neurons": wl["neurons"]},
         "neuron_steps_per_s": wl["neurons"] * a.steps / dt,
-        "step_hbm_fraction": spec["step_bytes"] / (ms_per_step * 1e-3) /
+        "step_hbm_fraction": step_bytes_ / (ms_per_step * 1e-3) /
                              1e9 / HBM_PEAK_GBPS,
         "roofline": {"bound": "hbm", "kernel": spec["label"],
                      "achieved": achieved, "peak": HBM_PEAK_GBPS,

@@ -87,6 +87,10 @@ typedef struct sg_conn {
      * (k_per_dest = the largest nonzero count), -1 past the slot's count
      * (synapse_kernel.h:305-366 loops over nonzero_counts[to]). */
     const int32_t *sparse_from;
+    /* SG_SPARSE, optional: per 64-destination tile, the fewest and the most
+     * nonzero visits of its slots ({lo, hi} per tile); a tile's walk ends
+     * at its hi, and visits below its lo need no check. */
+    const int32_t *tile_counts;
 } sg_conn;
 
 /* Fields of the interleaved per-synapse record of a tiled Izhikevich

@@ -189,10 +189,16 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
         dest_trace = d.post_trace[to];
         dest_tss = d.time_since_spike[to];
     }
-    const int count = visit_count<TYPE>(c);
+    int count = visit_count<TYPE>(c);
     // No visit is ever skipped: every k < count has a source.
     const bool noskip = TYPE == SG_FULLY_CONNECTED || TYPE == SG_SUBSET ||
         ((TYPE == SG_CONVERGENT || TYPE == SG_DIVERGENT) && c.wrap);
+    // Visits below `sure` have a source in every lane of the tile.
+    int sure = noskip ? count : 0;
+    if (TYPE == SG_SPARSE && c.tile_counts) {
+        sure = c.tile_counts[2 * blockIdx.x];
+        count = c.tile_counts[2 * blockIdx.x + 1];
+    }
     SynapseIter<TYPE> walk;
     walk.init(c, live ? slot : 0);
     float sum = 0.0f;
@@ -306,7 +312,7 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
             }
             };
             for (int h = 0; h < kPerWave; h += kSub) {
-                if (noskip && kw + h + kSub <= count)
+                if (kw + h + kSub <= sure)
                     batch(h, std::false_type{});
                 else
                     batch(h, std::true_type{});
@@ -316,8 +322,12 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
             __syncthreads();
             if (live && (wave == 0 || (kLong && wave == 1))) {
                 const int n = min(CHUNK, count - k0);
-                if (noskip) {
+                if (k0 + n <= sure) {
                     for (int j = 0; j < n; ++j) sum += vals[buf][wave][j][lane];
+                    if (TYPE == SG_SPARSE) {   // keep the walk in step
+                        walk.k += n;
+                        walk.wi += (long)n * walk.wstep;
+                    }
                 } else {
                     ordered_sum_n(vals[buf][wave], walk, n, lane, sum);
                 }

@@ -419,6 +419,7 @@ Props *Engine::run(const Props &args) {
         }
         d.store_sum = store_sum.count(&cs) ? 1 : 0;
         d.sparse_from = cs.d_sparse_from;
+        d.tile_counts = cs.d_tile_counts;
         // A second-order host activates with its slaves' copy of its
         // weights (synapse_data.cpp:20-23), which may hold any value.
         if (c.so_host) {

@@ -123,6 +123,7 @@ State::~State() {
         free_dev(cs->d_tss[1]);
         free_dev(cs->d_plane);
         free_dev(cs->d_sparse_from);
+        free_dev(cs->d_tile_counts);
     }
     free_dev(scratch_);
     free_dev(d_tables_);
@@ -906,6 +907,25 @@ void State::upload_conn_derived(ConnState &cs) {
                   "hipMalloc sparse");
         check_hip(hipMemcpy(cs.d_sparse_from, idx.data(), idx.size() * 4,
                             hipMemcpyHostToDevice), "hipMemcpy sparse");
+        // per 64-destination tile: the fewest and the most nonzero visits
+        // of its slots
+        const int tiles = (cs.num_dest + 63) / 64;
+        std::vector<int32_t> tc((size_t)tiles * 2);
+        for (int t = 0; t < tiles; ++t) {
+            int lo = K, hi = 0;
+            for (int slot = t * 64; slot < std::min(cs.num_dest, t * 64 + 64);
+                 ++slot) {
+                lo = std::min(lo, cnt[slot]);
+                hi = std::max(hi, cnt[slot]);
+            }
+            tc[(size_t)t * 2] = lo;
+            tc[(size_t)t * 2 + 1] = hi;
+        }
+        if (cs.d_tile_counts) check_hip(hipFree(cs.d_tile_counts), "hipFree");
+        check_hip(hipMalloc((void **)&cs.d_tile_counts, tc.size() * 4),
+                  "hipMalloc tile counts");
+        check_hip(hipMemcpy(cs.d_tile_counts, tc.data(), tc.size() * 4,
+                            hipMemcpyHostToDevice), "hipMemcpy tile counts");
     }
     if ((!cs.iz || cs.direct) && cs.from->out_type == SG_OUT_BIT &&
         !cs.d_plane)

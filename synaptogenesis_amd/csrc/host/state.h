@@ -91,6 +91,7 @@ struct ConnState {
     int32_t *d_tss[2] = {nullptr, nullptr};
     uint64_t *d_plane = nullptr;      // spike plane of a BIT source (base)
     int32_t *d_sparse_from = nullptr; // SG_SPARSE source indices, tiled
+    int32_t *d_tile_counts = nullptr; // SG_SPARSE {fewest, most} per tile
     int tss_cur = 0;
     bool pending_update = false;
     bool fused_gol = false;   // handled by sg_hip_gol_step

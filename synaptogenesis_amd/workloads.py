@@ -7,6 +7,8 @@ seeded weights and delays injected through the C-ABI callbacks.
   per-synapse delays U{0..31}; E->I convergent 15x15 stride 2 wrap and
   I->E divergent 9x9 stride 2 wrap (sub), both direct; weights U[0,0.5);
   thalamic noise N(0, 3) into E and I every step.
+* ``sparse_sheet`` -- the same sheet with E->E as a SPARSE connection
+  (half of its convergent weights zero and dropped).
 * ``gol``    -- game of life on a square board, one 3x3 convolutional
   self-connection with wrap, weight 1 with the centre cleared.
 * ``izfc``   -- two Izhikevich layers of 1000, FULLY_CONNECTED direct,
@@ -82,6 +84,44 @@ def sheet(e_rows=256, e_cols=320, noise_std=3.0):
                          ni, np.float32(0.2) * np.float32(-65.0), np.float32)},
             "name": "izhikevich E/I sheet E%dx%d I%dx%d" % (
                 e_rows, e_cols, i_rows, i_cols)}
+
+
+def sparse_sheet(e_rows=256, e_cols=320, density=0.5, noise_std=3.0):
+    """The E/I sheet with a SPARSE E->E connection: each of its 15x15
+    convergent weights is kept with probability `density` (U[0,0.5),
+    otherwise 0), and "sparse": "true" drops the zeros after weight
+    initialisation (weight_matrix.cpp:510-663), leaving each destination a
+    compacted list of its nonzero synapses.  Adds to the dict:
+    `nonzero` (E->E synapses kept), `launch_bytes` (the fused E->E pass by
+    the reference's element types: 60 B per kept synapse for the seven FP32
+    per-synapse arrays read and written and the int32 delay, 12 B per kept
+    synapse for the three int32 index arrays the sparse loop reads, 4 B per
+    destination for its nonzero count, plus the per-source and
+    per-destination words of the dense pass) and `step_bytes`."""
+    wl = sheet(e_rows, e_cols, noise_std)
+    ne, ni = e_rows * e_cols, (e_rows // 2) * (e_cols // 2)
+    for c in wl["net"]["connections"]:
+        if c["name"] == "EE":
+            c["sparse"] = "true"
+    st = Stream(6)
+    n = ne * 225
+    keep = st.uniform(n, stream=30) < density
+    nonzero = int(keep.sum())
+
+    def ee():
+        w = st.uniform(n, stream=10, low=0.0, high=0.5)
+        w[~keep] = 0.0
+        return w
+    wl["weights"]["EE"] = ee
+    wl["nonzero"] = nonzero
+    wl["visits_per_step"] = nonzero + ni * 225 + ne * 16
+    wl["launch_bytes"] = nonzero * 72 + ne * 4 + ne * 12 + ne * 28
+    wl["step_bytes"] = (nonzero * 72 + ne * 4 + ni * 225 * 4 + ne * 16 * 4 +
+                        (ne + ni) * 112 + ne * 8 + (ne + ne + ni) * 4)
+    wl["name"] = ("izhikevich E/I sheet E%dx%d I%dx%d, sparse E->E "
+                  "(density %g)" % (e_rows, e_cols, e_rows // 2, e_cols // 2,
+                                    density))
+    return wl
 
 
 def gol(n=1024):
