@@ -63,6 +63,8 @@ def step_bytes(ne, ni):
             ne * 8 + (ne + ne + ni) * 4)
 
 
+SPARSE_TRAFFIC_SUMMARY = os.path.join(
+    ROOT, "profiles", "iz_update_activate_sparse_sheet_traffic.txt")
 ENS_TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles",
                                    "ens_bf16_ensemble_traffic.txt")
 
@@ -91,6 +93,7 @@ def _workloads():
                       label="iz_update_activate (E->E)",
                       launch_bytes=fused_bytes(ne),
                       step_bytes=step_bytes(ne, ni), cpu_steps=20,
+                      traffic_summary=TRAFFIC_SUMMARY,
                       detail=" (E->E conv 15x15 s1 wrap STDP+STP, E->I conv "
                              "15x15 s2 direct, I->E div 9x9 s2 direct)",
                       data="synthetic (Philox-seeded weights U[0,0.5), "
@@ -99,6 +102,7 @@ def _workloads():
                        kernel="iz_update_activate",
                        label="iz_update_activate (E->E sparse)",
                        launch_bytes=None, step_bytes=None, cpu_steps=20,
+                       traffic_summary=SPARSE_TRAFFIC_SUMMARY,
                        detail=" (E->E SPARSE from conv 15x15 s1 wrap STDP+STP,"
                               " E->I conv 15x15 s2 direct, I->E div 9x9 s2 "
                               "direct)",
@@ -595,7 +599,8 @@ def main():
     kernels = {k[:-3]: float(v) / float(timing[k[:-3] + " launches"])
                for k, v in timing.items() if k.endswith(" ms")}
 
-    traffic = measured_traffic() if a.workload == "sheet" else None
+    tpath = spec.get("traffic_summary")
+    traffic = measured_traffic(tpath) if tpath else None
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(wl, a.cpu_steps or spec["cpu_steps"])
@@ -627,8 +632,8 @@ def main():
                                       if traffic else None),
                      "traffic_frac": (traffic / (kernel_ms * 1e-3) / 1e9 /
                                       HBM_PEAK_GBPS if traffic else None),
-                     "traffic_source": (os.path.relpath(TRAFFIC_SUMMARY, ROOT)
-                                        if a.workload == "sheet" else None),
+                     "traffic_source": (os.path.relpath(tpath, ROOT)
+                                        if traffic else None),
                      "avg_ms": kernel_ms},
         "kernel_avg_ms": kernels,
         "cpu_baseline": cpu,
