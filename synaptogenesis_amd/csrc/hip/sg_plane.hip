@@ -102,6 +102,52 @@ __global__ __launch_bounds__(64 * W) void plane_base_kernel(
     }
 }
 
+// Fully connected, at most kListMax sources: every destination visits the
+// sources in index order, so the spiking ones form one list shared by the
+// whole layer.  A block (one wavefront, one tile of 64 destinations)
+// builds it in LDS from the plane -- a popcount per plane word, a
+// wavefront prefix sum over the words, each word's set bits written at its
+// offset -- and then each lane adds its weights of the listed sources in
+// list order (the same 256-byte row of the tiled weights for all lanes per
+// entry).  Silent visits add +0 in the reference's loop, so the sum is its
+// sum bit for bit.
+constexpr int kListMax = 4096;
+
+__global__ __launch_bounds__(64) void fc_list_kernel(sg_conn c,
+                                                     const uint64_t *plane) {
+    __shared__ int list[kListMax];
+    __shared__ int total;
+    const int lane = threadIdx.x;
+    const int np = (c.from_size + 63) >> 6;          // <= 64 words
+    const uint64_t word = lane < np ? plane[lane] : 0ull;
+    const int n = __popcll(word);
+    int pre = n;                                      // inclusive scan
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(pre, d, 64);
+        if (lane >= d) pre += v;
+    }
+    int at = pre - n;
+    for (uint64_t m = word; m; m &= m - 1)
+        list[at++] = lane * 64 + (int)__builtin_ctzll(m);
+    if (lane == 63) total = pre;
+    __syncthreads();
+    const int m = total;
+    const int slot = blockIdx.x * 64 + lane;
+    if (slot >= c.num_dest) return;
+    const float *wrow = c.weights + tiled_index(slot, 0, c.k_per_dest);
+    float sum = 0.0f;
+    int j = 0;
+    for (; j + 8 <= m; j += 8) {
+        float v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) v[b] = wrow[(long)list[j + b] * 64];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) sum += v[b];
+    }
+    for (; j < m; ++j) sum += wrow[(long)list[j] * 64];
+    put_sum(c, slot_to_index(c, slot), sum);
+}
+
 template <int TYPE, int W, int VPW, int NBUF>
 static hipError_t launch_plane_w(const sg_conn &c, hipStream_t st) {
     dim3 g((c.num_dest + 63) / 64), b(64 * W);
@@ -145,6 +191,11 @@ hipError_t launch_plane_base(const sg_conn &c, hipStream_t st) {
     }
     switch (c.type) {
         case SG_FULLY_CONNECTED:
+            if (c.from_size <= kListMax && !c.convolutional) {
+                hipLaunchKernelGGL(fc_list_kernel, dim3((c.num_dest + 63) / 64),
+                                   dim3(64), 0, st, c, c.plane);
+                return hipGetLastError();
+            }
             return launch_plane_t<SG_FULLY_CONNECTED>(c, st);
         case SG_SUBSET: return launch_plane_t<SG_SUBSET>(c, st);
         case SG_CONVERGENT: return launch_plane_t<SG_CONVERGENT>(c, st);
