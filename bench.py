@@ -15,12 +15,17 @@ to get the dominant kernel's average duration for the roofline.  The CPU
 baseline times the reference's serial build (oracle/_ref, compiled from the
 reference's sources) on a bounded number of steps of the same workload.
 
---workload also measures the other single-GPU configurations: izfc (two
+--workload also measures the other single-GPU configurations: sparse (the
+sheet with a SPARSE E->E connection), izfc (two
 Izhikevich layers of 1000, fully connected) and gol1024 / gol16384 (game of
 life boards; the larger one does not fit the 256 MB Infinity Cache).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
-                    [--workload sheet|izfc|gol1024|gol16384]
+                    [--workload sheet|sparse|izfc|gol1024|gol16384|
+                                ensemble|adjoint]
+
+Default --steps: the configuration's own run length (2000 steps for the
+sheets, 1000 for the others).
 """
 
 import argparse
@@ -93,7 +98,7 @@ def _workloads():
                       label="iz_update_activate (E->E)",
                       launch_bytes=fused_bytes(ne),
                       step_bytes=step_bytes(ne, ni), cpu_steps=20,
-                      traffic_summary=TRAFFIC_SUMMARY,
+                      traffic_summary=TRAFFIC_SUMMARY, steps=2000,
                       detail=" (E->E conv 15x15 s1 wrap STDP+STP, E->I conv "
                              "15x15 s2 direct, I->E div 9x9 s2 direct)",
                       data="synthetic (Philox-seeded weights U[0,0.5), "
@@ -102,7 +107,7 @@ def _workloads():
                        kernel="iz_update_activate",
                        label="iz_update_activate (E->E sparse)",
                        launch_bytes=None, step_bytes=None, cpu_steps=20,
-                       traffic_summary=SPARSE_TRAFFIC_SUMMARY,
+                       traffic_summary=SPARSE_TRAFFIC_SUMMARY, steps=2000,
                        detail=" (E->E SPARSE from conv 15x15 s1 wrap STDP+STP,"
                               " E->I conv 15x15 s2 direct, I->E div 9x9 s2 "
                               "direct)",
@@ -575,7 +580,7 @@ def main():
         bench_adjoint(a)
         return
     if a.steps is None:
-        a.steps = 1000
+        a.steps = _workloads()[a.workload].get("steps", 1000)
 
     spec = _workloads()[a.workload]
     wl = spec["make"]()

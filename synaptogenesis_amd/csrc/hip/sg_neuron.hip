@@ -72,12 +72,6 @@ __device__ __forceinline__ float init_value(int dist, float a, float b,
     return fired ? val : 0.0f;
 }
 
-// iz_attribute_kernel (state/impl/izhikevich_attributes.cpp:102-213) for
-// neuron nid of layer l, plus the optional side outputs of sg_iz_layer:
-// the init-config value, the outgoing connections' time since spike (from
-// the words before the shift) and the plane of the new spike bits.  Every
-// lane of the wavefront calls it (the plane is a wavefront ballot);
-// nid >= l.size only takes part in the ballot.
 // One updater step of a compact record's per-source {R, w, pre, tss}:
 // tss is the time since spike the step read (the updater's next pass
 // reads it from here).
@@ -95,6 +89,12 @@ __device__ __forceinline__ void source_step(float *stp, long f, int tss,
     reinterpret_cast<f32x4 *>(stp)[f] = v;
 }
 
+// iz_attribute_kernel (state/impl/izhikevich_attributes.cpp:102-213) for
+// neuron nid of layer l, plus the optional side outputs of sg_iz_layer:
+// the init-config value, the outgoing connections' time since spike (from
+// the words before the shift) and the plane of the new spike bits.  Every
+// lane of the wavefront calls it (the plane is a wavefront ballot);
+// nid >= l.size only takes part in the ballot.
 __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
     const bool valid = nid < l.size;
     uint32_t spike = 0u;
