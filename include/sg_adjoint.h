@@ -24,12 +24,15 @@
  * chaotic).
  *
  * Inputs x[t] are 0/1 (rate-coded spikes) in BF16, [T][B][n_in].  The
- * input current W1 x of a block of `checkpoint` steps is one MFMA GEMM
- * (BF16 operands, FP32 accumulation); the recurrences run per batch row;
- * dW1 is one MFMA GEMM per block of steps over the adjoint states, which
- * enter as three BF16 terms (hi + mid + lo) so the product keeps FP32
- * precision.  The forward keeps only the state at every `checkpoint`-th
- * step; the backward recomputes each block of steps from its checkpoint.
+ * input currents W1 x of all steps are one MFMA GEMM (BF16 operands, FP32
+ * accumulation), kept in the workspace; the recurrences run per batch row;
+ * dW1 is one MFMA GEMM over every step's adjoint states, which enter as
+ * three BF16 terms (hi + mid + lo) so the product keeps FP32 precision.
+ * The forward keeps the recurrent state only at every `checkpoint`-th
+ * step; the backward recomputes each block of steps from its checkpoint
+ * (the input currents need no recomputation).  The workspace holds
+ * 2 * T * B * n_hid floats for the currents and the adjoint states besides
+ * the checkpoints; T * B is at most 4,194,240.
  *
  * All pointers are device pointers; `stream` is a hipStream_t (NULL: the
  * default stream).  Calls are asynchronous and return 0, or an error code

@@ -1,26 +1,33 @@
 // Forward pass and adjoint of the two-layer leaky network of
 // include/sg_adjoint.h.
 //
-// Layer 1's recurrence does not read layer 2, so each block of
-// `checkpoint` steps (M = checkpoint * batch rows, row = step * batch + b)
-// runs as parallel passes with only elementwise recurrences in time:
-//   forward   gemm_nt      A1[M][n_hid] = X[M][n_in] . W1^T   (MFMA BF16)
+// Layer 1's recurrence does not read layer 2, so the input currents of all
+// steps are one GEMM and each block of `checkpoint` steps (M = checkpoint *
+// batch rows, row = step * batch + b) runs as parallel passes with only
+// elementwise recurrences in time:
+//   forward   gemm_nt      A1[T*batch][n_hid] = X . W1^T for every step
+//                          (MFMA BF16), kept for the backward
+//             per block:
 //             scan1_fwd    one thread per (b, hidden unit): v1, s1 over
 //                          the block's steps
 //             out_current  A2[M][n_out] = S1 . W2^T, one wavefront per row
 //             scan2_fwd    one thread per (b, output): v2, s2, counts
-//   backward  the forward passes again from the block's checkpoint,
-//             recording v1 and v2, then
+//   backward  per block, last first: the recurrences again from the
+//             block's checkpoint (the currents are kept), recording v1
+//             and v2, then
 //             scan2_bwd    one thread per (b, output), steps in reverse:
 //                          d2 = dL/dv2
 //             scan1_bwd    one thread per (b, hidden unit), steps in
-//                          reverse: d1 = dL/dv1 from W2^T d2
-//             gemm_tn      D1^T . X (MFMA BF16, D1 as hi + mid + lo BF16
-//                          terms), split over row ranges; each range's
-//                          tile accumulates over the blocks of steps
-//             dw2_partial  D2^T . S1 over 16-row chunks, likewise
-// and at the end add_parts / dw2_reduce add the ranges and chunks in
-// order into dW1 and dW2.
+//                          reverse: d1 = dL/dv1 from W2^T d2, kept for
+//                          every step
+//             dw2_partial  D2^T . S1 over 16-row chunks, accumulated over
+//                          the blocks
+//             and after the last block
+//             gemm_tn      dW1 = D1^T . X over every step's rows (MFMA
+//                          BF16, D1 as hi + mid + lo BF16 terms), split
+//                          into row ranges
+// and add_parts / dw2_reduce add the ranges and chunks in order into dW1
+// and dW2.
 // Every sum runs in a fixed order: results are deterministic.
 
 #include <hip/hip_runtime.h>
@@ -39,7 +46,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxOut = 16;
-constexpr int kTnSplit = 4;     // row ranges of the dW1 product
+constexpr int kTnSplit = 16;    // row ranges of the dW1 product
 
 __device__ __forceinline__ uint32_t bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -513,21 +520,21 @@ struct Layout {
 Layout layout(const sg_snn_cfg &c) {
     const size_t B = c.batch, H = c.n_hid, O = c.n_out, ck = c.checkpoint;
     const size_t slots = (size_t)c.steps / ck + 1;
-    const size_t M = ck * B;
+    const size_t M = ck * B, MT = (size_t)c.steps * B;
     Layout l;
     size_t off = 0;
     auto take = [&](size_t floats) { size_t o = off; off += al(floats * 4); return o; };
     l.state = take(slots * slot_floats(c.batch, c.n_hid, c.n_out));
     l.counts = take(B * O);
     l.grad = take(B * O);
-    l.a1 = take(M * H);
+    l.a1 = take(MT * H);            // input currents of every step
     l.s1 = take(M * H);
     l.a2 = take(M * O);
     l.rv1 = take(M * H);
     l.rv2 = take(M * O);
     l.carry1 = take(B * H);
     l.carry2 = take(B * O);
-    l.lam = take(M * H);
+    l.lam = take(MT * H);           // dL/dv1 of every step
     l.g2 = take(M * O);
     l.part2 = take((M + kDw2Rows - 1) / kDw2Rows * O * H);
     l.part1 = take((size_t)kTnSplit * H * c.n_in);
@@ -546,6 +553,8 @@ int check(const sg_snn_cfg *c) {
     if (c->batch > 16384) return fail("batch above 16384");
     if (c->checkpoint < 1 || c->steps % c->checkpoint != 0)
         return fail("checkpoint must divide steps");
+    if ((long)c->steps * c->batch > 65535L * 64)
+        return fail("steps * batch above 4194240");
     if (c->mode != SG_SNN_SPIKE && c->mode != SG_SNN_RATE)
         return fail("unknown mode");
     return 0;
@@ -571,17 +580,16 @@ ScanArgs scan_args(const sg_snn_cfg &c) {
 
 unsigned blocks(long threads) { return (unsigned)((threads + 255) / 256); }
 
-// The forward passes of one block of steps from checkpoint `in`: state to
-// `out` (or null), counts per count_mode, v records when rec_v1 is set.
-void forward_block(const sg_snn_cfg &c, const ScanArgs &a, const uint16_t *xb,
-                   const uint16_t *w1, const uint16_t *w2, const Layout &l,
-                   unsigned char *ws, const float *in, float *out,
-                   int count_mode, bool record, hipStream_t s) {
+// The forward passes of block `blk` of steps from checkpoint `in`: state
+// to `out` (or null), counts per count_mode, v records when rec_v1 is set.
+// The block's input currents are already in a1.
+void forward_block(const sg_snn_cfg &c, const ScanArgs &a, int blk,
+                   const Layout &l, unsigned char *ws, const float *in,
+                   float *out, int count_mode, bool record, hipStream_t s) {
     const int M = c.checkpoint * c.batch;
-    float *a1 = (float *)(ws + l.a1), *s1 = (float *)(ws + l.s1);
+    float *a1 = (float *)(ws + l.a1) + (long)blk * M * c.n_hid;
+    float *s1 = (float *)(ws + l.s1);
     float *a2 = (float *)(ws + l.a2), *cnt = (float *)(ws + l.counts);
-    hipLaunchKernelGGL(gemm_nt, dim3((c.n_hid + 63) / 64, (M + 63) / 64),
-                       dim3(256), 0, s, xb, w1, a1, M, c.n_hid, c.n_in);
     hipLaunchKernelGGL(scan1_fwd, dim3(blocks((long)c.batch * c.n_hid)),
                        dim3(256), 0, s, a, a1, in, out, s1,
                        record ? (float *)(ws + l.rv1) : nullptr);
@@ -619,16 +627,20 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     float *state = (float *)(ws + l.state);
     float *cnt = (float *)(ws + l.counts);
     const long slot = slot_floats(c.batch, c.n_hid, c.n_out);
-    const long xblock = (long)c.checkpoint * c.batch * c.n_in;
     const ScanArgs a = scan_args(c);
     fill(state, slot, 0.0f, s);
+    // input currents of every step in one GEMM (rows t * batch + b); the
+    // backward reuses them and recomputes only the recurrences
+    const int MT = c.steps * c.batch;
+    hipLaunchKernelGGL(gemm_nt, dim3((c.n_hid + 63) / 64, (MT + 63) / 64),
+                       dim3(256), 0, s, x, w1, (float *)(ws + l.a1), MT,
+                       c.n_hid, c.n_in);
     hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)c.n_hid * kMaxOut)),
                        dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), c.n_hid,
                        c.n_out);
     for (int blk = 0; blk < c.steps / c.checkpoint; ++blk)
-        forward_block(c, a, x + blk * xblock, w1, w2, l, ws,
-                      state + blk * slot, state + (blk + 1) * slot,
-                      blk == 0 ? 1 : 2, false, s);
+        forward_block(c, a, blk, l, ws, state + blk * slot,
+                      state + (blk + 1) * slot, blk == 0 ? 1 : 2, false, s);
     hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), c.batch * 4, s, cnt,
                        labels, c.batch, c.n_out, loss, counts);
     const hipError_t e = hipGetLastError();
@@ -658,7 +670,6 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     const int H = c.n_hid, O = c.n_out, B = c.batch;
     const int M = c.checkpoint * B;
     const long slot = slot_floats(B, H, O);
-    const long xblock = (long)M * c.n_in;
     const ScanArgs a = scan_args(c);
     hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, s,
                        cnt, labels, B, O, grad_scale, g);
@@ -670,19 +681,20 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     fill(carry1, (long)B * H, 0.0f, s);
     fill(carry2, (long)B * O, 0.0f, s);
     for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
-        const uint16_t *xb = x + blk * xblock;
-        forward_block(c, a, xb, w1, w2, l, ws, state + blk * slot, nullptr, 0,
-                      true, s);
+        forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0, true,
+                      s);
         hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)), dim3(256), 0,
                            s, a, rv2, g, carry2, g2);
         hipLaunchKernelGGL(scan1_bwd, dim3(blocks((long)B * H)), dim3(256), 0,
-                           s, a, w2, rv1, g2, carry1, lam);
-        hipLaunchKernelGGL(gemm_tn, dim3((c.n_in + 63) / 64, (H + 63) / 64,
-                                         kTnSplit),
-                           dim3(256), 0, s, lam, xb, part1, M, H, c.n_in);
+                           s, a, w2, rv1, g2, carry1, lam + (long)blk * M * H);
         hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
                            dim3(256), 0, s, g2, s1, part2, M, H, O);
     }
+    // dW1 over every step's rows in one GEMM
+    const int MT = c.steps * B;
+    hipLaunchKernelGGL(gemm_tn, dim3((c.n_in + 63) / 64, (H + 63) / 64,
+                                     kTnSplit),
+                       dim3(256), 0, s, lam, x, part1, MT, H, c.n_in);
     hipLaunchKernelGGL(add_parts, dim3(1024), dim3(256), 0, s, part1, dw1,
                        (long)H * c.n_in, kTnSplit);
     hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 255) / 256), dim3(256), 0, s,
