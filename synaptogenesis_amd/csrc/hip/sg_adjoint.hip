@@ -243,6 +243,8 @@ __global__ void add_parts(const float *__restrict__ part, float *__restrict__ D,
     }
 }
 
+constexpr int kWin = 8;   // steps whose loads a scan issues together
+
 struct ScanArgs {
     int B, H, O, steps;          // batch, hidden, outputs, steps in block
     int mode;
@@ -267,15 +269,22 @@ __global__ void scan1_fwd(ScanArgs a, const float *__restrict__ a1,
     if (i >= a.B * H) return;
     const long BH = (long)a.B * H;
     float v = state_in[i], s = state_in[BH + i];
-    float next = a1[i];
-    for (int t = 0; t < a.steps; ++t) {
-        const long idx = (long)t * BH + i;
-        const float in = next;
-        if (t + 1 < a.steps) next = a1[idx + BH];
-        v = a.beta * v - a.reset_theta * s + in;
-        s = act(v - a.theta, a.mode, a.slope);
-        s1_out[idx] = s;
-        if (rec_v1) rec_v1[idx] = v;
+    // the block's currents in windows of kWin steps, each window's loads
+    // issued together
+    for (int t0 = 0; t0 < a.steps; t0 += kWin) {
+        float in[kWin];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j)
+            if (t0 + j < a.steps) in[j] = a1[(long)(t0 + j) * BH + i];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+            if (t0 + j >= a.steps) break;
+            const long idx = (long)(t0 + j) * BH + i;
+            v = a.beta * v - a.reset_theta * s + in[j];
+            s = act(v - a.theta, a.mode, a.slope);
+            s1_out[idx] = s;
+            if (rec_v1) rec_v1[idx] = v;
+        }
     }
     if (state_out) {
         state_out[i] = v;
@@ -304,16 +313,31 @@ __global__ __launch_bounds__(256) void out_current(
     if (row >= M) return;
     float p[kMaxOut];
     for (int o = 0; o < kMaxOut; ++o) p[o] = 0.0f;
-    for (int u = lane; u < H; u += 64) {
-        const float sv = s1[(long)row * H + u];
-        const u32x4 lo = *(const u32x4 *)&w2t[u * kMaxOut];
-        const u32x4 hi = *(const u32x4 *)&w2t[u * kMaxOut + 8];
+    // eight units per lane in flight: the loads of a group are issued
+    // before its products (the same u order as one at a time)
+    constexpr int G = 8;
+    for (int u0 = lane; u0 < H; u0 += 64 * G) {
+        float sv[G];
+        u32x4 lo[G], hi[G];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            p[2 * q] += bf16_value(lo[q] & 0xFFFFu) * sv;
-            p[2 * q + 1] += bf16_value(lo[q] >> 16) * sv;
-            p[8 + 2 * q] += bf16_value(hi[q] & 0xFFFFu) * sv;
-            p[9 + 2 * q] += bf16_value(hi[q] >> 16) * sv;
+        for (int j = 0; j < G; ++j) {
+            const int u = u0 + 64 * j;
+            if (u < H) {
+                sv[j] = s1[(long)row * H + u];
+                lo[j] = *(const u32x4 *)&w2t[u * kMaxOut];
+                hi[j] = *(const u32x4 *)&w2t[u * kMaxOut + 8];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if (u0 + 64 * j >= H) break;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                p[2 * q] += bf16_value(lo[j][q] & 0xFFFFu) * sv[j];
+                p[2 * q + 1] += bf16_value(lo[j][q] >> 16) * sv[j];
+                p[8 + 2 * q] += bf16_value(hi[j][q] & 0xFFFFu) * sv[j];
+                p[9 + 2 * q] += bf16_value(hi[j][q] >> 16) * sv[j];
+            }
         }
     }
 #pragma unroll
@@ -337,12 +361,19 @@ __global__ void scan2_fwd(ScanArgs a, const float *__restrict__ a2,
     const long BH = (long)a.B * a.H, BO = (long)a.B * O;
     float v = state_in[2 * BH + i], s = state_in[2 * BH + BO + i];
     float cnt = count_mode == 2 ? counts[i] : 0.0f;
-    for (int t = 0; t < a.steps; ++t) {
-        const long idx = (long)t * BO + i;
-        v = a.beta * v - a.reset_theta * s + a2[idx];
-        s = act(v - a.theta, a.mode, a.slope);
-        cnt += s;
-        if (rec_v2) rec_v2[idx] = v;
+    for (int t0 = 0; t0 < a.steps; t0 += kWin) {
+        float in[kWin];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j)
+            if (t0 + j < a.steps) in[j] = a2[(long)(t0 + j) * BO + i];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+            if (t0 + j >= a.steps) break;
+            v = a.beta * v - a.reset_theta * s + in[j];
+            s = act(v - a.theta, a.mode, a.slope);
+            cnt += s;
+            if (rec_v2) rec_v2[(long)(t0 + j) * BO + i] = v;
+        }
     }
     if (state_out) {
         state_out[2 * BH + i] = v;
@@ -363,13 +394,21 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
     const long BO = (long)a.B * O;
     const float gb = g[i];
     float dn = carry2[i];
-    for (int t = a.steps - 1; t >= 0; --t) {
-        const long idx = (long)t * BO + i;
-        const float d = (gb - a.reset_theta * dn) *
-                        act_grad(rec_v2[idx] - a.theta, a.mode, a.slope) +
-                        a.beta * dn;
-        g2[idx] = d;
-        dn = d;
+    for (int t1 = a.steps - 1; t1 >= 0; t1 -= kWin) {
+        float vv[kWin];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j)
+            if (t1 - j >= 0) vv[j] = rec_v2[(long)(t1 - j) * BO + i];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+            const int t = t1 - j;
+            if (t < 0) break;
+            const float d = (gb - a.reset_theta * dn) *
+                            act_grad(vv[j] - a.theta, a.mode, a.slope) +
+                            a.beta * dn;
+            g2[(long)t * BO + i] = d;
+            dn = d;
+        }
     }
     carry2[i] = dn;
 }
@@ -392,20 +431,32 @@ __global__ void scan1_bwd(ScanArgs a, const uint16_t *__restrict__ w2,
     for (int o = 0; o < kMaxOut; ++o)
         wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
     float dn = carry1[i];
-    float vnext = rec_v1[(long)(a.steps - 1) * BH + i];
-    for (int t = a.steps - 1; t >= 0; --t) {
-        const long idx = (long)t * BH + i;
-        const float v = vnext;
-        if (t > 0) vnext = rec_v1[idx - BH];
-        const float *d2 = g2 + (long)t * BO + (long)b * O;
-        float gs = 0.0f;
+    // windows of kWin steps from the last: each window's loads (v1 and the
+    // output adjoints) issued together, then the recurrence
+    for (int t1 = a.steps - 1; t1 >= 0; t1 -= kWin) {
+        float vv[kWin], gsum[kWin];
 #pragma unroll
-        for (int o = 0; o < kMaxOut; ++o)
-            if (o < O) gs += wc[o] * d2[o];
-        const float d = (gs - a.reset_theta * dn) *
-                        act_grad(v - a.theta, a.mode, a.slope) + a.beta * dn;
-        lam[idx] = d;
-        dn = d;
+        for (int j = 0; j < kWin; ++j) {
+            const int t = t1 - j;
+            if (t < 0) break;
+            vv[j] = rec_v1[(long)t * BH + i];
+            const float *d2 = g2 + (long)t * BO + (long)b * O;
+            float gs = 0.0f;
+#pragma unroll
+            for (int o = 0; o < kMaxOut; ++o)
+                if (o < O) gs += wc[o] * d2[o];
+            gsum[j] = gs;
+        }
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+            const int t = t1 - j;
+            if (t < 0) break;
+            const float d = (gsum[j] - a.reset_theta * dn) *
+                            act_grad(vv[j] - a.theta, a.mode, a.slope) +
+                            a.beta * dn;
+            lam[(long)t * BH + i] = d;
+            dn = d;
+        }
     }
     carry1[i] = dn;
 }
