@@ -5,9 +5,10 @@
 // (game_of_life_attributes.cpp:14-33, attributes.h:321-333) in one pass.
 //
 // Words are double-buffered: the step reads words_in (state after t-1) and
-// writes words_out.  Each block stages a 64 x 16 tile of words plus a
-// one-cell halo in LDS (1.16 words read per cell), so each word is fetched
-// from HBM/L2 about once per step instead of nine times; the neighbour sum runs in the kernel's row-major k order with the
+// writes words_out.  Each block stages a 64 x 32 (small boards: 64 x 16)
+// tile of words plus a one-cell halo in LDS (1.09 / 1.16 words read per
+// cell), so each word is fetched from HBM/L2 about once per step instead of
+// nine times; the neighbour sum runs in the kernel's row-major k order with the
 // weights given, so any kernel values (not just 0/1) match the reference.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,7 +20,6 @@
 namespace sg {
 
 constexpr int kGolTx = 64;   // tile columns (one wavefront per tile row)
-constexpr int kGolTy = 16;   // tile rows
 constexpr int kGolWy = 4;    // wavefronts per block (tile rows per pass)
 
 // Row or column index x >= -1 of a wrapping board (a tile and its halo
@@ -38,7 +38,7 @@ __device__ __forceinline__ int gol_wrap(int x, int n) {
 // row-major order, skipping out-of-range neighbours as the reference does.
 // Each thread owns kGolRun consecutive rows of one column and reads the
 // newest bits of its (kGolRun + 2) x 3 neighbourhood once.
-template <bool INTW>
+template <bool INTW, int kGolTy>
 __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
         const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
         float *__restrict__ input, int rows, int cols, int wrap, float k0,
@@ -126,12 +126,19 @@ extern "C" int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
                                int bmax, void *stream) {
     using namespace sg;
     if (rows <= 0 || cols <= 0) return 0;
-    dim3 g((cols + kGolTx - 1) / kGolTx, (rows + kGolTy - 1) / kGolTy);
+    // 32-row tiles where the board gives at least 2048 of them (less halo
+    // per cell), else 16-row tiles (more blocks for small boards)
+    const long tiles32 = (long)((cols + kGolTx - 1) / kGolTx) *
+                         ((rows + 31) / 32);
+    const int ty = tiles32 >= 2048 ? 32 : 16;
+    dim3 g((cols + kGolTx - 1) / kGolTx, (rows + ty - 1) / ty);
     bool intw = true;
     for (int q = 0; q < 9; ++q)
         intw = intw && std::fabs(k[q]) <= 1048576.0f &&
                k[q] == (float)(int)k[q];
-    auto *kern = intw ? gol_step_kernel<true> : gol_step_kernel<false>;
+    auto *kern = ty == 32
+        ? (intw ? gol_step_kernel<true, 32> : gol_step_kernel<false, 32>)
+        : (intw ? gol_step_kernel<true, 16> : gol_step_kernel<false, 16>);
     hipLaunchKernelGGL(kern, g, dim3(kGolTx, kGolWy), 0, (hipStream_t)stream,
                        words_in, words_out, input, rows, cols, wrap, k[0],
                        k[1], k[2], k[3], k[4], k[5], k[6], k[7], k[8], smin,

@@ -80,6 +80,21 @@ def test_full_size_game_of_life_fused(gpu_lib):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("rows,cols,wrap", [(2050, 2100, True),
+                                            (2048, 2112, False)])
+def test_large_game_of_life_fused(gpu_lib, rows, cols, wrap):
+    """Boards large enough for the fused step's 32-row tiles (ragged edge
+    tiles on the first), 3 steps, wrapping and not: final history words bit
+    for bit."""
+    print("game of life %dx%d wrap=%s" % (rows, cols, wrap), flush=True)
+    case = netdefs.gol("gol_%dx%d" % (rows, cols), rows, cols, 3, 304,
+                       wrap=wrap)
+    case.capture = []
+    case.final_neuron = [("gol", "board", "output")]
+    _check(case, gpu_lib)
+
+
+@pytest.mark.gpu
 def test_full_size_game_of_life_captured(gpu_lib):
     """1024x1024 board, 6 steps with per-step capture (activator + neuron
     kernels): every step's words bit for bit."""
