@@ -70,11 +70,10 @@ __device__ __forceinline__ void put_sum(const sg_conn &c, int to, float sum) {
 // visit in serial order): with m = min(32, visits) and rev[i] the delay of
 // the i-th visit counted back from the last, the result is the first i
 // whose delayed bit of `word` is set, else min(32, cur + m).
-__device__ __forceinline__ int tss_next(int cur, int m, const uint8_t *rev,
+__device__ __forceinline__ int tss_next(int cur, int m, const uint32_t *rd,
                                         uint32_t word) {
     int t = min(32, cur + m);
     if (word != 0u) {
-        const uint32_t *rd = (const uint32_t *)rev;
         for (int q = 0; q < 8 && q * 4 < m; ++q) {
             const uint32_t four = rd[q];
             int hit = -1;

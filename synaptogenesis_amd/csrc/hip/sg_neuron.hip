@@ -118,6 +118,30 @@ __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
         }
         const float a = l.a[nid];
         const float b = l.b[nid];
+        // every load of the rest of the step, issued before the loop: the
+        // stores in between could alias them, so the compiler would not
+        // move them up by itself
+        const float cc = l.c[nid], dd = l.d[nid];
+        const float dopamine = l.dopamine[nid];
+        const float post_trace = l.post_trace[nid];
+        const int tss_self = l.time_since_spike[nid];
+        int t_cur[SG_MAX_TSS_OUT], t_vis[SG_MAX_TSS_OUT];
+        uint32_t t_word[SG_MAX_TSS_OUT], t_rev[SG_MAX_TSS_OUT][8];
+#pragma unroll
+        for (int t = 0; t < SG_MAX_TSS_OUT; ++t) {
+            if (t >= l.ntss) break;
+            const sg_tss_out &o = l.tss[t];
+            t_cur[t] = o.cur[nid];
+            t_vis[t] = o.visits[nid];
+            t_word[t] = l.output[(long)o.word * l.size + nid];
+            const uint4 *rv =
+                reinterpret_cast<const uint4 *>(o.rev_delays + (long)nid * 32);
+            const uint4 r0 = rv[0], r1 = rv[1];
+            t_rev[t][0] = r0.x; t_rev[t][1] = r0.y;
+            t_rev[t][2] = r0.z; t_rev[t][3] = r0.w;
+            t_rev[t][4] = r1.x; t_rev[t][5] = r1.y;
+            t_rev[t][6] = r1.z; t_rev[t][7] = r1.w;
+        }
 
         for (int i = 0; i < 10 && (double)voltage < 30.0; ++i) {
             float current = -ampa * voltage;
@@ -153,13 +177,13 @@ __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
         l.gabaa[nid] = 0.0f;
         l.gabab[nid] = 0.0f;
         l.mult[nid] = 0.0f;
-        l.dopamine[nid] = (float)((double)l.dopamine[nid] * 0.95);
+        l.dopamine[nid] = (float)((double)dopamine * 0.95);
 
-        for (int t = 0; t < l.ntss; ++t) {
+#pragma unroll
+        for (int t = 0; t < SG_MAX_TSS_OUT; ++t) {
+            if (t >= l.ntss) break;
             const sg_tss_out &o = l.tss[t];
-            const int tn = tss_next(o.cur[nid], o.visits[nid],
-                                    o.rev_delays + (long)nid * 32,
-                                    l.output[(long)o.word * l.size + nid]);
+            const int tn = tss_next(t_cur[t], t_vis[t], t_rev[t], t_word[t]);
             o.next[nid] = tn;
             if (o.stp) source_step(o.stp, nid, tn, o.stp_sub, o.stp_flag);
         }
@@ -170,11 +194,10 @@ __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
 
         l.post_trace[nid] = prev_bit
             ? (float)0.004
-            : (float)((double)l.post_trace[nid] * 0.95);
-        l.time_since_spike[nid] = prev_bit
-            ? 0 : min(32, l.time_since_spike[nid] + 1);
-        l.voltage[nid] = spike ? l.c[nid] : voltage;
-        l.recovery[nid] = recovery + ((float)spike * l.d[nid]);
+            : (float)((double)post_trace * 0.95);
+        l.time_since_spike[nid] = prev_bit ? 0 : min(32, tss_self + 1);
+        l.voltage[nid] = spike ? cc : voltage;
+        l.recovery[nid] = recovery + ((float)spike * dd);
     }
     if (l.plane) {
         const uint64_t m = __ballot(spike != 0u);

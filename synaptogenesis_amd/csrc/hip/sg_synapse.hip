@@ -108,7 +108,9 @@ __global__ __launch_bounds__(kBlock) void tss_step_kernel(
         const uint8_t *rev_delays, const uint32_t *words, int n) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= n) return;
-    next[f] = tss_next(cur[f], visits[f], rev_delays + (long)f * 32, words[f]);
+    next[f] = tss_next(cur[f], visits[f],
+                       (const uint32_t *)(rev_delays + (long)f * 32),
+                       words[f]);
 }
 
 // ---------------------------------------------------------------------------
