@@ -161,16 +161,62 @@ static hipError_t launch_plane_w(const sg_conn &c, hipStream_t st) {
     return hipGetLastError();
 }
 
-// A destination with at most 16 visits: one wavefront per tile, 16 visits
-// each.  Otherwise sixteen wavefronts of 8 visits share a tile (windows of
+// At most 16 visits per destination: one wavefront per tile, each lane
+// walks its destination's visits in order, reading the plane words through
+// the caches (no LDS staging, no barrier: a block is one short wavefront
+// that fits the single wave slot a CU has left beside a long kernel).
+// Every visit's plane word is loaded first, then the weights of the
+// spiking visits, then the sum in visit order (silent visits add +0).
+template <int TYPE>
+__global__ __launch_bounds__(64) void plane_small_kernel(sg_conn c,
+                                                         const uint64_t *plane) {
+    constexpr int V = 16;
+    const int slot = blockIdx.x * 64 + threadIdx.x;
+    if (slot >= c.num_dest) return;
+    const int count = plane_visits<TYPE>(c);
+    const float *wbase = c.convolutional
+        ? c.weights : c.weights + tiled_index(slot, 0, c.k_per_dest);
+    const long wstep = c.convolutional ? 1 : 64;
+    SynapseIter<TYPE> it;
+    it.seek(c, slot, 0);
+    int from[V];
+    uint64_t word[V];
+#pragma unroll
+    for (int b = 0; b < V; ++b) {
+        from[b] = -1;
+        if (b < count) {
+            from[b] = it.source();
+            it.advance();
+        }
+        word[b] = from[b] >= 0 ? plane[from[b] >> 6] : 0ull;
+    }
+    float v[V];
+#pragma unroll
+    for (int b = 0; b < V; ++b) {
+        v[b] = 0.0f;
+        if (from[b] >= 0 && ((word[b] >> (from[b] & 63)) & 1u))
+            v[b] = wbase[b * wstep];
+    }
+    float sum = 0.0f;
+#pragma unroll
+    for (int b = 0; b < V; ++b) sum += v[b];
+    put_sum(c, slot_to_index(c, slot), sum);
+}
+
+// A destination with at most 16 visits: plane_small_kernel.  Otherwise
+// sixteen wavefronts of 8 visits share a tile (windows of
 // 128 visits, one LDS buffer): the most wavefronts in flight per tile,
 // which is what hides latency for the few-tile layers this path serves
 // (measured on the E/I sheet's E->I connection: 12.9 us against 19.9 us
 // with four wavefronts of 16 visits).
 template <int TYPE>
 static hipError_t launch_plane_t(const sg_conn &c, hipStream_t st) {
-    if (cursor<TYPE>(c, 0).count <= 16)
-        return launch_plane_w<TYPE, 1, 16, 2>(c, st);
+    if (cursor<TYPE>(c, 0).count <= 16) {
+        hipLaunchKernelGGL(plane_small_kernel<TYPE>,
+                           dim3((c.num_dest + 63) / 64), dim3(64), 0, st, c,
+                           c.plane);
+        return hipGetLastError();
+    }
     return launch_plane_w<TYPE, 16, 8, 1>(c, st);
 }
 
