@@ -637,6 +637,11 @@ def main():
                                       if traffic else None),
                      "traffic_frac": (traffic / (kernel_ms * 1e-3) / 1e9 /
                                       HBM_PEAK_GBPS if traffic else None),
+                     "note": ("achieved counts the reference's element "
+                              "types (FP32 arrays, int32 delays); the "
+                              "kernel's compact record moves `traffic` "
+                              "bytes, at traffic_GBps"
+                              if a.workload in ("sheet", "sparse") else None),
                      "traffic_source": (os.path.relpath(tpath, ROOT)
                                         if traffic else None),
                      "avg_ms": kernel_ms},
