@@ -52,9 +52,8 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
     // Stage (kGolTy + 2) x (kGolTx + 2) words; out-of-range cells of a
     // non-wrapping board hold 0.
     const int tid = ty * kGolTx + tx;
-    for (int i = tid; i < (kGolTy + 2) * (kGolTx + 2);
-         i += kGolTx * kGolWy) {
-        const int lr = i / (kGolTx + 2), lc = i % (kGolTx + 2);
+    const bool vec = (cols & 3) == 0 && col0 + kGolTx <= cols;
+    auto stage = [&](int lr, int lc) {
         int r = row0 + lr - 1, c = col0 + lc - 1;
         uint32_t v = 0;
         if (wrap) {
@@ -65,6 +64,34 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
             v = in[(long)r * cols + c];
         }
         tile[lr][lc] = v;
+    };
+    if (vec) {
+        // interior columns as 16-byte loads (the tile's 64 columns are in
+        // range and 16-byte aligned), the two halo columns one word each
+        constexpr int Q = kGolTx / 4;
+        for (int i = tid; i < (kGolTy + 2) * Q; i += kGolTx * kGolWy) {
+            const int lr = i / Q, q = i % Q;
+            int r = row0 + lr - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (wrap) {
+                r = gol_wrap(r, rows);
+                v = *reinterpret_cast<const uint4 *>(in + (long)r * cols + col0 +
+                                                     4 * q);
+            } else if (r >= 0 && r < rows) {
+                v = *reinterpret_cast<const uint4 *>(in + (long)r * cols + col0 +
+                                                     4 * q);
+            }
+            tile[lr][1 + 4 * q] = v.x;
+            tile[lr][2 + 4 * q] = v.y;
+            tile[lr][3 + 4 * q] = v.z;
+            tile[lr][4 + 4 * q] = v.w;
+        }
+        for (int i = tid; i < 2 * (kGolTy + 2); i += kGolTx * kGolWy)
+            stage(i >> 1, (i & 1) ? kGolTx + 1 : 0);
+    } else {
+        for (int i = tid; i < (kGolTy + 2) * (kGolTx + 2);
+             i += kGolTx * kGolWy)
+            stage(i / (kGolTx + 2), i % (kGolTx + 2));
     }
     __syncthreads();
 
@@ -113,7 +140,9 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
         const bool live = prev_live ? (n >= smin && n <= smax)
                                     : (n >= bmin && n <= bmax);
         const long idx = (long)r * cols + c;
-        out[idx] = (self >> 1) | ((live ? 1u : 0u) << 31);
+        // streamed: the next step reads it from HBM anyway
+        __builtin_nontemporal_store((self >> 1) | ((live ? 1u : 0u) << 31),
+                                    &out[idx]);
         if (input) input[idx] = sum;
     }
 }
