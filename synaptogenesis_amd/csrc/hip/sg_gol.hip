@@ -52,7 +52,8 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
     // Stage (kGolTy + 2) x (kGolTx + 2) words; out-of-range cells of a
     // non-wrapping board hold 0.
     const int tid = ty * kGolTx + tx;
-    const bool vec = (cols & 3) == 0 && col0 + kGolTx <= cols;
+    const bool vec = (cols & 3) == 0 && col0 + kGolTx <= cols &&
+                     ((uintptr_t)in & 15) == 0;
     auto stage = [&](int lr, int lc) {
         int r = row0 + lr - 1, c = col0 + lc - 1;
         uint32_t v = 0;
