@@ -305,13 +305,11 @@ __global__ void transpose_w2(const uint16_t *__restrict__ w2,
 // A2[row][o] = sum over u of W2[o][u] s1[row][u], one wavefront per row:
 // lane partial sums over u = lane + 64 j (j ascending), then a fixed
 // butterfly across the wavefront.
-__global__ __launch_bounds__(256) void out_current(
-    const float *__restrict__ s1, const uint16_t *__restrict__ w2t,
-    float *__restrict__ a2, int M, int H, int O) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= M) return;
-    float p[kMaxOut];
+// p[o] (lane-sum across the wavefront still to do) for one row.
+__device__ __forceinline__ void row_partials(const float *__restrict__ s1,
+                                             const uint16_t *__restrict__ w2t,
+                                             long row, int H, int lane,
+                                             float (&p)[kMaxOut]) {
     for (int o = 0; o < kMaxOut; ++o) p[o] = 0.0f;
     // eight units per lane in flight: the loads of a group are issued
     // before its products (the same u order as one at a time)
@@ -323,7 +321,7 @@ __global__ __launch_bounds__(256) void out_current(
         for (int j = 0; j < G; ++j) {
             const int u = u0 + 64 * j;
             if (u < H) {
-                sv[j] = s1[(long)row * H + u];
+                sv[j] = s1[row * H + u];
                 lo[j] = *(const u32x4 *)&w2t[u * kMaxOut];
                 hi[j] = *(const u32x4 *)&w2t[u * kMaxOut + 8];
             }
@@ -340,6 +338,16 @@ __global__ __launch_bounds__(256) void out_current(
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void out_current(
+    const float *__restrict__ s1, const uint16_t *__restrict__ w2t,
+    float *__restrict__ a2, int M, int H, int O) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= M) return;
+    float p[kMaxOut];
+    row_partials(s1, w2t, row, H, lane, p);
 #pragma unroll
     for (int o = 0; o < kMaxOut; ++o)
         if (o < O) {
@@ -374,6 +382,48 @@ __global__ void scan2_fwd(ScanArgs a, const float *__restrict__ a2,
             cnt += s;
             if (rec_v2) rec_v2[(long)(t0 + j) * BO + i] = v;
         }
+    }
+    if (state_out) {
+        state_out[2 * BH + i] = v;
+        state_out[2 * BH + BO + i] = s;
+    }
+    if (count_mode != 0) counts[i] = cnt;
+}
+
+// out_current and scan2_fwd in one launch for checkpoints of at most 16
+// steps: block = batch row b, wavefront t = step t of the block (row
+// t * B + b) reduces W2 . s1 exactly as out_current into LDS, then threads
+// o < O run layer 2's recurrence as scan2_fwd does.  Same values, one
+// launch and no A2 round trip through memory.
+__global__ __launch_bounds__(1024) void out_scan2_fwd(
+        ScanArgs a, const float *__restrict__ s1,
+        const uint16_t *__restrict__ w2t, const float *__restrict__ state_in,
+        float *__restrict__ state_out, float *__restrict__ counts,
+        int count_mode, float *__restrict__ rec_v2) {
+    __shared__ float a2s[16][kMaxOut];
+    const int b = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int O = a.O;
+    {
+        float p[kMaxOut];
+        row_partials(s1, w2t, (long)t * a.B + b, a.H, lane, p);
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o)
+            if (o < O) {
+                const float v = wave_sum(p[o]);
+                if (lane == 0) a2s[t][o] = v;
+            }
+    }
+    __syncthreads();
+    if (threadIdx.x >= O) return;
+    const int i = b * O + threadIdx.x;
+    const long BH = (long)a.B * a.H, BO = (long)a.B * O;
+    float v = state_in[2 * BH + i], s = state_in[2 * BH + BO + i];
+    float cnt = count_mode == 2 ? counts[i] : 0.0f;
+    for (int tt = 0; tt < a.steps; ++tt) {
+        v = a.beta * v - a.reset_theta * s + a2s[tt][threadIdx.x];
+        s = act(v - a.theta, a.mode, a.slope);
+        cnt += s;
+        if (rec_v2) rec_v2[(long)tt * BO + i] = v;
     }
     if (state_out) {
         state_out[2 * BH + i] = v;
@@ -644,6 +694,14 @@ void forward_block(const sg_snn_cfg &c, const ScanArgs &a, int blk,
     hipLaunchKernelGGL(scan1_fwd, dim3(blocks((long)c.batch * c.n_hid)),
                        dim3(256), 0, s, a, a1, in, out, s1,
                        record ? (float *)(ws + l.rv1) : nullptr);
+    if (c.checkpoint <= 16) {
+        hipLaunchKernelGGL(out_scan2_fwd, dim3(c.batch),
+                           dim3(64 * c.checkpoint), 0, s, a, s1,
+                           (const uint16_t *)(ws + l.w2t), in, out, cnt,
+                           count_mode,
+                           record ? (float *)(ws + l.rv2) : nullptr);
+        return;
+    }
     hipLaunchKernelGGL(out_current, dim3((M + 3) / 4), dim3(256), 0, s, s1,
                        (const uint16_t *)(ws + l.w2t), a2, M, c.n_hid,
                        c.n_out);
