@@ -12,6 +12,8 @@
 //                          the block's steps
 //             out_current  A2[M][n_out] = S1 . W2^T, one wavefront per row
 //             scan2_fwd    one thread per (b, output): v2, s2, counts
+//                          (checkpoints of at most 16 steps: both in one
+//                          launch, out_scan2_fwd, a block per batch row)
 //   backward  per block, last first: the recurrences again from the
 //             block's checkpoint (the currents are kept), recording v1
 //             and v2, then
