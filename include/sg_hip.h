@@ -91,6 +91,11 @@ typedef struct sg_conn {
      * nonzero visits of its slots ({lo, hi} per tile); a tile's walk ends
      * at its hi, and visits below its lo need no check. */
     const int32_t *tile_counts;
+    /* The launch runs beside a long kernel on another stream: a base
+     * activator on the plane path then uses blocks of four wavefronts
+     * (they start as soon as a CU has four free wave slots) instead of
+     * sixteen. */
+    int beside;
 } sg_conn;
 
 /* Fields of the interleaved per-synapse record of a tiled Izhikevich

@@ -203,8 +203,10 @@ __global__ __launch_bounds__(64) void plane_small_kernel(sg_conn c,
     put_sum(c, slot_to_index(c, slot), sum);
 }
 
-// A destination with at most 16 visits: plane_small_kernel.  Otherwise
-// sixteen wavefronts of 8 visits share a tile (windows of
+// A destination with at most 16 visits: plane_small_kernel.  Beside a
+// long kernel (sg_conn.beside): four wavefronts of 8 visits per tile, two
+// LDS buffers.  Otherwise sixteen wavefronts of 8 visits share a tile
+// (windows of
 // 128 visits, one LDS buffer): the most wavefronts in flight per tile,
 // which is what hides latency for the few-tile layers this path serves
 // (measured on the E/I sheet's E->I connection: 12.9 us against 19.9 us
@@ -217,6 +219,7 @@ static hipError_t launch_plane_t(const sg_conn &c, hipStream_t st) {
                            c.plane);
         return hipGetLastError();
     }
+    if (c.beside) return launch_plane_w<TYPE, 4, 8, 2>(c, st);
     return launch_plane_w<TYPE, 16, 8, 1>(c, st);
 }
 

@@ -484,6 +484,7 @@ Props *Engine::run(const Props &args) {
     auto activate = [&](ConnState &cs, hipStream_t st) {
         if (cs.fused_gol) return;
         sg_conn d = conn_desc(cs);
+        d.beside = st == side;
         if (!cs.iz || cs.direct) {
             kt.time("activate_base", [&] {
                 check_hip(sg_hip_activate_base(&d, st), "activate");
