@@ -162,13 +162,12 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
     const int w = t >> 6, lane = t & 63;
     const int wn = (w >> 1) * 32, wk = (w & 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
-    // staging: thread (row pair rp, 4 columns nc) loads rows rp, rp + 1 of
-    // its 4 columns of L and X (16 B and 8 B vector loads, coalesced across
-    // the 16 threads of a row) and stores each column's two rows as one
-    // 32-bit word of the transposed tiles
-    const int rp = (t >> 4) * 2, nc = (t & 15) * 4;
-    const bool lvec = (N & 3) == 0 && ((uintptr_t)L & 15) == 0;
-    const bool xvec = (K & 3) == 0 && ((uintptr_t)X & 7) == 0;
+    // staging: thread (row pair rp, column c) loads rows rp, rp + 1 of
+    // columns c, c + 16, c + 32, c + 48 of L and X (coalesced across the 16
+    // threads of a row) and stores each column's two rows as one 32-bit
+    // word of the transposed tiles; neighbouring threads take neighbouring
+    // tile rows, so the stores spread over the LDS banks
+    const int rp = (t >> 4) * 2, nc = t & 15;
     f32x4 acc[2][2] = {};
     for (int r0 = rb; r0 < re; r0 += 32) {
         float lv[2][4];
@@ -177,21 +176,12 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
             const int r = r0 + rp + h;
             const bool in = r < re;
             const float *lrow = L + (long)r * N + n0 + nc;
-            if (in && lvec && n0 + nc + 4 <= N) {
-                const f32x4 v = *(const f32x4 *)lrow;
-                for (int e = 0; e < 4; ++e) lv[h][e] = v[e];
-            } else {
-                for (int e = 0; e < 4; ++e)
-                    lv[h][e] = (in && n0 + nc + e < N) ? lrow[e] : 0.0f;
-            }
             const uint16_t *xrow = X + (long)r * K + k0 + nc;
-            if (in && xvec && k0 + nc + 4 <= K) {
-                uint2 v = *(const uint2 *)xrow;
-                xv[h][0] = (uint16_t)v.x; xv[h][1] = (uint16_t)(v.x >> 16);
-                xv[h][2] = (uint16_t)v.y; xv[h][3] = (uint16_t)(v.y >> 16);
-            } else {
-                for (int e = 0; e < 4; ++e)
-                    xv[h][e] = (in && k0 + nc + e < K) ? xrow[e] : (uint16_t)0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                lv[h][e] = (in && n0 + nc + 16 * e < N) ? lrow[16 * e] : 0.0f;
+                xv[h][e] = (in && k0 + nc + 16 * e < K) ? xrow[16 * e]
+                                                        : (uint16_t)0;
             }
         }
         for (int e = 0; e < 4; ++e) {
@@ -206,8 +196,9 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
                 term[h][2] = bf16_rne(rem - bf16_value(mid));
             }
             for (int q = 0; q < 3; ++q)
-                *(uint32_t *)&Ah[q][nc + e][rp] = term[0][q] | (term[1][q] << 16);
-            *(uint32_t *)&Bs[nc + e][rp] =
+                *(uint32_t *)&Ah[q][nc + 16 * e][rp] =
+                    term[0][q] | (term[1][q] << 16);
+            *(uint32_t *)&Bs[nc + 16 * e][rp] =
                 (uint32_t)xv[0][e] | ((uint32_t)xv[1][e] << 16);
         }
         __syncthreads();
