@@ -517,12 +517,18 @@ __global__ void dw2_partial(const float *__restrict__ g2,
     if (u >= H) return;
     float acc[kMaxOut];
     for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
-    const int r1 = min(M, (c + 1) * kDw2Rows);
-    for (int r = c * kDw2Rows; r < r1; ++r) {
-        const float sv = s1[(long)r * H + u];
+    const int r0 = c * kDw2Rows, r1 = min(M, r0 + kDw2Rows);
+    // the chunk's s1 values first, then the products in row order
+    float sv[kDw2Rows];
+#pragma unroll
+    for (int j = 0; j < kDw2Rows; ++j)
+        sv[j] = r0 + j < r1 ? s1[(long)(r0 + j) * H + u] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < kDw2Rows; ++j) {
+        if (r0 + j >= r1) break;
 #pragma unroll
         for (int o = 0; o < kMaxOut; ++o)
-            if (o < O) acc[o] += g2[(long)r * O + o] * sv;
+            if (o < O) acc[o] += g2[(long)(r0 + j) * O + o] * sv[j];
     }
 #pragma unroll
     for (int o = 0; o < kMaxOut; ++o)
