@@ -18,7 +18,9 @@
 //             block's checkpoint (the currents are kept), recording v1
 //             and v2, then
 //             scan2_bwd    one thread per (b, output), steps in reverse:
-//                          d2 = dL/dv2
+//                          d2 = dL/dv2 (n_hid a multiple of 256 and
+//                          checkpoints of at most 16 steps: run by the
+//                          first threads of each scan1_bwd block instead)
 //             scan1_bwd    one thread per (b, hidden unit), steps in
 //                          reverse: d1 = dL/dv1 from W2^T d2, kept for
 //                          every step
@@ -37,6 +39,7 @@
 
 #include <cstring>
 #include <string>
+#include <utility>
 
 #include "sg_adjoint.h"
 
@@ -459,16 +462,49 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
 // Adjoint of layer 1 over one block of steps, in reverse; thread = (b, u):
 // d1 = (sum over o of W2[o][u] d2[o] - r theta d1') S'(v1 - theta) +
 // beta d1'.
-__global__ void scan1_bwd(ScanArgs a, const uint16_t *__restrict__ w2,
-                          const float *__restrict__ rec_v1,
-                          const float *__restrict__ g2,
-                          float *__restrict__ carry1,
-                          float *__restrict__ lam) {
+// FUSE2 (n_hid a multiple of 256, checkpoints of at most 16 steps): a
+// block lies in one batch row b, and its first O threads first run layer
+// 2's adjoint for that row exactly as scan2_bwd does -- into LDS, and into
+// g2 and the carry (carry2_out) from the row's first block -- so that
+// scan2_bwd needs no launch of its own.
+template <bool FUSE2>
+__global__ __launch_bounds__(256) void scan1_bwd(
+        ScanArgs a, const uint16_t *__restrict__ w2,
+        const float *__restrict__ rec_v1, float *__restrict__ g2,
+        float *__restrict__ carry1, float *__restrict__ lam,
+        const float *__restrict__ rec_v2, const float *__restrict__ g,
+        const float *__restrict__ carry2_in, float *__restrict__ carry2_out) {
+    __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = a.H, O = a.O;
+    const long BH = (long)a.B * H, BO = (long)a.B * O;
+    if (FUSE2) {
+        const int b0 = (int)(((long)blockIdx.x * blockDim.x) / H);
+        const bool first = ((long)blockIdx.x * blockDim.x) % H == 0;
+        if (threadIdx.x < O && b0 < a.B) {
+            const int i2 = b0 * O + threadIdx.x;
+            const float gb = g[i2];
+            float dn2 = carry2_in[i2];
+            float v2[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                if (t < a.steps) v2[t] = rec_v2[(long)t * BO + i2];
+#pragma unroll
+            for (int t = 15; t >= 0; --t) {
+                if (t >= a.steps) continue;
+                const float d = (gb - a.reset_theta * dn2) *
+                                act_grad(v2[t] - a.theta, a.mode, a.slope) +
+                                a.beta * dn2;
+                d2s[t][threadIdx.x] = d;
+                if (first) g2[(long)t * BO + i2] = d;
+                dn2 = d;
+            }
+            if (first) carry2_out[i2] = dn2;
+        }
+        __syncthreads();
+    }
     if (i >= a.B * H) return;
     const int b = i / H, u = i % H;
-    const long BH = (long)a.B * H, BO = (long)a.B * O;
     float wc[kMaxOut];
 #pragma unroll
     for (int o = 0; o < kMaxOut; ++o)
@@ -483,7 +519,7 @@ __global__ void scan1_bwd(ScanArgs a, const uint16_t *__restrict__ w2,
             const int t = t1 - j;
             if (t < 0) break;
             vv[j] = rec_v1[(long)t * BH + i];
-            const float *d2 = g2 + (long)t * BO + (long)b * O;
+            const float *d2 = FUSE2 ? d2s[t] : g2 + (long)t * BO + (long)b * O;
             float gs = 0.0f;
 #pragma unroll
             for (int o = 0; o < kMaxOut; ++o)
@@ -613,7 +649,8 @@ size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace carve-up, in bytes from the base.
 struct Layout {
-    size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, lam,
+    size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, carry2b,
+        lam,
         g2, part2, part1, w2t, total;
 };
 
@@ -634,6 +671,7 @@ Layout layout(const sg_snn_cfg &c) {
     l.rv2 = take(M * O);
     l.carry1 = take(B * H);
     l.carry2 = take(B * O);
+    l.carry2b = take(B * O);
     l.lam = take(MT * H);           // dL/dv1 of every step
     l.g2 = take(M * O);
     l.part2 = take((M + kDw2Rows - 1) / kDw2Rows * O * H);
@@ -788,13 +826,27 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     fill(part2, (long)chunks * O * H, 0.0f, s);
     fill(carry1, (long)B * H, 0.0f, s);
     fill(carry2, (long)B * O, 0.0f, s);
+    // layer 2's scan folded into layer 1's where each 256-thread block lies
+    // in one batch row; the carry then alternates between two buffers, as
+    // the row's other blocks still read the one its first block replaces
+    const bool fuse2 = H % 256 == 0 && c.checkpoint <= 16;
+    float *carry2b = (float *)(ws + l.carry2b);
     for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
         forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0, true,
                       s);
-        hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)), dim3(256), 0,
-                           s, a, rv2, g, carry2, g2);
-        hipLaunchKernelGGL(scan1_bwd, dim3(blocks((long)B * H)), dim3(256), 0,
-                           s, a, w2, rv1, g2, carry1, lam + (long)blk * M * H);
+        float *lam_blk = lam + (long)blk * M * H;
+        if (fuse2) {
+            hipLaunchKernelGGL(scan1_bwd<true>, dim3(blocks((long)B * H)),
+                               dim3(256), 0, s, a, w2, rv1, g2, carry1,
+                               lam_blk, rv2, g, carry2, carry2b);
+            std::swap(carry2, carry2b);
+        } else {
+            hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)),
+                               dim3(256), 0, s, a, rv2, g, carry2, g2);
+            hipLaunchKernelGGL(scan1_bwd<false>, dim3(blocks((long)B * H)),
+                               dim3(256), 0, s, a, w2, rv1, g2, carry1,
+                               lam_blk, nullptr, nullptr, nullptr, nullptr);
+        }
         hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
                            dim3(256), 0, s, g2, s1, part2, M, H, O);
     }

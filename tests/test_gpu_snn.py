@@ -140,6 +140,24 @@ def test_checkpoint_spacing():
 
 
 @pytest.mark.gpu
+def test_folded_output_scan_matches_separate():
+    """n_hid 256: checkpoints of 8 steps run layer 2's backward scan inside
+    layer 1's launch, checkpoints of 20 steps in a launch of its own; the
+    losses and counts agree bit for bit and the gradients to 1e-6."""
+    print("case snn folded output scan", flush=True)
+    out = {}
+    for ck in (8, 20):
+        cfg, x, w1, w2, labels = _case(adj.SPIKE, batch=6, steps=40, n_in=50,
+                                       n_hid=256, n_out=10, ck=ck)
+        out[ck] = _run(cfg, x, w1, w2, labels)
+    (l0, c0, a0, b0), (l1, c1, a1, b1) = out[8], out[20]
+    assert l0 == l1 and np.array_equal(c0, c1)
+    assert np.abs(a0).max() > 0 and np.abs(b0).max() > 0
+    _close(a0, a1, 1e-6)
+    _close(b0, b1, 1e-6)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", [adj.RATE, adj.SPIKE])
 def test_full_size_against_fp64(mode):
     print("case snn full size", mode, flush=True)
