@@ -25,12 +25,12 @@
 //                          reverse: d1 = dL/dv1 from W2^T d2, kept for
 //                          every step
 //             dw2_partial  D2^T . S1 over 16-row chunks, accumulated over
-//                          the blocks
+//                          the blocks (fused: per batch row, in scan1_bwd)
 //             and after the last block
 //             gemm_tn      dW1 = D1^T . X over every step's rows (MFMA
 //                          BF16, D1 as hi + mid + lo BF16 terms), split
 //                          into row ranges
-// and add_parts / dw2_reduce add the ranges and chunks in order into dW1
+// and add_parts / dw2_reduce add the ranges and chunks in a fixed order into dW1
 // and dW2.
 // Every sum runs in a fixed order: results are deterministic.
 
@@ -464,16 +464,20 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
 // beta d1'.
 // FUSE2 (n_hid a multiple of 256, checkpoints of at most 16 steps): a
 // block lies in one batch row b, and its first O threads first run layer
-// 2's adjoint for that row exactly as scan2_bwd does -- into LDS, and into
-// g2 and the carry (carry2_out) from the row's first block -- so that
-// scan2_bwd needs no launch of its own.
+// 2's adjoint for that row exactly as scan2_bwd does -- into LDS, and the
+// carry (carry2_out) from the row's first block -- so that scan2_bwd needs
+// no launch of its own; each thread also adds its share of dW2, the sum
+// over the block's steps (latest first) of d2[o] * s1[u], to
+// part2[b][o][u] (dw2_reduce then adds the batch rows in a fixed order), so that
+// dw2_partial needs none either.
 template <bool FUSE2>
 __global__ __launch_bounds__(256) void scan1_bwd(
         ScanArgs a, const uint16_t *__restrict__ w2,
         const float *__restrict__ rec_v1, float *__restrict__ g2,
         float *__restrict__ carry1, float *__restrict__ lam,
         const float *__restrict__ rec_v2, const float *__restrict__ g,
-        const float *__restrict__ carry2_in, float *__restrict__ carry2_out) {
+        const float *__restrict__ carry2_in, float *__restrict__ carry2_out,
+        const float *__restrict__ s1, float *__restrict__ part2) {
     __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = a.H, O = a.O;
@@ -496,7 +500,6 @@ __global__ __launch_bounds__(256) void scan1_bwd(
                                 act_grad(v2[t] - a.theta, a.mode, a.slope) +
                                 a.beta * dn2;
                 d2s[t][threadIdx.x] = d;
-                if (first) g2[(long)t * BO + i2] = d;
                 dn2 = d;
             }
             if (first) carry2_out[i2] = dn2;
@@ -510,20 +513,32 @@ __global__ __launch_bounds__(256) void scan1_bwd(
     for (int o = 0; o < kMaxOut; ++o)
         wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
     float dn = carry1[i];
-    // windows of kWin steps from the last: each window's loads (v1 and the
-    // output adjoints) issued together, then the recurrence
+    float acc[kMaxOut];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
+    // windows of kWin steps from the last: each window's loads (v1, s1 and
+    // the output adjoints) issued together, then the recurrence
     for (int t1 = a.steps - 1; t1 >= 0; t1 -= kWin) {
-        float vv[kWin], gsum[kWin];
+        float vv[kWin], gsum[kWin], sv[kWin];
 #pragma unroll
         for (int j = 0; j < kWin; ++j) {
             const int t = t1 - j;
             if (t < 0) break;
             vv[j] = rec_v1[(long)t * BH + i];
+            if (FUSE2) sv[j] = s1[(long)t * BH + i];
+        }
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+            const int t = t1 - j;
+            if (t < 0) break;
             const float *d2 = FUSE2 ? d2s[t] : g2 + (long)t * BO + (long)b * O;
             float gs = 0.0f;
 #pragma unroll
             for (int o = 0; o < kMaxOut; ++o)
-                if (o < O) gs += wc[o] * d2[o];
+                if (o < O) {
+                    gs += wc[o] * d2[o];
+                    if (FUSE2) acc[o] += d2[o] * sv[j];
+                }
             gsum[j] = gs;
         }
 #pragma unroll
@@ -538,6 +553,11 @@ __global__ __launch_bounds__(256) void scan1_bwd(
         }
     }
     carry1[i] = dn;
+    if (FUSE2) {
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o)
+            if (o < O) part2[((long)b * O + o) * H + u] += acc[o];
+    }
 }
 
 constexpr int kDw2Rows = 16;   // rows of one partial sum of dW2
@@ -571,14 +591,23 @@ __global__ void dw2_partial(const float *__restrict__ g2,
         if (o < O) part[((long)c * O + o) * H + u] += acc[o];
 }
 
-// dW2[o][u] = the chunks' partial sums, in chunk order.
-__global__ void dw2_reduce(const float *__restrict__ part,
-                           float *__restrict__ dw2, int chunks, int H, int O) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= O * H) return;
+// dW2[o][u] = the partial sums (chunks or batch rows): a block covers 64
+// outputs, its wavefront w adds the w-th quarter of the slots in slot
+// order, and the quarters are added in order.
+__global__ __launch_bounds__(256) void dw2_reduce(
+        const float *__restrict__ part, float *__restrict__ dw2, int chunks,
+        int H, int O) {
+    __shared__ float q[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const int per = (chunks + 3) / 4;
+    const int c0 = w * per, c1 = min(chunks, c0 + per);
     float acc = 0.0f;
-    for (int c = 0; c < chunks; ++c) acc += part[(long)c * O * H + i];
-    dw2[i] = acc;
+    if (i < O * H)
+        for (int c = c0; c < c1; ++c) acc += part[(long)c * O * H + i];
+    q[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && i < O * H) dw2[i] = ((q[0][lane] + q[1][lane]) + q[2][lane]) + q[3][lane];
 }
 
 // Mean softmax cross-entropy over the batch (one block; sums in batch
@@ -654,6 +683,21 @@ struct Layout {
         g2, part2, part1, w2t, total;
 };
 
+// Layer 2's backward scan and dW2's partial sums run inside layer 1's
+// backward scan where each 256-thread block lies in one batch row and a
+// block of steps fits its LDS (scan1_bwd<true>).
+bool fused_backward(const sg_snn_cfg &c) {
+    return c.n_hid % 256 == 0 && c.checkpoint <= 16;
+}
+
+// dW2 partial-sum slots: one per batch row (fused backward) or per
+// kDw2Rows rows of a block of steps.
+size_t dw2_slots(const sg_snn_cfg &c) {
+    return fused_backward(c)
+        ? (size_t)c.batch
+        : ((size_t)c.checkpoint * c.batch + kDw2Rows - 1) / kDw2Rows;
+}
+
 Layout layout(const sg_snn_cfg &c) {
     const size_t B = c.batch, H = c.n_hid, O = c.n_out, ck = c.checkpoint;
     const size_t slots = (size_t)c.steps / ck + 1;
@@ -674,7 +718,7 @@ Layout layout(const sg_snn_cfg &c) {
     l.carry2b = take(B * O);
     l.lam = take(MT * H);           // dL/dv1 of every step
     l.g2 = take(M * O);
-    l.part2 = take((M + kDw2Rows - 1) / kDw2Rows * O * H);
+    l.part2 = take(dw2_slots(c) * O * H);
     l.part1 = take((size_t)kTnSplit * H * c.n_in);
     l.w2t = take((H * kMaxOut + 1) / 2);
     l.total = off;
@@ -821,15 +865,14 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
                        cnt, labels, B, O, grad_scale, g);
     hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)H * kMaxOut)),
                        dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), H, O);
-    const int chunks = (M + kDw2Rows - 1) / kDw2Rows;
+    const int chunks = (int)dw2_slots(c);
     fill(part1, (long)kTnSplit * H * c.n_in, 0.0f, s);
     fill(part2, (long)chunks * O * H, 0.0f, s);
     fill(carry1, (long)B * H, 0.0f, s);
     fill(carry2, (long)B * O, 0.0f, s);
-    // layer 2's scan folded into layer 1's where each 256-thread block lies
-    // in one batch row; the carry then alternates between two buffers, as
-    // the row's other blocks still read the one its first block replaces
-    const bool fuse2 = H % 256 == 0 && c.checkpoint <= 16;
+    // fused: layer 2's carry alternates between two buffers, as the row's
+    // other blocks still read the one its first block replaces
+    const bool fuse2 = fused_backward(c);
     float *carry2b = (float *)(ws + l.carry2b);
     for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
         forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0, true,
@@ -838,17 +881,18 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
         if (fuse2) {
             hipLaunchKernelGGL(scan1_bwd<true>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
-                               lam_blk, rv2, g, carry2, carry2b);
+                               lam_blk, rv2, g, carry2, carry2b, s1, part2);
             std::swap(carry2, carry2b);
         } else {
             hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)),
                                dim3(256), 0, s, a, rv2, g, carry2, g2);
             hipLaunchKernelGGL(scan1_bwd<false>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
-                               lam_blk, nullptr, nullptr, nullptr, nullptr);
+                               lam_blk, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr);
+            hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
+                               dim3(256), 0, s, g2, s1, part2, M, H, O);
         }
-        hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
-                           dim3(256), 0, s, g2, s1, part2, M, H, O);
     }
     // dW1 over every step's rows in one GEMM
     const int MT = c.steps * B;
@@ -857,7 +901,7 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
                        dim3(256), 0, s, lam, x, part1, MT, H, c.n_in);
     hipLaunchKernelGGL(add_parts, dim3(1024), dim3(256), 0, s, part1, dw1,
                        (long)H * c.n_in, kTnSplit);
-    hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 255) / 256), dim3(256), 0, s,
+    hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 63) / 64), dim3(256), 0, s,
                        part2, dw2, chunks, H, O);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(std::string("launch: ") + hipGetErrorString(e));
