@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void gemm_nt(const uint16_t *__restrict__ A,
         }
 }
 
-// part[z][N][K] += L[rows of range z][N]^T . X[rows of range z][K]: FP32 L
+// part[z][N][K] = L[rows of range z][N]^T . X[rows of range z][K]: FP32 L
 // as three BF16 terms, BF16 X.  64 (n) x 64 (k) tile per block, the M rows
 // split into gridDim.z contiguous ranges.
 __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
             const int col = k0 + wk + j * 16 + li;
             for (int q = 0; q < 4; ++q) {
                 const int row = n0 + wn + i * 16 + (lane >> 4) * 4 + q;
-                if (row < N && col < K) P[(long)row * K + col] += acc[i][j][q];
+                if (row < N && col < K) P[(long)row * K + col] = acc[i][j][q];
             }
         }
 }
@@ -468,7 +468,8 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
 // carry (carry2_out) from the row's first block -- so that scan2_bwd needs
 // no launch of its own; each thread also adds its share of dW2, the sum
 // over the block's steps (latest first) of d2[o] * s1[u], to
-// part2[b][o][u] (dw2_reduce then adds the batch rows in a fixed order), so that
+// part2[b][o][u] (stored by the first block of steps run, add2 == 0;
+// dw2_reduce then adds the batch rows in a fixed order), so that
 // dw2_partial needs none either.
 template <bool FUSE2>
 __global__ __launch_bounds__(256) void scan1_bwd(
@@ -477,7 +478,7 @@ __global__ __launch_bounds__(256) void scan1_bwd(
         float *__restrict__ carry1, float *__restrict__ lam,
         const float *__restrict__ rec_v2, const float *__restrict__ g,
         const float *__restrict__ carry2_in, float *__restrict__ carry2_out,
-        const float *__restrict__ s1, float *__restrict__ part2) {
+        const float *__restrict__ s1, float *__restrict__ part2, int add2) {
     __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = a.H, O = a.O;
@@ -556,7 +557,10 @@ __global__ __launch_bounds__(256) void scan1_bwd(
     if (FUSE2) {
 #pragma unroll
         for (int o = 0; o < kMaxOut; ++o)
-            if (o < O) part2[((long)b * O + o) * H + u] += acc[o];
+            if (o < O) {
+                float *p = part2 + ((long)b * O + o) * H + u;
+                *p = add2 ? *p + acc[o] : acc[o];
+            }
     }
 }
 
@@ -866,13 +870,12 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)H * kMaxOut)),
                        dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), H, O);
     const int chunks = (int)dw2_slots(c);
-    fill(part1, (long)kTnSplit * H * c.n_in, 0.0f, s);
-    fill(part2, (long)chunks * O * H, 0.0f, s);
+    const bool fuse2 = fused_backward(c);
+    if (!fuse2) fill(part2, (long)chunks * O * H, 0.0f, s);
     fill(carry1, (long)B * H, 0.0f, s);
     fill(carry2, (long)B * O, 0.0f, s);
     // fused: layer 2's carry alternates between two buffers, as the row's
     // other blocks still read the one its first block replaces
-    const bool fuse2 = fused_backward(c);
     float *carry2b = (float *)(ws + l.carry2b);
     for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
         forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0, true,
@@ -881,7 +884,8 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
         if (fuse2) {
             hipLaunchKernelGGL(scan1_bwd<true>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
-                               lam_blk, rv2, g, carry2, carry2b, s1, part2);
+                               lam_blk, rv2, g, carry2, carry2b, s1, part2,
+                               blk != c.steps / c.checkpoint - 1);
             std::swap(carry2, carry2b);
         } else {
             hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)),
@@ -889,7 +893,7 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
             hipLaunchKernelGGL(scan1_bwd<false>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
                                lam_blk, nullptr, nullptr, nullptr, nullptr,
-                               nullptr, nullptr);
+                               nullptr, nullptr, 0);
             hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
                                dim3(256), 0, s, g2, s1, part2, M, H, O);
         }
