@@ -51,7 +51,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxOut = 16;
-constexpr int kTnSplit = 16;    // row ranges of the dW1 product
+constexpr int kTnSplit = 8;     // row ranges of the dW1 product
 
 __device__ __forceinline__ uint32_t bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
