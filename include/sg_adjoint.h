@@ -29,10 +29,11 @@
  * dW1 is one MFMA GEMM over every step's adjoint states, which enter as
  * three BF16 terms (hi + mid + lo) so the product keeps FP32 precision.
  * The forward keeps the recurrent state only at every `checkpoint`-th
- * step; the backward recomputes each block of steps from its checkpoint
- * (the input currents need no recomputation).  The workspace holds
- * 2 * T * B * n_hid floats for the currents and the adjoint states besides
- * the checkpoints; T * B is at most 4,194,240.
+ * step; the backward recomputes layer 1's recurrence of each block of
+ * steps from its checkpoint (the input currents, and layer 2's potentials
+ * of every step, T * B * n_out floats, are kept by the forward).  The
+ * workspace holds 2 * T * B * n_hid floats for the currents and the
+ * adjoint states besides the checkpoints; T * B is at most 4,194,240.
  *
  * All pointers are device pointers; `stream` is a hipStream_t (NULL: the
  * default stream).  Calls are asynchronous and return 0, or an error code

@@ -13,10 +13,11 @@
 //             out_current  A2[M][n_out] = S1 . W2^T, one wavefront per row
 //             scan2_fwd    one thread per (b, output): v2, s2, counts
 //                          (checkpoints of at most 16 steps: both in one
-//                          launch, out_scan2_fwd, a block per batch row)
-//   backward  per block, last first: the recurrences again from the
+//                          launch, out_scan2_fwd, a block per batch row);
+//                          v2 of every step is kept for the backward
+//   backward  per block, last first: layer 1's recurrence again from the
 //             block's checkpoint (the currents are kept), recording v1
-//             and v2, then
+//             and s1, then
 //             scan2_bwd    one thread per (b, output), steps in reverse:
 //                          d2 = dL/dv2 (n_hid a multiple of 256 and
 //                          checkpoints of at most 16 steps: run by the
@@ -716,7 +717,7 @@ Layout layout(const sg_snn_cfg &c) {
     l.s1 = take(M * H);
     l.a2 = take(M * O);
     l.rv1 = take(M * H);
-    l.rv2 = take(M * O);
+    l.rv2 = take(MT * O);           // v2 of every step, from the forward
     l.carry1 = take(B * H);
     l.carry2 = take(B * O);
     l.carry2b = take(B * O);
@@ -769,30 +770,34 @@ unsigned blocks(long threads) { return (unsigned)((threads + 255) / 256); }
 // The forward passes of block `blk` of steps from checkpoint `in`: state
 // to `out` (or null), counts per count_mode, v records when rec_v1 is set.
 // The block's input currents are already in a1.
+// One block of steps from the checkpoint `in`.  Forward (recompute
+// false): both layers, the next checkpoint to `out`, v2 of every step
+// kept.  Backward (recompute true): layer 1 only, recording v1 and s1.
 void forward_block(const sg_snn_cfg &c, const ScanArgs &a, int blk,
                    const Layout &l, unsigned char *ws, const float *in,
-                   float *out, int count_mode, bool record, hipStream_t s) {
+                   float *out, int count_mode, bool recompute,
+                   hipStream_t s) {
     const int M = c.checkpoint * c.batch;
     float *a1 = (float *)(ws + l.a1) + (long)blk * M * c.n_hid;
     float *s1 = (float *)(ws + l.s1);
     float *a2 = (float *)(ws + l.a2), *cnt = (float *)(ws + l.counts);
+    float *rv2 = (float *)(ws + l.rv2) + (long)blk * M * c.n_out;
     hipLaunchKernelGGL(scan1_fwd, dim3(blocks((long)c.batch * c.n_hid)),
                        dim3(256), 0, s, a, a1, in, out, s1,
-                       record ? (float *)(ws + l.rv1) : nullptr);
+                       recompute ? (float *)(ws + l.rv1) : nullptr);
+    if (recompute) return;
     if (c.checkpoint <= 16) {
         hipLaunchKernelGGL(out_scan2_fwd, dim3(c.batch),
                            dim3(64 * c.checkpoint), 0, s, a, s1,
                            (const uint16_t *)(ws + l.w2t), in, out, cnt,
-                           count_mode,
-                           record ? (float *)(ws + l.rv2) : nullptr);
+                           count_mode, rv2);
         return;
     }
     hipLaunchKernelGGL(out_current, dim3((M + 3) / 4), dim3(256), 0, s, s1,
                        (const uint16_t *)(ws + l.w2t), a2, M, c.n_hid,
                        c.n_out);
     hipLaunchKernelGGL(scan2_fwd, dim3(blocks((long)c.batch * c.n_out)),
-                       dim3(256), 0, s, a, a2, in, out, cnt, count_mode,
-                       record ? (float *)(ws + l.rv2) : nullptr);
+                       dim3(256), 0, s, a, a2, in, out, cnt, count_mode, rv2);
 }
 
 }  // namespace
@@ -867,8 +872,6 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     const ScanArgs a = scan_args(c);
     hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, s,
                        cnt, labels, B, O, grad_scale, g);
-    hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)H * kMaxOut)),
-                       dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), H, O);
     const int chunks = (int)dw2_slots(c);
     const bool fuse2 = fused_backward(c);
     if (!fuse2) fill(part2, (long)chunks * O * H, 0.0f, s);
@@ -881,15 +884,16 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
         forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0, true,
                       s);
         float *lam_blk = lam + (long)blk * M * H;
+        const float *rv2_blk = rv2 + (long)blk * M * O;
         if (fuse2) {
             hipLaunchKernelGGL(scan1_bwd<true>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
-                               lam_blk, rv2, g, carry2, carry2b, s1, part2,
+                               lam_blk, rv2_blk, g, carry2, carry2b, s1, part2,
                                blk != c.steps / c.checkpoint - 1);
             std::swap(carry2, carry2b);
         } else {
             hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)),
-                               dim3(256), 0, s, a, rv2, g, carry2, g2);
+                               dim3(256), 0, s, a, rv2_blk, g, carry2, g2);
             hipLaunchKernelGGL(scan1_bwd<false>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
                                lam_blk, nullptr, nullptr, nullptr, nullptr,
