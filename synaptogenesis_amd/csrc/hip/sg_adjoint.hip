@@ -7,7 +7,8 @@
 // elementwise recurrences in time:
 //   forward   gemm_nt      A1[T*batch][n_hid] = X . W1^T for every step
 //                          (MFMA BF16), kept for the backward
-//             per block:
+//             T <= 1024: scan1_fwd_all and out_scan2_fwd_all, the passes
+//             below over all steps in two launches; else per block:
 //             scan1_fwd    one thread per (b, hidden unit): v1, s1 over
 //                          the block's steps
 //             out_current  A2[M][n_out] = S1 . W2^T, one wavefront per row
@@ -53,6 +54,7 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMaxOut = 16;
 constexpr int kTnSplit = 8;     // row ranges of the dW1 product
+constexpr int kAllSteps = 1024; // longest forward run as two passes (LDS)
 
 __device__ __forceinline__ uint32_t bf16_rne(float f) {
     uint32_t u = __float_as_uint(f);
@@ -427,6 +429,89 @@ __global__ __launch_bounds__(1024) void out_scan2_fwd(
         state_out[2 * BH + BO + i] = s;
     }
     if (count_mode != 0) counts[i] = cnt;
+}
+
+// The forward over all T steps in two passes (T <= 1024; layer 1 does not
+// read layer 2).  scan1_fwd_all: layer 1, thread = (b, u), the recurrence
+// of scan1_fwd over every step; s1 of every step to s1_all, the state after
+// each block of ck steps to that block's checkpoint slot.
+__global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
+                              const float *__restrict__ a1,
+                              float *__restrict__ state,
+                              float *__restrict__ s1_all) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = a.H;
+    if (i >= a.B * H) return;
+    const long BH = (long)a.B * H;
+    float v = state[i], s = state[BH + i];
+    int left = ck;
+    float *next = state + slot;
+    for (int t0 = 0; t0 < T; t0 += kWin) {
+        float in[kWin];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j)
+            if (t0 + j < T) in[j] = a1[(long)(t0 + j) * BH + i];
+#pragma unroll
+        for (int j = 0; j < kWin; ++j) {
+            if (t0 + j >= T) break;
+            v = a.beta * v - a.reset_theta * s + in[j];
+            s = act(v - a.theta, a.mode, a.slope);
+            s1_all[(long)(t0 + j) * BH + i] = s;
+            if (--left == 0) {
+                left = ck;
+                next[i] = v;
+                next[BH + i] = s;
+                next += slot;
+            }
+        }
+    }
+}
+
+// out_scan2_fwd_all: block = batch row b; wavefront w reduces W2 . s1 of
+// steps t = w, w + 16, ... exactly as out_scan2_fwd into LDS ([T][16]
+// floats), then threads o < O run layer 2's recurrence over every step: v2
+// to rec_v2, the state after each block of ck steps to that block's
+// checkpoint slot, the spike counts to counts.
+__global__ __launch_bounds__(1024) void out_scan2_fwd_all(
+        ScanArgs a, int T, int ck, long slot,
+        const float *__restrict__ s1_all, const uint16_t *__restrict__ w2t,
+        float *__restrict__ state, float *__restrict__ counts,
+        float *__restrict__ rec_v2) {
+    extern __shared__ float a2all[];
+    const int b = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = blockDim.x >> 6;
+    const int O = a.O;
+    for (int t = w; t < T; t += nw) {
+        float p[kMaxOut];
+        row_partials(s1_all, w2t, (long)t * a.B + b, a.H, lane, p);
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o)
+            if (o < O) {
+                const float v = wave_sum(p[o]);
+                if (lane == 0) a2all[t * kMaxOut + o] = v;
+            }
+    }
+    __syncthreads();
+    if (threadIdx.x >= O) return;
+    const int i = b * O + threadIdx.x;
+    const long BH = (long)a.B * a.H, BO = (long)a.B * O;
+    float v = state[2 * BH + i], s = state[2 * BH + BO + i];
+    float cnt = 0.0f;
+    int left = ck;
+    float *next = state + slot;
+    for (int t = 0; t < T; ++t) {
+        v = a.beta * v - a.reset_theta * s + a2all[t * kMaxOut + threadIdx.x];
+        s = act(v - a.theta, a.mode, a.slope);
+        cnt += s;
+        rec_v2[(long)t * BO + i] = v;
+        if (--left == 0) {
+            left = ck;
+            next[2 * BH + i] = v;
+            next[2 * BH + BO + i] = s;
+            next += slot;
+        }
+    }
+    counts[i] = cnt;
 }
 
 // Adjoint of layer 2 over one block of steps, in reverse; thread = (b, o):
@@ -837,9 +922,24 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)c.n_hid * kMaxOut)),
                        dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), c.n_hid,
                        c.n_out);
-    for (int blk = 0; blk < c.steps / c.checkpoint; ++blk)
-        forward_block(c, a, blk, l, ws, state + blk * slot,
-                      state + (blk + 1) * slot, blk == 0 ? 1 : 2, false, s);
+    if (c.steps <= kAllSteps) {
+        // two passes over all steps; s1 of every step goes to the adjoint
+        // states' buffer, which only the backward writes
+        float *s1_all = (float *)(ws + l.lam);
+        hipLaunchKernelGGL(scan1_fwd_all, dim3(blocks((long)c.batch * c.n_hid)),
+                           dim3(256), 0, s, a, c.steps, c.checkpoint, slot,
+                           (const float *)(ws + l.a1), state, s1_all);
+        hipLaunchKernelGGL(out_scan2_fwd_all, dim3(c.batch), dim3(1024),
+                           (size_t)c.steps * kMaxOut * sizeof(float), s, a,
+                           c.steps, c.checkpoint, slot, s1_all,
+                           (const uint16_t *)(ws + l.w2t), state, cnt,
+                           (float *)(ws + l.rv2));
+    } else {
+        for (int blk = 0; blk < c.steps / c.checkpoint; ++blk)
+            forward_block(c, a, blk, l, ws, state + blk * slot,
+                          state + (blk + 1) * slot, blk == 0 ? 1 : 2, false,
+                          s);
+    }
     hipLaunchKernelGGL(loss_kernel, dim3(1), dim3(256), c.batch * 4, s, cnt,
                        labels, c.batch, c.n_out, loss, counts);
     const hipError_t e = hipGetLastError();

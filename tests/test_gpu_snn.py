@@ -158,6 +158,23 @@ def test_folded_output_scan_matches_separate():
 
 
 @pytest.mark.gpu
+def test_long_run_per_block_forward():
+    """1040 steps: above 1024 steps the forward runs block by block instead
+    of in two passes over all steps; loss, counts and gradients against the
+    FP64 model as for short runs."""
+    print("case snn 1040 steps", flush=True)
+    cfg, x, w1, w2, labels = _case(adj.RATE, batch=2, steps=1040, n_in=20,
+                                   n_hid=256, n_out=5, ck=10, scale=1.5)
+    loss, counts, g1, g2 = _run(cfg, x, w1, w2, labels)
+    _, _, oloss, ocounts, d1, d2 = _oracle(cfg, x, w1, w2, labels)
+    # FP32 sums over 1040 steps: 1e-4 where 20 steps are held to 1e-5
+    assert abs(loss - oloss) <= 1e-4 * abs(oloss)
+    _close(counts, ocounts, 1e-4)
+    _close(g1, d1, 1e-3)
+    _close(g2, d2, 1e-3)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", [adj.RATE, adj.SPIKE])
 def test_full_size_against_fp64(mode):
     print("case snn full size", mode, flush=True)
