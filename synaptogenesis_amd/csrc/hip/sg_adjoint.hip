@@ -18,7 +18,7 @@
 //                          v2 of every step is kept for the backward
 //   backward  per block, last first: layer 1's recurrence again from the
 //             block's checkpoint (the currents are kept), recording v1
-//             and s1, then
+//             and s1 (fused below: inside scan1_bwd, in registers), then
 //             scan2_bwd    one thread per (b, output), steps in reverse:
 //                          d2 = dL/dv2 (n_hid a multiple of 256 and
 //                          checkpoints of at most 16 steps: run by the
@@ -552,11 +552,14 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
 // block lies in one batch row b, and its first O threads first run layer
 // 2's adjoint for that row exactly as scan2_bwd does -- into LDS, and the
 // carry (carry2_out) from the row's first block -- so that scan2_bwd needs
-// no launch of its own; each thread also adds its share of dW2, the sum
-// over the block's steps (latest first) of d2[o] * s1[u], to
-// part2[b][o][u] (stored by the first block of steps run, add2 == 0;
-// dw2_reduce then adds the batch rows in a fixed order), so that
-// dw2_partial needs none either.
+// no launch of its own.  Each thread then runs layer 1's recurrence of its
+// (b, u) over the block's steps again from the checkpoint ck_state (the
+// currents a1 of the block; v1 in registers, s1 = S(v1 - theta) as
+// scan1_fwd computes them), so that no recomputing launch writes v1 and
+// s1 to memory, and adds its share of dW2, the sum over the block's steps
+// (latest first) of d2[o] * s1[u], to part2[b][o][u] (stored by the first
+// block of steps run, add2 == 0; dw2_reduce then adds the batch rows in a
+// fixed order), so that dw2_partial needs no launch either.
 template <bool FUSE2>
 __global__ __launch_bounds__(256) void scan1_bwd(
         ScanArgs a, const uint16_t *__restrict__ w2,
@@ -564,7 +567,8 @@ __global__ __launch_bounds__(256) void scan1_bwd(
         float *__restrict__ carry1, float *__restrict__ lam,
         const float *__restrict__ rec_v2, const float *__restrict__ g,
         const float *__restrict__ carry2_in, float *__restrict__ carry2_out,
-        const float *__restrict__ s1, float *__restrict__ part2, int add2) {
+        const float *__restrict__ a1, const float *__restrict__ ck_state,
+        float *__restrict__ part2, int add2) {
     __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = a.H, O = a.O;
@@ -600,32 +604,63 @@ __global__ __launch_bounds__(256) void scan1_bwd(
     for (int o = 0; o < kMaxOut; ++o)
         wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
     float dn = carry1[i];
-    float acc[kMaxOut];
+    if (FUSE2) {
+        // layer 1 forward from the checkpoint, all loads issued first
+        float in[16], vr[16];
 #pragma unroll
-    for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
-    // windows of kWin steps from the last: each window's loads (v1, s1 and
-    // the output adjoints) issued together, then the recurrence
+        for (int t = 0; t < 16; ++t)
+            if (t < a.steps) in[t] = a1[(long)t * BH + i];
+        float v = ck_state[i], s = ck_state[BH + i];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            if (t >= a.steps) break;
+            v = a.beta * v - a.reset_theta * s + in[t];
+            s = act(v - a.theta, a.mode, a.slope);
+            vr[t] = v;
+        }
+        float acc[kMaxOut];
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
+#pragma unroll
+        for (int t = 15; t >= 0; --t) {
+            if (t >= a.steps) continue;
+            const float sv = act(vr[t] - a.theta, a.mode, a.slope);
+            float gs = 0.0f;
+#pragma unroll
+            for (int o = 0; o < kMaxOut; ++o)
+                if (o < O) {
+                    gs += wc[o] * d2s[t][o];
+                    acc[o] += d2s[t][o] * sv;
+                }
+            const float d = (gs - a.reset_theta * dn) *
+                            act_grad(vr[t] - a.theta, a.mode, a.slope) +
+                            a.beta * dn;
+            lam[(long)t * BH + i] = d;
+            dn = d;
+        }
+        carry1[i] = dn;
+#pragma unroll
+        for (int o = 0; o < kMaxOut; ++o)
+            if (o < O) {
+                float *p = part2 + ((long)b * O + o) * H + u;
+                *p = add2 ? *p + acc[o] : acc[o];
+            }
+        return;
+    }
+    // windows of kWin steps from the last: each window's loads (v1 and the
+    // output adjoints) issued together, then the recurrence
     for (int t1 = a.steps - 1; t1 >= 0; t1 -= kWin) {
-        float vv[kWin], gsum[kWin], sv[kWin];
+        float vv[kWin], gsum[kWin];
 #pragma unroll
         for (int j = 0; j < kWin; ++j) {
             const int t = t1 - j;
             if (t < 0) break;
             vv[j] = rec_v1[(long)t * BH + i];
-            if (FUSE2) sv[j] = s1[(long)t * BH + i];
-        }
-#pragma unroll
-        for (int j = 0; j < kWin; ++j) {
-            const int t = t1 - j;
-            if (t < 0) break;
-            const float *d2 = FUSE2 ? d2s[t] : g2 + (long)t * BO + (long)b * O;
+            const float *d2 = g2 + (long)t * BO + (long)b * O;
             float gs = 0.0f;
 #pragma unroll
             for (int o = 0; o < kMaxOut; ++o)
-                if (o < O) {
-                    gs += wc[o] * d2[o];
-                    if (FUSE2) acc[o] += d2[o] * sv[j];
-                }
+                if (o < O) gs += wc[o] * d2[o];
             gsum[j] = gs;
         }
 #pragma unroll
@@ -640,14 +675,6 @@ __global__ __launch_bounds__(256) void scan1_bwd(
         }
     }
     carry1[i] = dn;
-    if (FUSE2) {
-#pragma unroll
-        for (int o = 0; o < kMaxOut; ++o)
-            if (o < O) {
-                float *p = part2 + ((long)b * O + o) * H + u;
-                *p = add2 ? *p + acc[o] : acc[o];
-            }
-    }
 }
 
 constexpr int kDw2Rows = 16;   // rows of one partial sum of dW2
@@ -981,23 +1008,26 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     // other blocks still read the one its first block replaces
     float *carry2b = (float *)(ws + l.carry2b);
     for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
-        forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0, true,
-                      s);
         float *lam_blk = lam + (long)blk * M * H;
         const float *rv2_blk = rv2 + (long)blk * M * O;
         if (fuse2) {
+            // layer 1 recomputed inside the scan, from the checkpoint
+            const float *a1_blk = (const float *)(ws + l.a1) + (long)blk * M * H;
             hipLaunchKernelGGL(scan1_bwd<true>, dim3(blocks((long)B * H)),
-                               dim3(256), 0, s, a, w2, rv1, g2, carry1,
-                               lam_blk, rv2_blk, g, carry2, carry2b, s1, part2,
+                               dim3(256), 0, s, a, w2, nullptr, g2, carry1,
+                               lam_blk, rv2_blk, g, carry2, carry2b, a1_blk,
+                               state + blk * slot, part2,
                                blk != c.steps / c.checkpoint - 1);
             std::swap(carry2, carry2b);
         } else {
+            forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0,
+                          true, s);
             hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)),
                                dim3(256), 0, s, a, rv2_blk, g, carry2, g2);
             hipLaunchKernelGGL(scan1_bwd<false>, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
                                lam_blk, nullptr, nullptr, nullptr, nullptr,
-                               nullptr, nullptr, 0);
+                               nullptr, nullptr, nullptr, 0);
             hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
                                dim3(256), 0, s, g2, s1, part2, M, H, O);
         }
