@@ -16,18 +16,19 @@
 //                          (checkpoints of at most 16 steps: both in one
 //                          launch, out_scan2_fwd, a block per batch row);
 //                          v2 of every step is kept for the backward
-//   backward  per block, last first: layer 1's recurrence again from the
-//             block's checkpoint (the currents are kept), recording v1
-//             and s1 (fused below: inside scan1_bwd, in registers), then
+//   backward  n_hid a multiple of 256 and checkpoints of at most 16
+//             steps: scan_bwd_fused, all of the passes below for every
+//             block of steps in one launch (dW2 summed per batch row);
+//             else per block, last first: layer 1's recurrence again from
+//             the block's checkpoint (the currents are kept), recording
+//             v1 and s1, then
 //             scan2_bwd    one thread per (b, output), steps in reverse:
-//                          d2 = dL/dv2 (n_hid a multiple of 256 and
-//                          checkpoints of at most 16 steps: run by the
-//                          first threads of each scan1_bwd block instead)
+//                          d2 = dL/dv2
 //             scan1_bwd    one thread per (b, hidden unit), steps in
 //                          reverse: d1 = dL/dv1 from W2^T d2, kept for
 //                          every step
 //             dw2_partial  D2^T . S1 over 16-row chunks, accumulated over
-//                          the blocks (fused: per batch row, in scan1_bwd)
+//                          the blocks
 //             and after the last block
 //             gemm_tn      dW1 = D1^T . X over every step's rows (MFMA
 //                          BF16, D1 as hi + mid + lo BF16 terms), split
@@ -41,7 +42,6 @@
 
 #include <cstring>
 #include <string>
-#include <utility>
 
 #include "sg_adjoint.h"
 
@@ -545,45 +545,49 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
     carry2[i] = dn;
 }
 
-// Adjoint of layer 1 over one block of steps, in reverse; thread = (b, u):
-// d1 = (sum over o of W2[o][u] d2[o] - r theta d1') S'(v1 - theta) +
-// beta d1'.
-// FUSE2 (n_hid a multiple of 256, checkpoints of at most 16 steps): a
-// block lies in one batch row b, and its first O threads first run layer
-// 2's adjoint for that row exactly as scan2_bwd does -- into LDS, and the
-// carry (carry2_out) from the row's first block -- so that scan2_bwd needs
-// no launch of its own.  Each thread then runs layer 1's recurrence of its
-// (b, u) over the block's steps again from the checkpoint ck_state (the
-// currents a1 of the block; v1 in registers, s1 = S(v1 - theta) as
-// scan1_fwd computes them), so that no recomputing launch writes v1 and
-// s1 to memory, and adds its share of dW2, the sum over the block's steps
-// (latest first) of d2[o] * s1[u], to part2[b][o][u] (stored by the first
-// block of steps run, add2 == 0; dw2_reduce then adds the batch rows in a
-// fixed order), so that dw2_partial needs no launch either.
-template <bool FUSE2>
-__global__ __launch_bounds__(256) void scan1_bwd(
-        ScanArgs a, const uint16_t *__restrict__ w2,
-        const float *__restrict__ rec_v1, float *__restrict__ g2,
-        float *__restrict__ carry1, float *__restrict__ lam,
+// The whole backward in one launch (n_hid a multiple of 256, checkpoints
+// of at most 16 steps): a block of 256 threads = 256 hidden units of one
+// batch row b; it walks the blocks of steps from the last, and per block
+//  - threads o < O run layer 2's adjoint for (b, o) over the block's steps
+//    exactly as scan2_bwd does, into LDS (every block of the row computes
+//    the same values; the carry stays in a register);
+//  - every thread runs layer 1's recurrence of its (b, u) again from the
+//    block's checkpoint (the currents a1; v1 in registers, s1 = S(v1 -
+//    theta) as scan1_fwd computes them), then its adjoint in reverse as
+//    scan1_bwd does (d1 of every step to lam, the carry in a register),
+//    and the block's share of dW2, the sum over its steps (latest first)
+//    of d2[o] * s1[u], added to a running sum in block order.
+// The dW2 sums go to part2[b][o][u] (dw2_reduce adds the batch rows in a
+// fixed order).  No scan2_bwd, dw2_partial or recomputing launch.
+__global__ __launch_bounds__(256) void scan_bwd_fused(
+        ScanArgs a, int nblk, long slot, const uint16_t *__restrict__ w2,
+        const float *__restrict__ a1, const float *__restrict__ state,
         const float *__restrict__ rec_v2, const float *__restrict__ g,
-        const float *__restrict__ carry2_in, float *__restrict__ carry2_out,
-        const float *__restrict__ a1, const float *__restrict__ ck_state,
-        float *__restrict__ part2, int add2) {
+        float *__restrict__ lam, float *__restrict__ part2) {
     __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = a.H, O = a.O;
     const long BH = (long)a.B * H, BO = (long)a.B * O;
-    if (FUSE2) {
-        const int b0 = (int)(((long)blockIdx.x * blockDim.x) / H);
-        const bool first = ((long)blockIdx.x * blockDim.x) % H == 0;
-        if (threadIdx.x < O && b0 < a.B) {
-            const int i2 = b0 * O + threadIdx.x;
-            const float gb = g[i2];
-            float dn2 = carry2_in[i2];
+    const long M = (long)a.steps * a.B;   // rows of a block of steps
+    const int b = i / H, u = i % H;
+    const bool out_thread = threadIdx.x < O;
+    const int i2 = b * O + (out_thread ? threadIdx.x : 0);
+    const float gb = out_thread ? g[i2] : 0.0f;
+    float wc[kMaxOut];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o)
+        wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
+    float dn = 0.0f, dn2 = 0.0f;
+    float tot[kMaxOut];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o) tot[o] = 0.0f;
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+        if (out_thread) {
+            const float *rv = rec_v2 + blk * M * O;
             float v2[16];
 #pragma unroll
             for (int t = 0; t < 16; ++t)
-                if (t < a.steps) v2[t] = rec_v2[(long)t * BO + i2];
+                if (t < a.steps) v2[t] = rv[(long)t * BO + i2];
 #pragma unroll
             for (int t = 15; t >= 0; --t) {
                 if (t >= a.steps) continue;
@@ -593,24 +597,16 @@ __global__ __launch_bounds__(256) void scan1_bwd(
                 d2s[t][threadIdx.x] = d;
                 dn2 = d;
             }
-            if (first) carry2_out[i2] = dn2;
         }
-        __syncthreads();
-    }
-    if (i >= a.B * H) return;
-    const int b = i / H, u = i % H;
-    float wc[kMaxOut];
-#pragma unroll
-    for (int o = 0; o < kMaxOut; ++o)
-        wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
-    float dn = carry1[i];
-    if (FUSE2) {
         // layer 1 forward from the checkpoint, all loads issued first
+        const float *ab = a1 + blk * M * H;
+        const float *ck = state + blk * slot;
         float in[16], vr[16];
 #pragma unroll
         for (int t = 0; t < 16; ++t)
-            if (t < a.steps) in[t] = a1[(long)t * BH + i];
-        float v = ck_state[i], s = ck_state[BH + i];
+            if (t < a.steps) in[t] = ab[(long)t * BH + i];
+        float v = ck[i], s = ck[BH + i];
+        __syncthreads();   // d2s of this block complete
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             if (t >= a.steps) break;
@@ -621,6 +617,7 @@ __global__ __launch_bounds__(256) void scan1_bwd(
         float acc[kMaxOut];
 #pragma unroll
         for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
+        float *lb = lam + blk * M * H;
 #pragma unroll
         for (int t = 15; t >= 0; --t) {
             if (t >= a.steps) continue;
@@ -635,18 +632,37 @@ __global__ __launch_bounds__(256) void scan1_bwd(
             const float d = (gs - a.reset_theta * dn) *
                             act_grad(vr[t] - a.theta, a.mode, a.slope) +
                             a.beta * dn;
-            lam[(long)t * BH + i] = d;
+            lb[(long)t * BH + i] = d;
             dn = d;
         }
-        carry1[i] = dn;
 #pragma unroll
         for (int o = 0; o < kMaxOut; ++o)
-            if (o < O) {
-                float *p = part2 + ((long)b * O + o) * H + u;
-                *p = add2 ? *p + acc[o] : acc[o];
-            }
-        return;
+            tot[o] = blk == nblk - 1 ? acc[o] : tot[o] + acc[o];
+        __syncthreads();   // d2s read by every thread before it changes
     }
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o)
+        if (o < O) part2[((long)b * O + o) * H + u] = tot[o];
+}
+
+// Adjoint of layer 1 over one block of steps, in reverse; thread = (b, u):
+// d1 = (sum over o of W2[o][u] d2[o] - r theta d1') S'(v1 - theta) +
+// beta d1'.
+__global__ void scan1_bwd(ScanArgs a, const uint16_t *__restrict__ w2,
+                          const float *__restrict__ rec_v1,
+                          const float *__restrict__ g2,
+                          float *__restrict__ carry1,
+                          float *__restrict__ lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int H = a.H, O = a.O;
+    if (i >= a.B * H) return;
+    const int b = i / H, u = i % H;
+    const long BH = (long)a.B * H, BO = (long)a.B * O;
+    float wc[kMaxOut];
+#pragma unroll
+    for (int o = 0; o < kMaxOut; ++o)
+        wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
+    float dn = carry1[i];
     // windows of kWin steps from the last: each window's loads (v1 and the
     // output adjoints) issued together, then the recurrence
     for (int t1 = a.steps - 1; t1 >= 0; t1 -= kWin) {
@@ -795,14 +811,13 @@ size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace carve-up, in bytes from the base.
 struct Layout {
-    size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, carry2b,
-        lam,
+    size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, lam,
         g2, part2, part1, w2t, total;
 };
 
-// Layer 2's backward scan and dW2's partial sums run inside layer 1's
-// backward scan where each 256-thread block lies in one batch row and a
-// block of steps fits its LDS (scan1_bwd<true>).
+// The whole backward runs in one launch (scan_bwd_fused) where each
+// 256-thread block lies in one batch row and a block of steps fits its
+// LDS table of output adjoints.
 bool fused_backward(const sg_snn_cfg &c) {
     return c.n_hid % 256 == 0 && c.checkpoint <= 16;
 }
@@ -832,7 +847,6 @@ Layout layout(const sg_snn_cfg &c) {
     l.rv2 = take(MT * O);           // v2 of every step, from the forward
     l.carry1 = take(B * H);
     l.carry2 = take(B * O);
-    l.carry2b = take(B * O);
     l.lam = take(MT * H);           // dL/dv1 of every step
     l.g2 = take(M * O);
     l.part2 = take(dw2_slots(c) * O * H);
@@ -1000,34 +1014,25 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, s,
                        cnt, labels, B, O, grad_scale, g);
     const int chunks = (int)dw2_slots(c);
-    const bool fuse2 = fused_backward(c);
-    if (!fuse2) fill(part2, (long)chunks * O * H, 0.0f, s);
-    fill(carry1, (long)B * H, 0.0f, s);
-    fill(carry2, (long)B * O, 0.0f, s);
-    // fused: layer 2's carry alternates between two buffers, as the row's
-    // other blocks still read the one its first block replaces
-    float *carry2b = (float *)(ws + l.carry2b);
-    for (int blk = c.steps / c.checkpoint - 1; blk >= 0; --blk) {
-        float *lam_blk = lam + (long)blk * M * H;
-        const float *rv2_blk = rv2 + (long)blk * M * O;
-        if (fuse2) {
-            // layer 1 recomputed inside the scan, from the checkpoint
-            const float *a1_blk = (const float *)(ws + l.a1) + (long)blk * M * H;
-            hipLaunchKernelGGL(scan1_bwd<true>, dim3(blocks((long)B * H)),
-                               dim3(256), 0, s, a, w2, nullptr, g2, carry1,
-                               lam_blk, rv2_blk, g, carry2, carry2b, a1_blk,
-                               state + blk * slot, part2,
-                               blk != c.steps / c.checkpoint - 1);
-            std::swap(carry2, carry2b);
-        } else {
+    const int nblk = c.steps / c.checkpoint;
+    if (fused_backward(c)) {
+        hipLaunchKernelGGL(scan_bwd_fused, dim3((unsigned)((long)B * H / 256)),
+                           dim3(256), 0, s, a, nblk, slot, w2,
+                           (const float *)(ws + l.a1), state, rv2, g, lam,
+                           part2);
+    } else {
+        fill(part2, (long)chunks * O * H, 0.0f, s);
+        fill(carry1, (long)B * H, 0.0f, s);
+        fill(carry2, (long)B * O, 0.0f, s);
+        for (int blk = nblk - 1; blk >= 0; --blk) {
             forward_block(c, a, blk, l, ws, state + blk * slot, nullptr, 0,
                           true, s);
             hipLaunchKernelGGL(scan2_bwd, dim3(blocks((long)B * O)),
-                               dim3(256), 0, s, a, rv2_blk, g, carry2, g2);
-            hipLaunchKernelGGL(scan1_bwd<false>, dim3(blocks((long)B * H)),
+                               dim3(256), 0, s, a, rv2 + (long)blk * M * O,
+                               g, carry2, g2);
+            hipLaunchKernelGGL(scan1_bwd, dim3(blocks((long)B * H)),
                                dim3(256), 0, s, a, w2, rv1, g2, carry1,
-                               lam_blk, nullptr, nullptr, nullptr, nullptr,
-                               nullptr, nullptr, nullptr, 0);
+                               lam + (long)blk * M * H);
             hipLaunchKernelGGL(dw2_partial, dim3((H + 255) / 256, chunks),
                                dim3(256), 0, s, g2, s1, part2, M, H, O);
         }

@@ -1077,7 +1077,7 @@ void State::to_device(bool allow_compact) {
         if (cs->packed) {
             std::vector<float> src;
             cs->compact = allow_compact && compact_sources(*cs, src);
-            const int nf = cs->compact ? SG_IZC_FIELDS : SG_IZ_FIELDS;
+            const int nf = cs->compact ? (int)SG_IZC_FIELDS : (int)SG_IZ_FIELDS;
             const size_t n = tiled_elems(cs->num_dest,
                                          (cs->k_per_dest + 7) / 8 * 8) * nf;
             if (cs->d_record && cs->record_fields != nf) {
