@@ -141,9 +141,10 @@ def test_checkpoint_spacing():
 
 @pytest.mark.gpu
 def test_folded_output_scan_matches_separate():
-    """n_hid 256: checkpoints of 8 steps run layer 2's backward scan inside
-    layer 1's launch, checkpoints of 20 steps in a launch of its own; the
-    losses and counts agree bit for bit and the gradients to 1e-6."""
+    """n_hid 256: checkpoints of 8 steps run the whole backward (layer 1
+    recomputed in registers, both adjoint scans, dW2's per-row sums) in one
+    launch, checkpoints of 20 steps as separate passes per block of steps;
+    the losses and counts agree bit for bit and the gradients to 1e-6."""
     print("case snn folded output scan", flush=True)
     out = {}
     for ck in (8, 20):
