@@ -175,9 +175,11 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
     // tile rows, so the stores spread over the LDS banks
     const int rp = (t >> 4) * 2, nc = t & 15;
     f32x4 acc[2][2] = {};
-    for (int r0 = rb; r0 < re; r0 += 32) {
-        float lv[2][4];
-        uint16_t xv[2][4];
+    float lv[2][4];
+    uint16_t xv[2][4];
+    // the staged values of rows r0 .. r0 + 31 (zeros past the range)
+    auto load = [&](int r0) {
+#pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int r = r0 + rp + h;
             const bool in = r < re;
@@ -190,6 +192,9 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
                                                         : (uint16_t)0;
             }
         }
+    };
+    load(rb);
+    for (int r0 = rb; r0 < re; r0 += 32) {
         for (int e = 0; e < 4; ++e) {
             uint32_t term[2][3];
             for (int h = 0; h < 2; ++h) {
@@ -208,6 +213,8 @@ __global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
                 (uint32_t)xv[0][e] | ((uint32_t)xv[1][e] << 16);
         }
         __syncthreads();
+        // the next rows' loads in flight during this tile's MFMAs
+        if (r0 + 32 < re) load(r0 + 32);
         bf16x8 b[2];
         for (int j = 0; j < 2; ++j)
             b[j] = *(const bf16x8 *)&Bs[wk + j * 16 + li][lk];
