@@ -59,7 +59,9 @@ void sg_ens_destroy(sg_ens *e);
 /* Launch shape: row_tiles 16-row tiles per block, each split over
  * k_split wavefronts along the fan-in (BF16: row_tiles * k_split in
  * {1,2,4,8,16} wavefronts; FP32: k_split must be 1, row_tiles 4-row
- * groups).  Changing it drops any captured graph. */
+ * groups).  Changing it drops any captured graph.  It first waits for
+ * every run still pending (on the ensemble's stream and on the stream of
+ * the last sg_ens_run), whose kernels read the work list it rewrites. */
 int sg_ens_set_tile(sg_ens *e, int row_tiles, int k_split);
 int sg_ens_get_tile(const sg_ens *e, int *row_tiles, int *k_split);
 
@@ -74,7 +76,8 @@ int sg_ens_autotune(sg_ens *e, int steps, int *row_tiles, int *k_split,
  * count and launch shape) and launch it once.  Asynchronous. */
 int sg_ens_run(sg_ens *e, int steps, int use_graph, void *stream);
 
-/* Wait for everything queued on the ensemble's own stream. */
+/* Wait for everything queued on the ensemble's own stream and on the
+ * stream of the last sg_ens_run. */
 int sg_ens_synchronize(sg_ens *e);
 
 /* Average duration of one step launch over `steps` launches, measured with

@@ -215,6 +215,10 @@ typedef struct sg_iz_layer {
     int noise, noise_dist, noise_op;
     float noise_a, noise_b, noise_val;
     uint64_t noise_seed, noise_step;
+    /* Optional: the draw's step is *noise_base + noise_step (a device
+     * counter, so that a captured HIP graph draws new values on every
+     * replay); NULL: noise_step itself. */
+    const uint64_t *noise_base;
 } sg_iz_layer;
 
 /* Base activator: every connection type, extractor and aggregator
@@ -302,10 +306,15 @@ int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
 /* Random init configs (engine/kernel/kernel.cpp:43-158 via
  * engine/instruction.h:208-282): dist 0 = uniform[a,b), 1 = normal(a, b),
  * 2 = poisson (fires `val` with probability a).  Counter-based Philox keyed
- * by (seed, step), so runs are reproducible. */
+ * by (seed, step), so runs are reproducible.  step_base: NULL, or a device
+ * counter added to `step` (as sg_iz_layer.noise_base). */
 int sg_hip_init_random(float *p, int n, int dist, float a, float b, float val,
                        int overwrite, uint64_t seed, uint64_t step,
-                       void *stream);
+                       const uint64_t *step_base, void *stream);
+
+/* *counter += add on the device (advances the step counter of a captured
+ * run of steps at its end). */
+int sg_hip_add_u64(uint64_t *counter, uint64_t add, void *stream);
 
 /* set_data (engine/kernel/kernel.cpp:14-40). */
 int sg_hip_set_f32(float *p, float val, long n, int overwrite, void *stream);

@@ -86,6 +86,58 @@ def workspace_bytes(cfg):
     return n.value
 
 
+_checked_labels = {}
+
+
+def validate(x, w1, w2, labels, cfg):
+    """Raise ValueError unless the arguments match `cfg` and each other:
+    x (steps, batch, n_in) BF16, w1 (n_hid, n_in) and w2 (n_out, n_hid) in
+    FP32 or BF16, labels (batch,) int32 with every label in [0, n_out), all
+    on one HIP device.  Runs before any kernel is launched: the kernels
+    index the buffers by cfg and would read past a smaller one.  The label
+    range needs one device read; a labels tensor already checked (same
+    storage, same version) is not read again."""
+    import torch
+    args = {"x": x, "w1": w1, "w2": w2, "labels": labels}
+    for name, t in args.items():
+        if not isinstance(t, torch.Tensor):
+            raise ValueError("%s must be a tensor" % name)
+        if not t.is_cuda:
+            raise ValueError("sg_snn runs on a HIP device only (%s is on "
+                             "%s)" % (name, t.device))
+    devs = {t.device for t in args.values()}
+    if len(devs) != 1:
+        raise ValueError("x, w1, w2 and labels must be on one device, got "
+                         "%s" % sorted(str(d) for d in devs))
+    for name in ("batch", "steps", "n_in", "n_hid", "n_out", "checkpoint"):
+        if getattr(cfg, name) <= 0:
+            raise ValueError("cfg.%s must be positive" % name)
+    want = {"x": (cfg.steps, cfg.batch, cfg.n_in),
+            "w1": (cfg.n_hid, cfg.n_in), "w2": (cfg.n_out, cfg.n_hid),
+            "labels": (cfg.batch,)}
+    for name, shape in want.items():
+        if tuple(args[name].shape) != shape:
+            raise ValueError("%s has shape %s, cfg needs %s"
+                             % (name, tuple(args[name].shape), shape))
+    if x.dtype != torch.bfloat16:
+        raise ValueError("x must be bfloat16, got %s" % x.dtype)
+    for name in ("w1", "w2"):
+        if args[name].dtype not in (torch.float32, torch.bfloat16):
+            raise ValueError("%s must be float32 or bfloat16, got %s"
+                             % (name, args[name].dtype))
+    if labels.dtype != torch.int32:
+        raise ValueError("labels must be int32, got %s" % labels.dtype)
+    key = (labels.data_ptr(), labels.device.index, cfg.batch)
+    version = (labels._version, cfg.n_out)
+    if _checked_labels.get(key) != version:
+        lo = int(labels.min())
+        hi = int(labels.max())
+        if lo < 0 or hi >= cfg.n_out:
+            raise ValueError("labels must lie in [0, %d), found [%d, %d]"
+                             % (cfg.n_out, lo, hi))
+        _checked_labels[key] = version
+
+
 def _function():
     import torch
 
@@ -94,11 +146,7 @@ def _function():
 
         @staticmethod
         def forward(ctx, x, w1, w2, labels, cfg):
-            for t in (x, w1, w2, labels):
-                if not t.is_cuda:
-                    raise RuntimeError("sg_snn runs on a HIP device only")
-            assert x.dtype == torch.bfloat16 and labels.dtype == torch.int32
-            assert tuple(x.shape) == (cfg.steps, cfg.batch, cfg.n_in)
+            validate(x, w1, w2, labels, cfg)
             w1b = w1.detach().to(torch.bfloat16).contiguous()
             w2b = w2.detach().to(torch.bfloat16).contiguous()
             x = x.contiguous()

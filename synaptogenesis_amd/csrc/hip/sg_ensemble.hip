@@ -295,6 +295,9 @@ struct sg_ens {
     uint32_t workload = 0;
     std::vector<int> widths;
     hipStream_t stream = nullptr;
+    // stream of the last sg_ens_run (the caller's, or `stream`): work there
+    // may still read the items, the layers and the state
+    hipStream_t last = nullptr;
     // per instance: state offsets (floats) of x0, h1[2], h2[2]
     std::vector<long> x0_off, h1_off[2], h2_off[2];
     std::vector<long> w_off;          // per (instance, layer - 1), bytes
@@ -374,9 +377,18 @@ void drop_graphs(sg_ens *e) {
     e->graph_defs.clear();
 }
 
+// Waits for the work of the last run, wherever it was launched.
+int wait_runs(sg_ens *e) {
+    if (e->last) ENS_CHECK(hipStreamSynchronize(e->last), "sync last run");
+    if (e->stream) ENS_CHECK(hipStreamSynchronize(e->stream), "sync");
+    return 0;
+}
+
 // Work list for the current launch shape: one item per block of rows,
 // layers with the longest fan-in first so the short ones fill the tail.
+// Waits for pending runs first: their kernels read the list it rewrites.
 int build_items(sg_ens *e) {
+    if (wait_runs(e)) return 1;
     std::vector<int> order(e->layers.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -532,6 +544,7 @@ sg_ens *sg_ens_create(int n, const int *widths, int precision, float leak,
 
 void sg_ens_destroy(sg_ens *e) {
     if (!e) return;
+    if (e->last) (void)hipStreamSynchronize(e->last);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
     if (e->d_w) (void)hipFree(e->d_w);
@@ -568,6 +581,7 @@ int sg_ens_run(sg_ens *e, int steps, int use_graph, void *stream) {
     if (steps < 0) return fail("negative step count");
     hipStream_t s = stream ? (hipStream_t)stream : e->stream;
     if (steps == 0) return 0;
+    e->last = s;
     if (!use_graph) {
         for (int t = 0; t < steps; ++t) {
             if (launch_step(e, e->parity, s)) return 1;
@@ -611,8 +625,7 @@ int sg_ens_run(sg_ens *e, int steps, int use_graph, void *stream) {
 
 int sg_ens_synchronize(sg_ens *e) {
     if (!e) return fail("null ensemble");
-    ENS_CHECK(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-    return 0;
+    return wait_runs(e);
 }
 
 int sg_ens_time_steps(sg_ens *e, int steps, float *avg_ms) {

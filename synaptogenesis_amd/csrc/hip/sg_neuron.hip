@@ -109,9 +109,11 @@ __device__ __forceinline__ void iz_step(const sg_iz_layer &l, int nid) {
         float base_current = l.input[nid];
         if (l.noise) {
             bool fired;
+            const uint64_t step = l.noise_base
+                ? *l.noise_base + l.noise_step : l.noise_step;
             const float v = init_value(l.noise_dist, l.noise_a, l.noise_b,
-                                       l.noise_val, l.noise_seed,
-                                       l.noise_step, nid, fired);
+                                       l.noise_val, l.noise_seed, step, nid,
+                                       fired);
             base_current = l.noise == 1
                 ? v : aggregate(l.noise_op, v, base_current);
             l.input[nid] = base_current;
@@ -270,9 +272,10 @@ __global__ __launch_bounds__(kNeuronBlock) void set_f32_kernel(
 
 __global__ __launch_bounds__(kNeuronBlock) void init_random_kernel(
         float *p, int n, int dist, float a, float b, float val, int overwrite,
-        uint64_t seed, uint64_t step) {
+        uint64_t seed, uint64_t step, const uint64_t *step_base) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (step_base) step += *step_base;
     bool fired;
     const float v = init_value(dist, a, b, val, seed, step, i, fired);
     if (dist == 2 && !overwrite) {
@@ -418,11 +421,21 @@ extern "C" int sg_hip_relay_neuron(float *input, float *output, int size,
 extern "C" int sg_hip_init_random(float *p, int n, int dist, float a,
                                   float b, float val, int overwrite,
                                   uint64_t seed, uint64_t step,
-                                  void *stream) {
+                                  const uint64_t *step_base, void *stream) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(init_random_kernel, dim3(nblocks(n)),
                        dim3(kNeuronBlock), 0, (hipStream_t)stream, p, n, dist,
-                       a, b, val, overwrite, seed, step);
+                       a, b, val, overwrite, seed, step, step_base);
+    return hipGetLastError();
+}
+
+__global__ void add_u64_kernel(uint64_t *counter, uint64_t add) {
+    if (threadIdx.x == 0) *counter += add;
+}
+
+extern "C" int sg_hip_add_u64(uint64_t *counter, uint64_t add, void *stream) {
+    hipLaunchKernelGGL(add_u64_kernel, dim3(1), dim3(64), 0,
+                       (hipStream_t)stream, counter, add);
     return hipGetLastError();
 }
 

@@ -2,13 +2,16 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <deque>
 #include <functional>
-#include <cstring>
+#include <initializer_list>
 #include <map>
 #include <mutex>
 #include <set>
+#include <unordered_map>
 
 #include "callbacks.h"
 
@@ -22,18 +25,280 @@ std::atomic<bool> g_interrupt(false);
 
 enum { IO_INPUT = 1, IO_OUTPUT = 2 };
 
-// Pinned host staging buffer of one (layer, key).
+// ---------------------------------------------------------------------------
+// Scheduler: streams and the dependencies between their launches.
+//
+// Every launch (a "task") names the arrays it reads and writes.  A task
+// waits for the last writer of everything it touches and, for what it
+// writes, for every reader since that write -- but only where those ran on
+// another stream; launches on one stream are ordered by the stream.  When
+// more than one stream is in use, each task records an event from its
+// stream's ring, and a wait targets the event of the exact task it depends
+// on (or a later one on the same stream, if the ring has come round: that
+// only waits longer).  This keeps the serial order's results while letting
+// independent launches overlap.  Inside a HIP graph capture the same
+// records and waits become the graph's edges.
+// ---------------------------------------------------------------------------
+
+enum ResKind { RES_IN = 0, RES_OUT = 1, RES_STATE = 2, RES_SYN = 3,
+               RES_TSS = 4, RES_CTR = 5, RES_INTER = 6 };
+typedef uint64_t Res;
+
+Res res(ResKind k, const void *p) {
+    return ((uint64_t)(uintptr_t)p << 3) | (uint64_t)k;
+}
+
+class Sched {
+  public:
+    ~Sched() {
+        for (auto &s : st_) {
+            if (s.s) (void)hipStreamSynchronize(s.s);
+            for (auto e : s.ring) (void)hipEventDestroy(e);
+            if (s.s) (void)hipStreamDestroy(s.s);
+        }
+    }
+
+    // At least n streams (created once, kept between runs; all at the
+    // default priority).
+    void ensure(int n) {
+        const unsigned flags = hipEventDisableTiming |
+                               hipEventDisableSystemFence;
+        while ((int)st_.size() < n) {
+            Str s;
+            check_hip(hipStreamCreate(&s.s), "hipStreamCreate");
+            s.ring.resize(kRing);
+            s.ring_seq.assign(kRing, 0);
+            for (auto &e : s.ring)
+                check_hip(hipEventCreateWithFlags(&e, flags),
+                          "hipEventCreate");
+            st_.push_back(s);
+        }
+        for (auto &s : st_) s.seen.resize(st_.size(), 0);
+    }
+    hipStream_t stream(int i) const { return st_[i].s; }
+    int count() const { return (int)st_.size(); }
+
+    // Streams that take part in this run: 1 keeps everything on stream 0
+    // and records no events.
+    void set_used(int n) {
+        used_ = n;
+        res_.clear();
+    }
+    bool multi() const { return used_ > 1; }
+
+    hipStream_t begin(int i, std::initializer_list<Res> reads,
+                      std::initializer_list<Res> writes) {
+        cur_r_.assign(reads.begin(), reads.end());
+        cur_w_.assign(writes.begin(), writes.end());
+        return start(i);
+    }
+    hipStream_t begin(int i, const std::vector<Res> &reads,
+                      const std::vector<Res> &writes) {
+        cur_r_ = reads;
+        cur_w_ = writes;
+        return start(i);
+    }
+
+    void end() {
+        const int i = cur_;
+        const uint64_t seq = ++st_[i].seq;
+        if (multi()) {
+            mark(i);
+            for (Res r : cur_r_) {
+                Entry &e = entry(r);
+                e.readers[i] = seq;
+            }
+            for (Res w : cur_w_) {
+                Entry &e = entry(w);
+                e.writer = i;
+                e.wseq = seq;
+                std::fill(e.readers.begin(), e.readers.end(), 0);
+            }
+        }
+        cur_ = -1;
+    }
+
+  private:
+    hipStream_t start(int i) {
+        if (!multi()) i = 0;
+        cur_ = i;
+        if (multi()) {
+            for (Res r : cur_r_) {
+                auto it = res_.find(r);
+                if (it != res_.end()) dep(i, it->second.writer,
+                                          it->second.wseq);
+            }
+            for (Res w : cur_w_) {
+                auto it = res_.find(w);
+                if (it == res_.end()) continue;
+                dep(i, it->second.writer, it->second.wseq);
+                for (int j = 0; j < used_; ++j)
+                    dep(i, j, it->second.readers[j]);
+            }
+        }
+        return st_[i].s;
+    }
+
+  public:
+    // Stream 0 waits for every other stream's work so far.
+    void join() {
+        if (!multi()) return;
+        for (int j = 1; j < used_; ++j) {
+            ++st_[j].seq;
+            mark(j);
+            dep(0, j, st_[j].seq);
+        }
+    }
+    // Every other stream waits for stream 0's work so far.
+    void fork() {
+        if (!multi()) return;
+        ++st_[0].seq;
+        mark(0);
+        for (int j = 1; j < used_; ++j) dep(j, 0, st_[0].seq);
+    }
+    // Forget the recorded readers and writers: only valid once the streams
+    // are ordered by a join and a fork.
+    void reset() { res_.clear(); }
+    void barrier() {
+        join();
+        fork();
+        reset();
+    }
+    void synchronize() {
+        for (int j = 0; j < used_ && j < (int)st_.size(); ++j)
+            check_hip(hipStreamSynchronize(st_[j].s), "stream sync");
+    }
+
+  private:
+    static constexpr int kRing = 64;
+    struct Str {
+        hipStream_t s = nullptr;
+        uint64_t seq = 0;
+        std::vector<hipEvent_t> ring;
+        std::vector<uint64_t> ring_seq;   // task whose record a slot holds
+        std::vector<uint64_t> seen;       // per stream: already waited for
+    };
+    struct Entry {
+        int writer = -1;
+        uint64_t wseq = 0;
+        std::vector<uint64_t> readers;
+    };
+    Entry &entry(Res r) {
+        Entry &e = res_[r];
+        if ((int)e.readers.size() < used_) e.readers.resize(used_, 0);
+        return e;
+    }
+    void mark(int i) {
+        Str &s = st_[i];
+        const int slot = (int)(s.seq % kRing);
+        check_hip(hipEventRecord(s.ring[slot], s.s), "hipEventRecord");
+        s.ring_seq[slot] = s.seq;
+    }
+    void dep(int i, int j, uint64_t q) {
+        if (j < 0 || j == i || q == 0) return;
+        Str &s = st_[i];
+        if (s.seen[j] >= q) return;
+        Str &o = st_[j];
+        const int slot = (int)(q % kRing);
+        check_hip(hipStreamWaitEvent(s.s, o.ring[slot], 0),
+                  "hipStreamWaitEvent");
+        s.seen[j] = std::max(s.seen[j], o.ring_seq[slot]);
+    }
+
+    std::vector<Str> st_;
+    int used_ = 1;
+    std::unordered_map<Res, Entry> res_;
+    int cur_ = -1;
+    std::vector<Res> cur_r_, cur_w_;
+};
+
+// A captured run of steps and the host state it was captured from.
+struct StepGraph {
+    std::vector<uint64_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~StepGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+struct Cache : EngineCache {
+    Sched sched;
+    // second word buffer of each game-of-life layer on the fused step
+    std::map<LayerState *, uint32_t *> gol_spare;
+    // last fed input of each input layer, on the device
+    std::map<LayerState *, float *> inter;
+    // step counter that captured steps read their random draws' step from
+    uint64_t *d_step = nullptr;
+    uint64_t *h_step = nullptr;     // pinned staging for it
+    std::vector<std::unique_ptr<StepGraph>> graphs;
+    ~Cache() override {
+        graphs.clear();
+        for (auto &kv : gol_spare) (void)hipFree(kv.second);
+        for (auto &kv : inter) (void)hipFree(kv.second);
+        if (d_step) (void)hipFree(d_step);
+        if (h_step) (void)hipHostFree(h_step);
+    }
+};
+
+Cache &cache_of(State &s) {
+    if (!s.engine_cache) s.engine_cache.reset(new Cache());
+    return static_cast<Cache &>(*s.engine_cache);
+}
+
+// Pinned host staging buffer of one (layer, key), and the event of the
+// last copy that used it.
 struct HostBuf {
     float *p = nullptr;
     int n = 0;
     bool dirty = false;
-    ~HostBuf() { if (p) (void)hipHostFree(p); }
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    ~HostBuf() {
+        if (done) (void)hipEventDestroy(done);
+        if (p) (void)hipHostFree(p);
+    }
     void alloc(int size) {
         n = size;
         check_hip(hipHostMalloc((void **)&p, (size_t)size * 4,
                                 hipHostMallocDefault), "hipHostMalloc");
         std::memset(p, 0, (size_t)size * 4);
+        check_hip(hipEventCreateWithFlags(&done, hipEventDisableTiming),
+                  "hipEventCreate");
     }
+    // the copy that used the buffer last has finished
+    void wait() {
+        if (!pending) return;
+        check_hip(hipEventSynchronize(done), "hipEventSynchronize");
+        pending = false;
+    }
+    void record(hipStream_t s) {
+        check_hip(hipEventRecord(done, s), "hipEventRecord");
+        pending = true;
+    }
+};
+
+// Input staging of one (layer, key): two pinned buffers, so that the host
+// fills one while a copy from the other may still be in flight.
+struct InBuf {
+    HostBuf b[2];
+    int cur = 0;        // the buffer written last
+    bool dirty = false; // written since its last copy to the device
+    void alloc(int size) { b[0].alloc(size); b[1].alloc(size); }
+    // The buffer the host writes next (its previous copy has finished);
+    // it starts as a copy of the last input, so a callback that writes
+    // part of it sees the rest as the single buffer would hold it.
+    float *next(bool clear) {
+        HostBuf &x = b[1 - cur];
+        x.wait();
+        if (clear) std::memset(x.p, 0, (size_t)x.n * 4);
+        else std::memcpy(x.p, b[cur].p, (size_t)x.n * 4);
+        cur = 1 - cur;
+        dirty = true;
+        return x.p;
+    }
+    HostBuf &last() { return b[cur]; }
 };
 
 struct ModuleLayer {
@@ -51,8 +316,9 @@ struct Module {
     int start_delay = 0, cutoff = 0, rate = 1, curr = 0;
 };
 
-// Optional per-kernel timing with HIP events recorded on the engine's
-// stream around each launch ("kernel timing": "true" in the run args).
+// Optional per-kernel timing with HIP events recorded on stream 0 around
+// each launch ("kernel timing": "true" in the run args; every launch then
+// runs on stream 0).
 struct KernelTimer {
     bool on = false;
     hipStream_t stream = nullptr;
@@ -169,74 +435,263 @@ void check_layer_model_supports(LayerState *ls, const std::string &key) {
         throw Error("Failed to retrieve neuron data \"" + key + "\"");
 }
 
+// ---- kernel descriptors (engine/kernel/synapse_data.cpp:9-64) -------------
+
+sg_conn conn_desc(ConnState &cs, bool store_sum) {
+    Connection &c = *cs.conn;
+    sg_conn d;
+    std::memset(&d, 0, sizeof(d));
+    d.type = c.type;
+    d.opcode = c.opcode;
+    d.out_type = cs.from->out_type;
+    d.delay = c.delay;
+    d.convolutional = c.convolutional;
+    d.from_size = c.from->size; d.from_rows = c.from->rows;
+    d.from_cols = c.from->cols;
+    d.to_size = c.to->size; d.to_rows = c.to->rows; d.to_cols = c.to->cols;
+    d.row_field = c.arbor.row_field; d.col_field = c.arbor.col_field;
+    d.row_stride = c.arbor.row_stride; d.col_stride = c.arbor.col_stride;
+    d.row_spacing = c.arbor.row_spacing;
+    d.col_spacing = c.arbor.col_spacing;
+    d.row_offset = c.arbor.row_offset; d.col_offset = c.arbor.col_offset;
+    d.wrap = c.arbor.wrap;
+    d.from_row_start = c.subset.from_row_start;
+    d.from_row_end = c.subset.from_row_end;
+    d.from_col_start = c.subset.from_col_start;
+    d.from_col_end = c.subset.from_col_end;
+    d.to_row_start = c.subset.to_row_start;
+    d.to_row_end = c.subset.to_row_end;
+    d.to_col_start = c.subset.to_col_start;
+    d.to_col_end = c.subset.to_col_end;
+    d.k_per_dest = cs.k_per_dest;
+    d.num_dest = cs.num_dest;
+    d.num_weights = c.num_weights;
+    const int per_word = cs.from->out_type == SG_OUT_BIT ? 32 : 1;
+    d.src = cs.from->output.du() +
+            (size_t)(c.delay / per_word) * c.from->size;
+    d.dst = cs.to->input.df() + (size_t)c.reg * c.to->size;
+    d.weights = cs.arr("weights").df();
+    // Plastic base-activator weights stay finite: the updater clamps
+    // them to [1e-4, max weight].
+    d.skip_silent = d.out_type == SG_OUT_BIT && cs.finite_weights;
+    // Delay-0 reads of an Izhikevich layer use the plane its neuron
+    // kernel wrote last step (packed once at run start); other BIT
+    // sources are packed per activation into the connection's own.
+    if (cs.from->model == Model::IZHIKEVICH && c.delay == 0 &&
+        cs.from->d_plane) {
+        d.plane = cs.from->d_plane;
+        d.plane_ready = 1;
+    } else {
+        d.plane = cs.d_plane;
+    }
+    d.store_sum = store_sum ? 1 : 0;
+    d.sparse_from = cs.d_sparse_from;
+    d.tile_counts = cs.d_tile_counts;
+    // A second-order host activates with its slaves' copy of its
+    // weights (synapse_data.cpp:20-23), which may hold any value.
+    if (c.so_host) {
+        d.weights = cs.arr("second order weights").df();
+        d.skip_silent = 0;
+    }
+    return d;
+}
+
+sg_iz_syn iz_syn(ConnState &cs) {
+    sg_iz_syn s;
+    std::memset(&s, 0, sizeof(s));
+    s.record = cs.packed ? cs.d_record : nullptr;
+    s.weights = cs.arr("weights").df();
+    s.short_traces = cs.arr("short trace").df();
+    s.long_traces = cs.arr("long trace").df();
+    s.presyn_traces = cs.arr("presyn trace").df();
+    s.stp_r = cs.arr("short term R").df();
+    s.stp_w = cs.arr("short term w").df();
+    s.weight_derivatives = cs.arr("weight derivatives").df();
+    s.delays = cs.d_delays;
+    s.tss_cur = cs.d_tss[cs.tss_cur];
+    s.src_stp = cs.compact ? cs.d_src_stp : nullptr;
+    s.baseline_conductance = cs.baseline_conductance;
+    s.learning_rate = cs.learning_rate;
+    s.max_weight = cs.conn->max_weight;
+    s.stp_flag = cs.stp_flag;
+    return s;
+}
+
+sg_iz_dest iz_dest(ConnState &cs, int kind) {
+    sg_iz_dest d;
+    std::memset(&d, 0, sizeof(d));
+    LayerState &t = *cs.to;
+    d.out_word0 = t.output.du();
+    d.post_trace = t.var("post trace").df();
+    d.time_since_spike = t.var("time since spike").di();
+    d.voltage = t.var("voltage").df();
+    d.input = t.input.df();
+    // ACTIV_EXTRACTIONS_SHORT / _LONG taus are floats
+    // (izhikevich_attributes.cpp:219-249).
+    switch (kind) {
+        case SG_IZ_ADD:
+            d.short_cond = t.var("ampa").df(); d.short_tau = (float)0.8;
+            d.long_cond = t.var("nmda").df(); d.long_tau = (float)0.993;
+            break;
+        case SG_IZ_SUB:
+            d.short_cond = t.var("gabaa").df(); d.short_tau = (float)0.833;
+            d.long_cond = t.var("gabab").df(); d.long_tau = (float)0.993;
+            break;
+        case SG_IZ_MULT:
+            d.short_cond = t.var("mult").df(); d.short_tau = (float)0.95;
+            break;
+        case SG_IZ_REWARD:
+            d.short_cond = t.var("dopamine").df(); d.short_tau = (float)0.95;
+            break;
+    }
+    return d;
+}
+
+void launch_update(State &state, ConnState &cs, hipStream_t st) {
+    sg_conn d = conn_desc(cs, false);
+    sg_iz_syn s = iz_syn(cs);
+    sg_iz_dest dd = iz_dest(cs, cs.iz_kind >= 0 ? cs.iz_kind : SG_IZ_ADD);
+    check_hip(sg_hip_iz_update(&d, &s, &dd, cs.update_kind,
+                               state.d_src_table(), state.d_dst_table(), st),
+              "iz update");
+}
+
+bool fusable(const ConnState &cs) {
+    return cs.iz && !cs.direct && cs.iz_kind >= 0 &&
+           cs.iz_kind != SG_IZ_GAP && cs.update_kind >= 0;
+}
+
 }  // namespace
+
+void flush_pending_updates(State &state) {
+    bool any = false;
+    for (auto &csp : state.conns())
+        if (csp->pending_update) {
+            if (!any) check_hip(hipDeviceSynchronize(), "flush sync");
+            any = true;
+            launch_update(state, *csp, nullptr);
+            csp->pending_update = false;
+        }
+    if (any) check_hip(hipDeviceSynchronize(), "flush");
+}
 
 void Engine::interrupt() {
     std::lock_guard<std::mutex> g(g_engine_lock);
     if (g_running) g_interrupt = true;
 }
 
-Props *Engine::run(const Props &args) {
-    {
-        std::lock_guard<std::mutex> g(g_engine_lock);
-        if (g_running) throw Error("Cannot run more than one engine at once!");
-        g_running = true;
-        g_interrupt = false;
+namespace {
+
+// One run of the engine over a state: the reference's single-thread loop
+// (engine/engine.cpp:226-315) with its parallel, sequential and feedforward
+// clusters (engine/cluster/*.cpp).
+class Runner {
+  public:
+    Runner(Network *net, const Props *env, State *state, const Props &args)
+        : net_(net), env_(env), st_(state), args_(args) {}
+    Props *run();
+
+  private:
+    // What one step's random draws and last-step work depend on.
+    struct StepCtx {
+        uint64_t step;            // the draws' step (added to *base if set)
+        const uint64_t *base;     // device step counter of captured steps
+        bool last;                // the run's last step
+    };
+    struct InitFold { int noise = 0, dist = 3, op = SG_ADD;
+                      float a = 0.0f, b = 0.0f, val = 0.0f; };
+    // a layer's activation plan: a connection's activation, a register
+    // fold, a second-order node's weight copy, a slave's fold into it, the
+    // host's activation
+    enum { ACT_CONN, ACT_FOLD, ACT_SO_COPY, ACT_SO_SLAVE, ACT_SO_HOST };
+    struct Act {
+        int kind;
+        ConnState *cs, *host;
+        const Dendrite *child, *parent;
+    };
+
+    Sched &sched() { return C_->sched; }
+    bool is_input(LayerState *ls) const {
+        auto it = io_types_.find(ls);
+        return it != io_types_.end() && (it->second & IO_INPUT);
     }
-    struct Release {
-        ~Release() {
-            std::lock_guard<std::mutex> g(g_engine_lock);
-            g_running = false;
-            g_interrupt = false;
-        }
-    } release;
-
-    // Devices: GPUs are 0..n-1, the host is n (resource_manager.h:53).
-    int ngpu = sg_hip_device_count();
-    if (ngpu <= 0)
-        throw Error("No HIP device available: this library runs the "
-                    "simulation on an MI355X GPU");
-    std::vector<std::string> devs;
-    if (args.has("devices")) devs.push_back(args.get("devices"));
-    for (auto &d : args.get_array("devices")) devs.push_back(d);
-    int device = 0;
-    for (auto &d : devs) {
-        int id = std::stoi(d);
-        if (id >= ngpu)
-            throw Error("Device " + d + " is not a GPU: this library "
-                        "simulates on GPUs only");
-        device = id;
+    bool is_base(const ConnState &cs) const {
+        return (!cs.iz || cs.direct || cs.iz_kind == SG_IZ_GAP) &&
+               !cs.fused_gol;
     }
-    check_hip(hipSetDevice(device), "hipSetDevice");
-
-    // Cluster orders (engine/cluster/*.cpp): parallel structures activate
-    // every layer, then update every layer; sequential and feedforward
-    // structures activate and update layer by layer in dependency order
-    // (sequential_cluster.cpp:100-113), weights backwards.
-    std::vector<std::vector<Layer *>> seq_order(net_->structures().size());
-    std::set<const Layer *> in_seq;
-    for (size_t i = 0; i < net_->structures().size(); ++i) {
-        const Structure &st = *net_->structures()[i];
-        if (st.cluster == CLUSTER_PARALLEL) continue;
-        if (st.cluster == CLUSTER_FEEDFORWARD) check_feedforward(st);
-        seq_order[i] = sequential_order(st);
-        for (auto &l : st.layers) in_seq.insert(l.get());
+    int main_of(const LayerState *ls) const {
+        return main_.at(ls->layer->structure);
     }
+    int side_of(const LayerState *ls) const {
+        return side_.at(ls->layer->structure);
+    }
+    bool split_of(const LayerState *ls) const {
+        return side_of(ls) != main_of(ls);
+    }
+    uint64_t global(long it) const { return step0_ + (uint64_t)it; }
 
-    // "compact records": "false" keeps every per-synapse field in the
-    // Izhikevich records uploaded by this run.
-    state_->to_device(args.get_bool("compact records", true));
+    void setup_modules();
+    void setup_plans();
+    void setup_streams();
 
-    const bool learning = args.get_bool("learning flag", true);
-    const bool suppress = args.get_bool("suppress output", false);
-    const int env_rate = std::max(1, args.get_int("environment rate", 1));
-    long iterations = args.has("iterations") ? args.get_int("iterations", 1)
-                                             : 0;
+    void init_layer(LayerState *ls, bool overwrite_default, const StepCtx &x);
+    void activate(ConnState &cs, int sidx);
+    void activate_layer(LayerState *ls, bool split);
+    void update_neurons(const std::vector<LayerState *> &lss, int sidx,
+                        const StepCtx &x);
 
-    // ---- environment: modules and buffers --------------------------------
-    std::vector<Module> modules;
-    std::map<LayerState *, int> io_types;
-    std::map<LayerState *, std::set<std::string>> in_keys, out_keys;
+    void pre_input(const StepCtx &x);
+    void input_phase();
+    void post_input(const StepCtx &x);
+    void neurons(const StepCtx &x);
+    void updates();
+    bool output_issue(long it);
+    void output_deliver();
+    void cycle_modules();
+
+    std::vector<uint64_t> graph_key(int block) const;
+    StepGraph *step_graph(int block);
+
+    Network *net_;
+    const Props *env_;
+    State *st_;
+    const Props &args_;
+    Cache *C_ = nullptr;
+
+    bool learning_ = true, suppress_ = false;
+    int env_rate_ = 1;
+    long iterations_ = 0;
+    uint64_t step0_ = 0;
+
+    std::vector<std::vector<Layer *>> seq_order_;
+    std::set<const Layer *> in_seq_;
+
+    std::vector<Module> modules_;
+    std::map<LayerState *, int> io_types_;
+    std::map<LayerState *, std::set<std::string>> in_keys_, out_keys_;
+    std::map<std::pair<LayerState *, std::string>, std::unique_ptr<InBuf>>
+        in_bufs_;
+    std::map<std::pair<LayerState *, std::string>, std::unique_ptr<HostBuf>>
+        out_bufs_;
+
+    KernelTimer kt_;
+
+    std::map<LayerState *, InitFold> init_fold_;
+    std::set<ConnState *> store_sum_;
+    std::map<LayerState *, std::vector<ConnState *>> tss_out_;
+    std::set<ConnState *> tss_folded_;
+    std::map<LayerState *, ConnState *> gol_fast_;
+    std::map<LayerState *, uint32_t *> gol_home_;
+    std::map<LayerState *, std::vector<Act>> plan_;
+    bool any_dendrites_ = false, any_second_order_ = false;
+
+    std::map<const Structure *, int> main_, side_;
+    int io_ = 0, nstreams_ = 1;
+    bool any_split_ = false;
+};
+
+// ---- environment: modules and buffers ---------------------------------------
+void Runner::setup_modules() {
     if (env_) {
         for (auto *mc : env_->get_child_array("modules")) {
             if (mc->get_bool("skip", false)) continue;
@@ -259,7 +714,7 @@ Props *Engine::run(const Props &args) {
                     : net_->layer(lc->get("layer"));
                 if (!l) throw Error("Could not find layer " + lc->get("layer"));
                 ModuleLayer ml;
-                ml.ls = state_->layer_state(l);
+                ml.ls = st_->layer_state(l);
                 if (lc->get_bool("input", false)) ml.io |= IO_INPUT;
                 if (lc->get_bool("output", false)) ml.io |= IO_OUTPUT;
                 for (auto &k : lc->get_array("input keys")) {
@@ -287,277 +742,50 @@ Props *Engine::run(const Props &args) {
                 auto &cb = Callbacks::get();
                 ml.fn = cb.find(cb.io, lc->get("function"), "io");
                 ml.id = lc->get_int("id", 0);
-                io_types[ml.ls] |= ml.io;
-                for (auto &k : ml.in_keys) in_keys[ml.ls].insert(k);
-                for (auto &k : ml.out_keys) out_keys[ml.ls].insert(k);
+                io_types_[ml.ls] |= ml.io;
+                for (auto &k : ml.in_keys) in_keys_[ml.ls].insert(k);
+                for (auto &k : ml.out_keys) out_keys_[ml.ls].insert(k);
                 m.layers.push_back(ml);
             }
-            if (!m.layers.empty()) modules.push_back(m);
+            if (!m.layers.empty()) modules_.push_back(m);
         }
     }
-    for (auto &kv : io_types) {
-        if ((kv.second & IO_INPUT) && in_keys[kv.first].empty())
-            in_keys[kv.first].insert("input");
-        if ((kv.second & IO_OUTPUT) && out_keys[kv.first].empty())
-            out_keys[kv.first].insert("output");
+    for (auto &kv : io_types_) {
+        if ((kv.second & IO_INPUT) && in_keys_[kv.first].empty())
+            in_keys_[kv.first].insert("input");
+        if ((kv.second & IO_OUTPUT) && out_keys_[kv.first].empty())
+            out_keys_[kv.first].insert("output");
     }
-    auto is_input = [&](LayerState *ls) {
-        auto it = io_types.find(ls);
-        return it != io_types.end() && (it->second & IO_INPUT);
-    };
-
-    std::map<std::pair<LayerState *, std::string>, std::unique_ptr<HostBuf>>
-        in_bufs, out_bufs;
-    std::map<LayerState *, float *> inter;   // last fed input, on device
-    std::vector<void *> dev_allocs;
-    for (auto &kv : in_keys)
+    for (auto &kv : in_keys_)
         for (auto &k : kv.second) {
-            in_bufs[{kv.first, k}].reset(new HostBuf());
-            in_bufs[{kv.first, k}]->alloc(kv.first->layer->size);
-            if (k == "input") {
+            const int n = kv.first->layer->size;
+            in_bufs_[{kv.first, k}].reset(new InBuf());
+            in_bufs_[{kv.first, k}]->alloc(n);
+            if (k == "input" && !C_->inter.count(kv.first)) {
                 float *d = nullptr;
-                check_hip(hipMalloc((void **)&d,
-                                    (size_t)kv.first->layer->size * 4),
-                          "hipMalloc");
-                check_hip(hipMemset(d, 0, (size_t)kv.first->layer->size * 4),
-                          "hipMemset");
-                inter[kv.first] = d;
-                dev_allocs.push_back(d);
+                check_hip(hipMalloc((void **)&d, (size_t)n * 4), "hipMalloc");
+                C_->inter[kv.first] = d;
             }
         }
-    for (auto &kv : out_keys)
+    for (auto &kv : out_keys_)
         for (auto &k : kv.second) {
-            out_bufs[{kv.first, k}].reset(new HostBuf());
-            out_bufs[{kv.first, k}]->alloc(kv.first->layer->size);
+            out_bufs_[{kv.first, k}].reset(new HostBuf());
+            out_bufs_[{kv.first, k}]->alloc(kv.first->layer->size);
         }
-    struct FreeAll {
-        std::vector<void *> &v;
-        ~FreeAll() { for (auto *p : v) (void)hipFree(p); }
-    } free_all{dev_allocs};
+}
 
-    hipStream_t stream;
-    check_hip(hipStreamCreate(&stream), "hipStreamCreate");
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } sg_{stream};
-    // A second stream for the base activators of non-input layers, which
-    // then run beside the conductance passes (see the main loop).
-    hipStream_t side;
-    check_hip(hipStreamCreate(&side), "hipStreamCreate");
-    StreamGuard sg2_{side};
-    // The fork and join events order two streams of one device: no
-    // system-scope fence (that is for the host and other devices).
-    hipEvent_t ev_fork, ev_join;
-    const unsigned ev_flags = hipEventDisableTiming | hipEventDisableSystemFence;
-    check_hip(hipEventCreateWithFlags(&ev_fork, ev_flags), "hipEventCreate");
-    check_hip(hipEventCreateWithFlags(&ev_join, ev_flags), "hipEventCreate");
-    struct EventGuard {
-        hipEvent_t a, b;
-        ~EventGuard() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
-    } eg_{ev_fork, ev_join};
-    KernelTimer kt;
-    kt.on = args.get_bool("kernel timing", false);
-    kt.stream = stream;
-
-    // ---- work folded into the Izhikevich neuron launch ---------------------
-    // (filled before the main loop, read by the helpers below)
-    struct InitFold { int noise = 0, dist = 3, op = SG_ADD;
-                      float a = 0.0f, b = 0.0f, val = 0.0f; };
-    std::map<LayerState *, InitFold> init_fold;     // init value drawn there
-    std::set<ConnState *> store_sum;                // raw-sum activators
-    std::map<LayerState *, std::vector<ConnState *>> tss_out;
-    std::set<ConnState *> tss_folded;               // stepped there
-
-    // ---- per-connection launch helpers -----------------------------------
-    auto conn_desc = [&](ConnState &cs) {
-        Connection &c = *cs.conn;
-        sg_conn d;
-        std::memset(&d, 0, sizeof(d));
-        d.type = c.type;
-        d.opcode = c.opcode;
-        d.out_type = cs.from->out_type;
-        d.delay = c.delay;
-        d.convolutional = c.convolutional;
-        d.from_size = c.from->size; d.from_rows = c.from->rows;
-        d.from_cols = c.from->cols;
-        d.to_size = c.to->size; d.to_rows = c.to->rows; d.to_cols = c.to->cols;
-        d.row_field = c.arbor.row_field; d.col_field = c.arbor.col_field;
-        d.row_stride = c.arbor.row_stride; d.col_stride = c.arbor.col_stride;
-        d.row_spacing = c.arbor.row_spacing;
-        d.col_spacing = c.arbor.col_spacing;
-        d.row_offset = c.arbor.row_offset; d.col_offset = c.arbor.col_offset;
-        d.wrap = c.arbor.wrap;
-        d.from_row_start = c.subset.from_row_start;
-        d.from_row_end = c.subset.from_row_end;
-        d.from_col_start = c.subset.from_col_start;
-        d.from_col_end = c.subset.from_col_end;
-        d.to_row_start = c.subset.to_row_start;
-        d.to_row_end = c.subset.to_row_end;
-        d.to_col_start = c.subset.to_col_start;
-        d.to_col_end = c.subset.to_col_end;
-        d.k_per_dest = cs.k_per_dest;
-        d.num_dest = cs.num_dest;
-        d.num_weights = c.num_weights;
-        const int per_word = cs.from->out_type == SG_OUT_BIT ? 32 : 1;
-        d.src = cs.from->output.du() +
-                (size_t)(c.delay / per_word) * c.from->size;
-        d.dst = cs.to->input.df() + (size_t)c.reg * c.to->size;
-        d.weights = cs.arr("weights").df();
-        // Plastic base-activator weights stay finite: the updater clamps
-        // them to [1e-4, max weight].
-        d.skip_silent = d.out_type == SG_OUT_BIT && cs.finite_weights;
-        // Delay-0 reads of an Izhikevich layer use the plane its neuron
-        // kernel wrote last step (packed once at run start); other BIT
-        // sources are packed per activation into the connection's own.
-        if (cs.from->model == Model::IZHIKEVICH && c.delay == 0 &&
-            cs.from->d_plane) {
-            d.plane = cs.from->d_plane;
-            d.plane_ready = 1;
-        } else {
-            d.plane = cs.d_plane;
-        }
-        d.store_sum = store_sum.count(&cs) ? 1 : 0;
-        d.sparse_from = cs.d_sparse_from;
-        d.tile_counts = cs.d_tile_counts;
-        // A second-order host activates with its slaves' copy of its
-        // weights (synapse_data.cpp:20-23), which may hold any value.
-        if (c.so_host) {
-            d.weights = cs.arr("second order weights").df();
-            d.skip_silent = 0;
-        }
-        return d;
-    };
-    auto iz_syn = [&](ConnState &cs) {
-        sg_iz_syn s;
-        std::memset(&s, 0, sizeof(s));
-        s.record = cs.packed ? cs.d_record : nullptr;
-        s.weights = cs.arr("weights").df();
-        s.short_traces = cs.arr("short trace").df();
-        s.long_traces = cs.arr("long trace").df();
-        s.presyn_traces = cs.arr("presyn trace").df();
-        s.stp_r = cs.arr("short term R").df();
-        s.stp_w = cs.arr("short term w").df();
-        s.weight_derivatives = cs.arr("weight derivatives").df();
-        s.delays = cs.d_delays;
-        s.tss_cur = cs.d_tss[cs.tss_cur];
-        s.src_stp = cs.compact ? cs.d_src_stp : nullptr;
-        s.baseline_conductance = cs.baseline_conductance;
-        s.learning_rate = cs.learning_rate;
-        s.max_weight = cs.conn->max_weight;
-        s.stp_flag = cs.stp_flag;
-        return s;
-    };
-    auto iz_dest = [&](ConnState &cs, int kind) {
-        sg_iz_dest d;
-        std::memset(&d, 0, sizeof(d));
-        LayerState &t = *cs.to;
-        d.out_word0 = t.output.du();
-        d.post_trace = t.var("post trace").df();
-        d.time_since_spike = t.var("time since spike").di();
-        d.voltage = t.var("voltage").df();
-        d.input = t.input.df();
-        // ACTIV_EXTRACTIONS_SHORT / _LONG taus are floats
-        // (izhikevich_attributes.cpp:219-249).
-        switch (kind) {
-            case SG_IZ_ADD:
-                d.short_cond = t.var("ampa").df(); d.short_tau = (float)0.8;
-                d.long_cond = t.var("nmda").df(); d.long_tau = (float)0.993;
-                break;
-            case SG_IZ_SUB:
-                d.short_cond = t.var("gabaa").df(); d.short_tau = (float)0.833;
-                d.long_cond = t.var("gabab").df(); d.long_tau = (float)0.993;
-                break;
-            case SG_IZ_MULT:
-                d.short_cond = t.var("mult").df(); d.short_tau = (float)0.95;
-                break;
-            case SG_IZ_REWARD:
-                d.short_cond = t.var("dopamine").df(); d.short_tau = (float)0.95;
-                break;
-        }
-        return d;
-    };
-    auto fusable = [&](ConnState &cs) {
-        return cs.iz && !cs.direct && cs.iz_kind >= 0 &&
-               cs.iz_kind != SG_IZ_GAP && cs.update_kind >= 0;
-    };
-    auto activate = [&](ConnState &cs, hipStream_t st) {
-        if (cs.fused_gol) return;
-        sg_conn d = conn_desc(cs);
-        d.beside = st == side;
-        if (!cs.iz || cs.direct) {
-            kt.time("activate_base", [&] {
-                check_hip(sg_hip_activate_base(&d, st), "activate");
-            });
-            return;
-        }
-        sg_iz_dest dd = iz_dest(cs, cs.iz_kind);
-        sg_iz_syn s = iz_syn(cs);
-        if (cs.iz_kind == SG_IZ_GAP) {
-            kt.time("iz_gap", [&] {
-                check_hip(sg_hip_iz_activate(&d, &s, &dd, SG_IZ_GAP, st),
-                          "gap activate");
-            });
-            return;
-        }
-        if (!tss_folded.count(&cs)) kt.time("tss_step", [&] {
-            check_hip(sg_hip_tss_step(cs.d_tss[1 - cs.tss_cur],
-                                      cs.d_tss[cs.tss_cur], cs.d_visits,
-                                      cs.d_rev_delays, d.src,
-                                      cs.conn->from->size, st),
-                      "tss step");
-        });
-        if (cs.pending_update) {
-            kt.time("iz_update_activate", [&] {
-                check_hip(sg_hip_iz_update_activate(
-                              &d, &s, &dd, cs.iz_kind, cs.update_kind,
-                              state_->d_src_table(), state_->d_dst_table(),
-                              st),
-                          "update+activate");
-            });
-            cs.pending_update = false;
-        } else {
-            kt.time("iz_activate", [&] {
-                check_hip(sg_hip_iz_activate(&d, &s, &dd, cs.iz_kind, st),
-                          "iz activate");
-            });
-        }
-        if (!tss_folded.count(&cs)) {
-            cs.tss_cur = 1 - cs.tss_cur;
-            // A compact record's per-source state takes this step's
-            // updater step here (the neuron kernel does it for folded
-            // connections).
-            if (cs.compact && learning && cs.update_kind >= 0)
-                check_hip(sg_hip_iz_source_update(
-                              cs.d_src_stp, cs.d_tss[cs.tss_cur],
-                              cs.conn->from->size, cs.update_kind == 1,
-                              cs.stp_flag, st),
-                          "source update");
-        }
-    };
-    auto update = [&](ConnState &cs) {
-        sg_conn d = conn_desc(cs);
-        sg_iz_syn s = iz_syn(cs);
-        sg_iz_dest dd = iz_dest(cs, cs.iz_kind >= 0 ? cs.iz_kind : SG_IZ_ADD);
-        kt.time("iz_update", [&] {
-            check_hip(sg_hip_iz_update(&d, &s, &dd, cs.update_kind,
-                                       state_->d_src_table(),
-                                       state_->d_dst_table(), stream),
-                      "iz update");
-        });
-    };
-
-    // ---- game-of-life fast path -------------------------------------------
-    // A game-of-life layer whose only input is one 3x3 convolutional
-    // self-connection (stride 1, offsets -1, add, delay 0), with one history
-    // word and no environment I/O, is stepped by sg_hip_gol_step.
-    std::map<LayerState *, ConnState *> gol_fast;
-    std::map<LayerState *, uint32_t *> gol_spare, gol_home;
-    for (auto &lsp : state_->layers()) {
+// ---- plans ------------------------------------------------------------------
+void Runner::setup_plans() {
+    // Game-of-life fast path: a game-of-life layer whose only input is one
+    // 3x3 convolutional self-connection (stride 1, offsets -1, add, delay
+    // 0), with one history word and no environment I/O, is stepped by
+    // sg_hip_gol_step over double-buffered words.
+    for (auto &lsp : st_->layers()) {
         LayerState *ls = lsp.get();
         if (ls->model != Model::GAME_OF_LIFE || ls->history != 1) continue;
-        if (io_types.count(ls) || ls->layer->config.has_child("init config") ||
-            in_seq.count(ls->layer))
+        if (io_types_.count(ls) ||
+            ls->layer->config.has_child("init config") ||
+            in_seq_.count(ls->layer))
             continue;
         if (ls->layer->inputs.size() != 1 || ls->layer->registers > 1)
             continue;
@@ -569,56 +797,18 @@ Props *Engine::run(const Props &args) {
             a.col_stride != 1 || a.row_spacing != 1 || a.col_spacing != 1 ||
             a.row_offset != -1 || a.col_offset != -1)
             continue;
-        ConnState *cs = state_->conn_state(c);
+        ConnState *cs = st_->conn_state(c);
         cs->fused_gol = true;
-        gol_fast[ls] = cs;
-        uint32_t *spare = nullptr;
-        check_hip(hipMalloc((void **)&spare, ls->output.bytes()), "hipMalloc");
-        dev_allocs.push_back(spare);
-        gol_spare[ls] = spare;
-        gol_home[ls] = ls->output.du();
+        gol_fast_[ls] = cs;
+        if (!C_->gol_spare.count(ls)) {
+            uint32_t *spare = nullptr;
+            check_hip(hipMalloc((void **)&spare, ls->output.bytes()),
+                      "hipMalloc");
+            C_->gol_spare[ls] = spare;
+        }
+        gol_home_[ls] = ls->output.du();
     }
-    struct GolReset {
-        std::map<LayerState *, ConnState *> &m;
-        ~GolReset() { for (auto &kv : m) kv.second->fused_gol = false; }
-    } gol_reset{gol_fast};
 
-    // ---- init configs (instruction.cpp:3-30) ------------------------------
-    auto init_layer = [&](LayerState *ls, bool overwrite_default, long step) {
-        Props *ic = ls->layer->config.get_child("init config");
-        const int n = ls->layer->size;
-        float *in = ls->input.df();
-        if (!ic) {
-            if (overwrite_default)
-                check_hip(sg_hip_set_f32(in, 0.0f, n, 1, stream), "clear");
-            return;
-        }
-        const bool ow = overwrite_default;
-        const std::string type = ic->get("type", "normal");
-        const uint64_t seed = ls->layer->id;
-        if (type == "flat") {
-            check_hip(sg_hip_set_f32(in, ic->get_float("value", 1.0f), n, ow,
-                                     stream), "init flat");
-        } else if (type == "uniform") {
-            check_hip(sg_hip_init_random(in, n, 0, ic->get_float("min", 0.0f),
-                                         ic->get_float("max", 1.0f), 0, ow,
-                                         seed, step, stream), "init uniform");
-        } else if (type == "normal") {
-            check_hip(sg_hip_init_random(in, n, 1,
-                                         ic->get_float("mean", 1.0f),
-                                         ic->get_float("std dev", 0.1f), 0, ow,
-                                         seed, step, stream), "init normal");
-        } else if (type == "poisson") {
-            const float rate = (float)(0.001 * ic->get_float("rate", 1.0f));
-            check_hip(sg_hip_init_random(in, n, 2, rate, 0.0f,
-                                         ic->get_float("value", 20.0f), ow,
-                                         seed, step, stream), "init poisson");
-        } else {
-            throw Error("Unrecognized init type: " + type);
-        }
-    };
-
-    // ---- folding plans ----------------------------------------------------
     // Init configs (instruction.cpp:3-30) of non-input Izhikevich layers:
     // the value -- 0 for a plain clear -- is drawn by the neuron kernel when
     // no base activator writes the input (it is then the input), or when
@@ -626,9 +816,9 @@ Props *Engine::run(const Props &args) {
     // the neuron kernel applies the activator's own aggregate to the value
     // and the sum, the reference's operation).  Gap junctions add to the
     // input too, so their layers keep the separate pass.
-    for (auto &lsp : state_->layers()) {
+    for (auto &lsp : st_->layers()) {
         LayerState *ls = lsp.get();
-        if (ls->model != Model::IZHIKEVICH || io_types.count(ls) ||
+        if (ls->model != Model::IZHIKEVICH || io_types_.count(ls) ||
             ls->layer->registers > 1)
             continue;
         InitFold f;
@@ -653,7 +843,7 @@ Props *Engine::run(const Props &args) {
         std::vector<ConnState *> base;
         bool gap = false;
         for (auto *c : ls->layer->inputs) {
-            ConnState *cs = state_->conn_state(c);
+            ConnState *cs = st_->conn_state(c);
             if (!cs->iz || cs->direct) base.push_back(cs);
             else if (cs->iz_kind == SG_IZ_GAP) gap = true;
         }
@@ -663,56 +853,39 @@ Props *Engine::run(const Props &args) {
         } else if (base.size() == 1 && base[0]->conn->type != SG_SUBSET) {
             f.noise = 2;
             f.op = base[0]->conn->opcode;
-            store_sum.insert(base[0]);
+            store_sum_.insert(base[0]);
         } else {
             continue;
         }
-        init_fold[ls] = f;
+        init_fold_[ls] = f;
     }
     // Time since spike of conductance connections from Izhikevich layers is
     // stepped by the source layer's neuron kernel (same rule as
     // sg_hip_tss_step, on the same pre-shift words, after the activation
     // that reads the previous value).
-    for (auto &csp : state_->conns()) {
+    for (auto &csp : st_->conns()) {
         ConnState *cs = csp.get();
         if (!cs->iz || cs->direct || cs->iz_kind < 0 ||
             cs->iz_kind == SG_IZ_GAP || cs->from->model != Model::IZHIKEVICH)
             continue;
         // In a sequential structure the source may update before this
         // connection activates within a step; step it at activation then.
-        if (in_seq.count(cs->from->layer) || in_seq.count(cs->to->layer))
+        if (in_seq_.count(cs->from->layer) || in_seq_.count(cs->to->layer))
             continue;
-        auto &v = tss_out[cs->from];
+        auto &v = tss_out_[cs->from];
         if ((int)v.size() >= SG_MAX_TSS_OUT) continue;
         v.push_back(cs);
-        tss_folded.insert(cs);
+        tss_folded_.insert(cs);
     }
-    for (auto &lsp : state_->layers())
-        if (lsp->d_plane)
-            check_hip(sg_hip_pack_plane(lsp->output.du(), lsp->layer->size, 0,
-                                        lsp->d_plane, stream),
-                      "pack plane");
 
     // Activation order of each layer: its dendritic tree depth first --
     // every internal node's subtree, then that node's register folded into
     // its parent's (cluster_node.cpp:115-212, kernel.cpp:161-187) -- and
     // then the node's own connections; without dendrites, the connections
-    // in creation order.  Internal registers start each run at their
-    // node's init value.
-    // kinds: a connection's activation, a register fold, a second-order
-    // node's weight copy, a slave's fold into it, the host's activation
-    enum { ACT_CONN, ACT_FOLD, ACT_SO_COPY, ACT_SO_SLAVE, ACT_SO_HOST };
-    struct Act {
-        int kind;
-        ConnState *cs, *host;
-        const Dendrite *child, *parent;
-    };
-    std::function<bool(const ConnState &)> is_base_act;
-    std::map<LayerState *, std::vector<Act>> plan;
-    bool any_dendrites = false;
-    for (auto &lsp : state_->layers()) {
+    // in creation order.
+    for (auto &lsp : st_->layers()) {
         LayerState *ls = lsp.get();
-        std::vector<Act> &v = plan[ls];
+        std::vector<Act> &v = plan_[ls];
         std::function<void(const Dendrite *)> dfs = [&](const Dendrite *d) {
             for (const Dendrite *ch : d->children) {
                 dfs(ch);
@@ -723,385 +896,798 @@ Props *Engine::run(const Props &args) {
                 // weights, let every slave fold into the copy, then
                 // activate the host with it
                 if (!d->host) return;
-                ConnState *host = state_->conn_state(d->host);
+                any_second_order_ = true;
+                ConnState *host = st_->conn_state(d->host);
                 v.push_back(Act{ACT_SO_COPY, host, host, nullptr, nullptr});
                 for (Connection *c : d->conns)
-                    v.push_back(Act{ACT_SO_SLAVE, state_->conn_state(c), host,
+                    v.push_back(Act{ACT_SO_SLAVE, st_->conn_state(c), host,
                                     nullptr, nullptr});
                 v.push_back(Act{ACT_SO_HOST, host, host, nullptr, nullptr});
                 return;
             }
             for (Connection *c : d->conns)
-                v.push_back(Act{ACT_CONN, state_->conn_state(c), nullptr,
+                v.push_back(Act{ACT_CONN, st_->conn_state(c), nullptr,
                                 nullptr, nullptr});
         };
         dfs(ls->layer->dendrites[0].get());
+        if (ls->layer->dendrites.size() > 1) any_dendrites_ = true;
+    }
+}
+
+// ---- streams ----------------------------------------------------------------
+// One stream per structure (engine/cluster/cluster.cpp:9-16,
+// parallel_cluster.cpp:27-40 give each layer of a cluster a compute stream
+// and the cluster an IO stream; here a structure is the unit); in a
+// parallel structure whose layers have both kinds of activators, the ones
+// that write a layer's input (base activators, gap junctions) on a second
+// stream of their own, beside the conductance passes (which write
+// conductances and synapse records); one IO stream for the environment's
+// copies.  Kernel timing keeps everything on stream 0.
+void Runner::setup_streams() {
+    const bool one = kt_.on;
+    const bool per_structure = !one &&
+                               args_.get_bool("structure streams", true);
+    const bool concurrent = !one &&
+                            args_.get_bool("concurrent activators", true);
+    int next = 0;
+    const auto &structs = net_->structures();
+    for (size_t si = 0; si < structs.size(); ++si)
+        main_[structs[si].get()] = per_structure ? next++ : 0;
+    if (next == 0) next = 1;
+    for (size_t si = 0; si < structs.size(); ++si) {
+        const Structure *s = structs[si].get();
+        bool any_base = false, any_other = false;
+        if (s->cluster == CLUSTER_PARALLEL)
+            for (auto &l : s->layers) {
+                LayerState *ls = st_->layer_state(l.get());
+                if (is_input(ls) || gol_fast_.count(ls)) continue;
+                for (auto *c : l->inputs) {
+                    const ConnState &cs = *st_->conn_state(c);
+                    if (cs.fused_gol) continue;
+                    (is_base(cs) ? any_base : any_other) = true;
+                }
+            }
+        const bool split = concurrent && any_base && any_other &&
+                           !any_dendrites_;
+        side_[s] = split ? next++ : main_[s];
+        any_split_ = any_split_ || split;
+    }
+    io_ = (!modules_.empty() && !one) ? next++ : 0;
+    nstreams_ = next;
+    sched().ensure(nstreams_);
+    sched().set_used(nstreams_);
+    kt_.stream = sched().stream(0);
+}
+
+// ---- launches ---------------------------------------------------------------
+// init configs (instruction.cpp:3-30)
+void Runner::init_layer(LayerState *ls, bool overwrite_default,
+                        const StepCtx &x) {
+    Props *ic = ls->layer->config.get_child("init config");
+    const int n = ls->layer->size;
+    float *in = ls->input.df();
+    const int m = main_of(ls);
+    const Res rin = res(RES_IN, ls);
+    if (!ic) {
+        if (!overwrite_default) return;
+        hipStream_t s = sched().begin(m, {}, {rin});
+        check_hip(sg_hip_set_f32(in, 0.0f, n, 1, s), "clear");
+        sched().end();
+        return;
+    }
+    const bool ow = overwrite_default;
+    const std::string type = ic->get("type", "normal");
+    const uint64_t seed = ls->layer->id;
+    const Res ctr = res(RES_CTR, C_->d_step);
+    int dist = -1;
+    float a = 0.0f, b = 0.0f, val = 0.0f;
+    if (type == "flat") {
+        hipStream_t s = sched().begin(m, {rin}, {rin});
+        check_hip(sg_hip_set_f32(in, ic->get_float("value", 1.0f), n, ow, s),
+                  "init flat");
+        sched().end();
+        return;
+    } else if (type == "uniform") {
+        dist = 0; a = ic->get_float("min", 0.0f); b = ic->get_float("max", 1.0f);
+    } else if (type == "normal") {
+        dist = 1; a = ic->get_float("mean", 1.0f);
+        b = ic->get_float("std dev", 0.1f);
+    } else if (type == "poisson") {
+        dist = 2; a = (float)(0.001 * ic->get_float("rate", 1.0f));
+        val = ic->get_float("value", 20.0f);
+    } else {
+        throw Error("Unrecognized init type: " + type);
+    }
+    hipStream_t s = sched().begin(m, {rin, ctr}, {rin});
+    check_hip(sg_hip_init_random(in, n, dist, a, b, val, ow, seed, x.step,
+                                 x.base, s), "init random");
+    sched().end();
+}
+
+void Runner::activate(ConnState &cs, int sidx) {
+    if (cs.fused_gol) return;
+    LayerState *from = cs.from, *to = cs.to;
+    sg_conn d = conn_desc(cs, store_sum_.count(&cs) > 0);
+    d.beside = sidx != main_of(to);
+    const Res out_from = res(RES_OUT, from), syn = res(RES_SYN, &cs),
+              tss = res(RES_TSS, &cs), in_to = res(RES_IN, to);
+    if (!cs.iz || cs.direct) {
+        // the connection's own plane scratch counts as its TSS resource
+        hipStream_t s = sched().begin(sidx, {out_from, syn, in_to},
+                                      {in_to, tss});
+        kt_.time("activate_base", [&] {
+            check_hip(sg_hip_activate_base(&d, s), "activate");
+        });
+        sched().end();
+        return;
+    }
+    sg_iz_dest dd = iz_dest(cs, cs.iz_kind);
+    sg_iz_syn sy = iz_syn(cs);
+    if (cs.iz_kind == SG_IZ_GAP) {
+        hipStream_t s = sched().begin(sidx, {res(RES_STATE, from), syn, in_to},
+                                      {in_to});
+        kt_.time("iz_gap", [&] {
+            check_hip(sg_hip_iz_activate(&d, &sy, &dd, SG_IZ_GAP, s),
+                      "gap activate");
+        });
+        sched().end();
+        return;
+    }
+    const bool folded = tss_folded_.count(&cs) > 0;
+    if (!folded) {
+        hipStream_t s = sched().begin(sidx, {out_from, tss}, {tss});
+        kt_.time("tss_step", [&] {
+            check_hip(sg_hip_tss_step(cs.d_tss[1 - cs.tss_cur],
+                                      cs.d_tss[cs.tss_cur], cs.d_visits,
+                                      cs.d_rev_delays, d.src,
+                                      cs.conn->from->size, s),
+                      "tss step");
+        });
+        sched().end();
+    }
+    const Res state_to = res(RES_STATE, to), out_to = res(RES_OUT, to);
+    hipStream_t s = sched().begin(sidx, {out_from, tss, syn, state_to, out_to},
+                                  {syn, state_to});
+    if (cs.pending_update) {
+        kt_.time("iz_update_activate", [&] {
+            check_hip(sg_hip_iz_update_activate(
+                          &d, &sy, &dd, cs.iz_kind, cs.update_kind,
+                          st_->d_src_table(), st_->d_dst_table(), s),
+                      "update+activate");
+        });
+        cs.pending_update = false;
+    } else {
+        kt_.time("iz_activate", [&] {
+            check_hip(sg_hip_iz_activate(&d, &sy, &dd, cs.iz_kind, s),
+                      "iz activate");
+        });
+    }
+    sched().end();
+    if (!folded) {
+        cs.tss_cur = 1 - cs.tss_cur;
+        // A compact record's per-source state takes this step's updater
+        // step here (the neuron kernel does it for folded connections).
+        if (cs.compact && learning_ && cs.update_kind >= 0) {
+            hipStream_t s2 = sched().begin(sidx, {tss}, {tss});
+            check_hip(sg_hip_iz_source_update(
+                          cs.d_src_stp, cs.d_tss[cs.tss_cur],
+                          cs.conn->from->size, cs.update_kind == 1,
+                          cs.stp_flag, s2),
+                      "source update");
+            sched().end();
+        }
+    }
+}
+
+void Runner::activate_layer(LayerState *ls, bool split) {
+    const int n = ls->layer->size;
+    const int m = main_of(ls), sd = side_of(ls);
+    const Res rin = res(RES_IN, ls);
+    for (const Act &a : plan_[ls]) {
+        if (a.kind == ACT_CONN || a.kind == ACT_SO_HOST) {
+            activate(*a.cs, split && is_base(*a.cs) ? sd : m);
+            continue;
+        }
+        if (a.kind == ACT_SO_COPY) {
+            const Res hs = res(RES_SYN, a.host);
+            hipStream_t s = sched().begin(m, {hs}, {hs});
+            const Array &w = a.host->arr("weights");
+            check_hip(hipMemcpyAsync(a.host->arr("second order weights").dev,
+                                     w.dev, w.dev_elems * 4,
+                                     hipMemcpyDeviceToDevice, s),
+                      "second order copy");
+            sched().end();
+            continue;
+        }
+        if (a.kind == ACT_SO_SLAVE) {
+            const sg_conn d = conn_desc(*a.cs, false);
+            const Res hs = res(RES_SYN, a.host);
+            hipStream_t s = sched().begin(
+                m, {res(RES_OUT, a.cs->from), res(RES_SYN, a.cs), hs}, {hs});
+            check_hip(sg_hip_activate_second_order(
+                          &d, a.host->arr("second order weights").df(), s),
+                      "second order");
+            sched().end();
+            continue;
+        }
+        float *in = ls->input.df();
+        hipStream_t s = sched().begin(m, {rin}, {rin});
+        check_hip(sg_hip_dendrite(in + (size_t)a.parent->reg * n,
+                                  in + (size_t)a.child->reg * n, n,
+                                  a.child->opcode, a.child->init, s),
+                  "dendrite");
+        sched().end();
+    }
+}
+
+void Runner::update_neurons(const std::vector<LayerState *> &lss, int sidx,
+                            const StepCtx &x) {
+    std::vector<sg_iz_layer> iz_layers;
+    std::vector<Res> reads, writes;
+    std::vector<ConnState *> stepped;   // time since spike stepped
+    for (LayerState *ls : lss) {
+        const int n = ls->layer->size;
+        const Res rin = res(RES_IN, ls), rout = res(RES_OUT, ls),
+                  rst = res(RES_STATE, ls);
+        switch (ls->model) {
+            case Model::IZHIKEVICH: {
+                sg_iz_layer l;
+                std::memset(&l, 0, sizeof(l));
+                l.size = n; l.history = ls->history;
+                l.input = ls->input.df(); l.output = ls->output.du();
+                l.ampa = ls->var("ampa").df(); l.nmda = ls->var("nmda").df();
+                l.gabaa = ls->var("gabaa").df();
+                l.gabab = ls->var("gabab").df();
+                l.mult = ls->var("mult").df();
+                l.dopamine = ls->var("dopamine").df();
+                l.voltage = ls->var("voltage").df();
+                l.recovery = ls->var("recovery").df();
+                l.post_trace = ls->var("post trace").df();
+                l.time_since_spike = ls->var("time since spike").di();
+                l.a = ls->var("a").df(); l.b = ls->var("b").df();
+                l.c = ls->var("c").df(); l.d = ls->var("d").df();
+                l.plane = ls->d_plane;
+                reads.insert(reads.end(), {rin, rout, rst});
+                writes.insert(writes.end(), {rin, rout, rst});
+                for (ConnState *cs : tss_out_[ls]) {
+                    stepped.push_back(cs);
+                    sg_tss_out &o = l.tss[l.ntss++];
+                    o.next = cs->d_tss[1 - cs->tss_cur];
+                    o.cur = cs->d_tss[cs->tss_cur];
+                    o.visits = cs->d_visits;
+                    o.rev_delays = cs->d_rev_delays;
+                    o.word = cs->conn->delay / 32;
+                    o.stp = (cs->compact && learning_ &&
+                             cs->update_kind >= 0)
+                        ? cs->d_src_stp : nullptr;
+                    o.stp_sub = cs->update_kind == 1;
+                    o.stp_flag = cs->stp_flag;
+                    const Res t = res(RES_TSS, cs);
+                    reads.push_back(t);
+                    writes.push_back(t);
+                }
+                auto fi = init_fold_.find(ls);
+                if (fi != init_fold_.end()) {
+                    const InitFold &f = fi->second;
+                    l.noise = f.noise; l.noise_dist = f.dist;
+                    l.noise_op = f.op;
+                    l.noise_a = f.a; l.noise_b = f.b; l.noise_val = f.val;
+                    l.noise_seed = ls->layer->id;
+                    l.noise_step = x.step;
+                    l.noise_base = x.base;
+                    if (x.base) reads.push_back(res(RES_CTR, C_->d_step));
+                }
+                iz_layers.push_back(l);
+                break;
+            }
+            case Model::GAME_OF_LIFE: {
+                auto gf = gol_fast_.find(ls);
+                if (gf != gol_fast_.end()) {
+                    uint32_t *spare = C_->gol_spare.at(ls);
+                    float k9[9];
+                    std::memcpy(k9, gf->second->arr("weights").f(), 36);
+                    hipStream_t s = sched().begin(
+                        sidx, {rout, rin, res(RES_SYN, gf->second)},
+                        {rout, rin});
+                    kt_.time("gol_step", [&] {
+                        check_hip(sg_hip_gol_step(
+                                      ls->output.du(), spare,
+                                      x.last ? ls->input.df() : nullptr,
+                                      ls->layer->rows, ls->layer->cols,
+                                      gf->second->conn->arbor.wrap, k9,
+                                      ls->survival_min, ls->survival_max,
+                                      ls->birth_min, ls->birth_max, s),
+                                  "gol step");
+                    });
+                    sched().end();
+                    C_->gol_spare[ls] = ls->output.du();
+                    ls->output.dev = spare;
+                } else {
+                    hipStream_t s = sched().begin(sidx, {rin, rout}, {rout});
+                    check_hip(sg_hip_gol_neuron(
+                                  ls->input.df(), ls->output.du(), n,
+                                  ls->history, ls->survival_min,
+                                  ls->survival_max, ls->birth_min,
+                                  ls->birth_max, s),
+                              "gol neuron");
+                    sched().end();
+                }
+                break;
+            }
+            case Model::RATE_ENCODING: {
+                hipStream_t s = sched().begin(sidx, {rin, rout}, {rout});
+                check_hip(sg_hip_rate_neuron(ls->input.df(), ls->output.df(),
+                                             n, ls->history, s),
+                          "rate neuron");
+                sched().end();
+                break;
+            }
+            case Model::RELAY: {
+                hipStream_t s = sched().begin(sidx, {rin, rout}, {rout});
+                check_hip(sg_hip_relay_neuron(ls->input.df(), ls->output.df(),
+                                              n, ls->history, ls->ramp, s),
+                          "relay neuron");
+                sched().end();
+                break;
+            }
+        }
+    }
+    if (!iz_layers.empty()) {
+        hipStream_t s = sched().begin(sidx, reads, writes);
+        kt_.time("iz_neuron", [&] {
+            check_hip(sg_hip_iz_neurons(iz_layers.data(),
+                                        (int)iz_layers.size(), s),
+                      "iz neuron");
+        });
+        sched().end();
+        for (ConnState *cs : stepped) cs->tss_cur = 1 - cs->tss_cur;
+    }
+}
+
+// ---- phases of a step ---------------------------------------------------------
+// 1. layers without environment input: init value (or clear), then every
+// activator; they read outputs of the previous step.
+void Runner::pre_input(const StepCtx &x) {
+    auto &layers = st_->layers();
+    for (auto &lsp : layers) {
+        LayerState *ls = lsp.get();
+        if (is_input(ls) || gol_fast_.count(ls) || in_seq_.count(ls->layer))
+            continue;
+        if (!init_fold_.count(ls)) init_layer(ls, true, x);
+    }
+    for (auto &lsp : layers) {
+        LayerState *ls = lsp.get();
+        if (is_input(ls) || gol_fast_.count(ls) || in_seq_.count(ls->layer))
+            continue;
+        activate_layer(ls, split_of(ls));
+    }
+}
+
+// 2. environment input: callbacks fill pinned buffers, which are copied to
+// the input layers on the IO stream.
+void Runner::input_phase() {
+    for (auto &m : modules_) {
+        if (m.start_delay > 0) continue;
+        for (auto &ml : m.layers) {
+            if (!(ml.io & IO_INPUT)) continue;
+            if (m.cutoff == 0 || m.curr < m.cutoff) {
+                if (m.curr % m.rate != 0) continue;
+                for (auto &k : ml.in_keys) {
+                    float *p = in_bufs_.at({ml.ls, k})->next(m.clear_first);
+                    ml.fn(ml.id, ml.ls->layer->size, p);
+                }
+            } else if (m.curr == m.cutoff) {
+                in_bufs_.at({ml.ls, "input"})->next(true);
+            }
+        }
+    }
+    for (auto &kv : in_keys_) {
+        LayerState *ls = kv.first;
+        const size_t bytes = (size_t)ls->layer->size * 4;
+        for (auto &k : kv.second) {
+            InBuf &b = *in_bufs_.at({ls, k});
+            if (k == "input") {
+                float *inter = C_->inter.at(ls);
+                const Res ri = res(RES_INTER, ls);
+                if (b.dirty) {
+                    hipStream_t s = sched().begin(io_, {}, {ri});
+                    check_hip(hipMemcpyAsync(inter, b.last().p, bytes,
+                                             hipMemcpyHostToDevice, s),
+                              "input H2D");
+                    sched().end();
+                    b.last().record(s);
+                    b.dirty = false;
+                }
+                hipStream_t s = sched().begin(io_, {ri}, {res(RES_IN, ls)});
+                check_hip(hipMemcpyAsync(ls->input.dev, inter, bytes,
+                                         hipMemcpyDeviceToDevice, s),
+                          "input D2D");
+                sched().end();
+            } else {
+                hipStream_t s = sched().begin(io_, {}, {res(RES_STATE, ls)});
+                check_hip(hipMemcpyAsync(ls->var(k).dev, b.last().p, bytes,
+                                         hipMemcpyHostToDevice, s),
+                          "aux input H2D");
+                sched().end();
+                b.last().record(s);
+                b.dirty = false;
+            }
+        }
+    }
+}
+
+// 3. post-input, per structure in order: the input layers of a parallel
+// structure activate; a sequential structure runs its layers one by one
+// (init, activators, neuron update) (parallel_cluster.cpp:112-120,
+// sequential_cluster.cpp:100-108).
+void Runner::post_input(const StepCtx &x) {
+    const auto &structs = net_->structures();
+    for (size_t si = 0; si < structs.size(); ++si) {
+        const Structure &st = *structs[si];
+        if (st.cluster == CLUSTER_PARALLEL) {
+            for (auto &l : st.layers) {
+                LayerState *ls = st_->layer_state(l.get());
+                if (!is_input(ls)) continue;
+                // Only an "input" key makes the layer's input register the
+                // environment's (cluster_node.cpp:20-44); auxiliary input
+                // keys leave it to the usual clear.
+                const auto ik = in_keys_.find(ls);
+                init_layer(ls, ik == in_keys_.end() ||
+                               !ik->second.count("input"), x);
+                activate_layer(ls, false);
+            }
+            continue;
+        }
+        for (Layer *l : seq_order_[si]) {
+            LayerState *ls = st_->layer_state(l);
+            if (is_input(ls)) {
+                const auto ik = in_keys_.find(ls);
+                init_layer(ls, ik == in_keys_.end() ||
+                               !ik->second.count("input"), x);
+            } else if (!init_fold_.count(ls)) {
+                init_layer(ls, true, x);
+            }
+            activate_layer(ls, false);
+            update_neurons({ls}, main_of(ls), x);
+        }
+    }
+}
+
+// 4. neuron updates of the parallel structures: the Izhikevich layers that
+// share a stream in one launch.
+void Runner::neurons(const StepCtx &x) {
+    std::map<int, std::vector<LayerState *>> by_stream;
+    std::vector<int> order;
+    for (auto &lsp : st_->layers()) {
+        LayerState *ls = lsp.get();
+        if (in_seq_.count(ls->layer)) continue;
+        const int m = main_of(ls);
+        if (!by_stream.count(m)) order.push_back(m);
+        by_stream[m].push_back(ls);
+    }
+    for (int m : order) update_neurons(by_stream[m], m, x);
+}
+
+// 5. weight updates; a plastic conductance connection's is deferred into
+// its next activation.
+void Runner::updates() {
+    if (!learning_) return;
+    for (auto &csp : st_->conns()) {
+        ConnState &cs = *csp;
+        if (cs.update_kind < 0) continue;
+        if (fusable(cs)) {
+            cs.pending_update = true;
+            continue;
+        }
+        LayerState *to = cs.to;
+        hipStream_t s = sched().begin(
+            main_of(to), {res(RES_SYN, &cs), res(RES_TSS, &cs),
+                          res(RES_OUT, to), res(RES_STATE, to)},
+            {res(RES_SYN, &cs)});
+        kt_.time("iz_update", [&] { launch_update(*st_, cs, s); });
+        sched().end();
+    }
+}
+
+// 6. output: device -> host copies on the IO stream; the callbacks run once
+// the copies have landed (output_deliver).
+bool Runner::output_issue(long it) {
+    if (!(it % env_rate_ == 0 && !suppress_ && !out_keys_.empty()))
+        return false;
+    for (auto &kv : out_keys_) {
+        LayerState *ls = kv.first;
+        const size_t bytes = (size_t)ls->layer->size * 4;
+        for (auto &k : kv.second) {
+            HostBuf &b = *out_bufs_.at({ls, k});
+            const void *src = k == "output" ? ls->output.dev
+                            : k == "input" ? ls->input.dev
+                            : ls->var(k).dev;
+            const Res r = k == "output" ? res(RES_OUT, ls)
+                        : k == "input" ? res(RES_IN, ls)
+                        : res(RES_STATE, ls);
+            hipStream_t s = sched().begin(io_, {r}, {});
+            check_hip(hipMemcpyAsync(b.p, src, bytes, hipMemcpyDeviceToHost,
+                                     s),
+                      "output D2H");
+            sched().end();
+            b.record(s);
+        }
+    }
+    return true;
+}
+
+void Runner::output_deliver() {
+    for (auto &kv : out_bufs_) kv.second->wait();
+    for (auto &m : modules_) {
+        if (m.start_delay > 0) continue;
+        if (!(m.cutoff == 0 || m.curr < m.cutoff)) continue;
+        if (m.curr % m.rate != 0) continue;
+        for (auto &ml : m.layers) {
+            if (!(ml.io & IO_OUTPUT)) continue;
+            for (auto &k : ml.out_keys)
+                ml.fn(ml.id, ml.ls->layer->size,
+                      out_bufs_.at({ml.ls, k})->p);
+        }
+    }
+}
+
+// modules cycle (io/module.cpp:98-108)
+void Runner::cycle_modules() {
+    for (auto &m : modules_) {
+        if (m.start_delay <= 0) {
+            if (m.cutoff == 0 || m.curr < m.cutoff) ++m.curr;
+            else if (m.cutoff != 0 && m.curr == m.cutoff) ++m.curr;
+        } else {
+            --m.start_delay;
+        }
+    }
+}
+
+// ---- captured steps -------------------------------------------------------------
+// Everything a captured run of steps bakes in: the device buffers (their
+// epoch), the streams, and the host-side state that changes from step to
+// step and returns after an even number of steps (each connection's
+// time-since-spike buffer and deferred update, the game-of-life word
+// buffers).
+std::vector<uint64_t> Runner::graph_key(int block) const {
+    std::vector<uint64_t> k{st_->device_epoch(), (uint64_t)block,
+                            (uint64_t)learning_, (uint64_t)nstreams_};
+    for (auto &kv : main_) k.push_back((uint64_t)kv.second);
+    for (auto &kv : side_) k.push_back((uint64_t)kv.second);
+    for (auto &csp : st_->conns())
+        k.push_back((uint64_t)csp->tss_cur |
+                    ((uint64_t)csp->pending_update << 1) |
+                    ((uint64_t)csp->fused_gol << 2));
+    for (auto &kv : gol_fast_)
+        k.push_back((uint64_t)(uintptr_t)kv.first->output.dev);
+    return k;
+}
+
+// The graph of `block` steps from the current host state: captured once
+// (steps issued as in an eager run, every draw's step read from the device
+// counter, which the graph advances by `block` at its end) and kept with
+// the state.
+StepGraph *Runner::step_graph(int block) {
+    const std::vector<uint64_t> key = graph_key(block);
+    for (auto &g : C_->graphs)
+        if (g->key == key) return g.get();
+    Sched &S = sched();
+    S.barrier();
+    hipStream_t s0 = S.stream(0);
+    std::unique_ptr<StepGraph> g(new StepGraph());
+    check_hip(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal),
+              "hipStreamBeginCapture");
+    try {
+        S.fork();
+        S.reset();
+        for (int j = 0; j < block; ++j) {
+            const StepCtx x{(uint64_t)j, C_->d_step, false};
+            pre_input(x);
+            post_input(x);
+            neurons(x);
+            updates();
+        }
+        S.join();
+        hipStream_t s = S.begin(0, {}, {res(RES_CTR, C_->d_step)});
+        check_hip(sg_hip_add_u64(C_->d_step, (uint64_t)block, s),
+                  "step counter");
+        S.end();
+        check_hip(hipStreamEndCapture(s0, &g->graph), "hipStreamEndCapture");
+    } catch (...) {
+        hipGraph_t partial = nullptr;
+        (void)hipStreamEndCapture(s0, &partial);
+        if (partial) (void)hipGraphDestroy(partial);
+        S.reset();
+        throw;
+    }
+    S.reset();
+    check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0),
+              "hipGraphInstantiate");
+    g->key = key;
+    if (C_->graphs.size() >= 4) C_->graphs.erase(C_->graphs.begin());
+    C_->graphs.push_back(std::move(g));
+    return C_->graphs.back().get();
+}
+
+Props *Runner::run() {
+    // Devices: GPUs are 0..n-1, the host is n (resource_manager.h:53).
+    int ngpu = sg_hip_device_count();
+    if (ngpu <= 0)
+        throw Error("No HIP device available: this library runs the "
+                    "simulation on an MI355X GPU");
+    std::vector<std::string> devs;
+    if (args_.has("devices")) devs.push_back(args_.get("devices"));
+    for (auto &d : args_.get_array("devices")) devs.push_back(d);
+    int device = 0;
+    for (auto &d : devs) {
+        int id = std::stoi(d);
+        if (id >= ngpu)
+            throw Error("Device " + d + " is not a GPU: this library "
+                        "simulates on GPUs only");
+        device = id;
+    }
+    check_hip(hipSetDevice(device), "hipSetDevice");
+
+    // Cluster orders (engine/cluster/*.cpp): parallel structures activate
+    // every layer, then update every layer; sequential and feedforward
+    // structures activate and update layer by layer in dependency order
+    // (sequential_cluster.cpp:100-113), weights backwards.
+    seq_order_.resize(net_->structures().size());
+    for (size_t i = 0; i < net_->structures().size(); ++i) {
+        const Structure &st = *net_->structures()[i];
+        if (st.cluster == CLUSTER_PARALLEL) continue;
+        if (st.cluster == CLUSTER_FEEDFORWARD) check_feedforward(st);
+        seq_order_[i] = sequential_order(st);
+        for (auto &l : st.layers) in_seq_.insert(l.get());
+    }
+
+    // "compact records": "false" keeps every per-synapse field in the
+    // Izhikevich records uploaded by this run.
+    st_->to_device(args_.get_bool("compact records", true));
+    C_ = &cache_of(*st_);
+    if (!C_->d_step) {
+        check_hip(hipMalloc((void **)&C_->d_step, 8), "hipMalloc");
+        check_hip(hipHostMalloc((void **)&C_->h_step, 8,
+                                hipHostMallocDefault), "hipHostMalloc");
+    }
+
+    learning_ = args_.get_bool("learning flag", true);
+    suppress_ = args_.get_bool("suppress output", false);
+    env_rate_ = std::max(1, args_.get_int("environment rate", 1));
+    iterations_ = args_.has("iterations") ? args_.get_int("iterations", 1) : 0;
+    kt_.on = args_.get_bool("kernel timing", false);
+    step0_ = st_->steps_done;
+
+    setup_modules();
+    setup_plans();
+    struct GolReset {
+        std::map<LayerState *, ConnState *> &m;
+        ~GolReset() { for (auto &kv : m) kv.second->fused_gol = false; }
+    } gol_reset{gol_fast_};
+    setup_streams();
+    Sched &S = sched();
+    hipStream_t s0 = S.stream(0);
+
+    // Run start: the engine's streams follow everything queued before on
+    // the state (uploads ran synchronously); planes of the Izhikevich
+    // layers' newest bits; internal dendritic registers at their nodes'
+    // init values; the last fed input cleared.
+    S.barrier();
+    for (auto &lsp : st_->layers()) {
+        LayerState *ls = lsp.get();
+        if (ls->d_plane) {
+            const Res r = res(RES_OUT, ls);
+            hipStream_t s = S.begin(0, {r}, {r});
+            check_hip(sg_hip_pack_plane(ls->output.du(), ls->layer->size, 0,
+                                        ls->d_plane, s), "pack plane");
+            S.end();
+        }
         const int n = ls->layer->size;
         for (size_t i = 1; i < ls->layer->dendrites.size(); ++i) {
-            any_dendrites = true;
             const Dendrite &d = *ls->layer->dendrites[i];
+            const Res r = res(RES_IN, ls);
+            hipStream_t s = S.begin(main_of(ls), {r}, {r});
             check_hip(sg_hip_set_f32(ls->input.df() + (size_t)d.reg * n,
-                                     d.init, n, 1, stream),
-                      "dendrite init");
+                                     d.init, n, 1, s), "dendrite init");
+            S.end();
         }
     }
-    auto activate_layer = [&](LayerState *ls, bool split_base) {
-        const int n = ls->layer->size;
-        for (const Act &a : plan[ls]) {
-            if (a.kind == ACT_CONN || a.kind == ACT_SO_HOST) {
-                activate(*a.cs, split_base && is_base_act(*a.cs) ? side
-                                                                : stream);
-                continue;
-            }
-            if (a.kind == ACT_SO_COPY) {
-                const Array &w = a.host->arr("weights");
-                check_hip(hipMemcpyAsync(a.host->arr("second order weights").dev,
-                                         w.dev, w.dev_elems * 4,
-                                         hipMemcpyDeviceToDevice, stream),
-                          "second order copy");
-                continue;
-            }
-            if (a.kind == ACT_SO_SLAVE) {
-                const sg_conn d = conn_desc(*a.cs);
-                check_hip(sg_hip_activate_second_order(
-                              &d, a.host->arr("second order weights").df(),
-                              stream),
-                          "second order");
-                continue;
-            }
-            float *in = ls->input.df();
-            check_hip(sg_hip_dendrite(in + (size_t)a.parent->reg * n,
-                                      in + (size_t)a.child->reg * n, n,
-                                      a.child->opcode, a.child->init,
-                                      stream),
-                      "dendrite");
-        }
-    };
-
-    // Split the pre-input activators over two streams only when both kinds
-    // are there (and no dendritic tree orders a layer's activators).  The
-    // activators that aggregate into a layer's input -- base activators and
-    // gap junctions -- all go to the second stream, in order.
-    auto is_base = [&](const ConnState &cs) {
-        return (!cs.iz || cs.direct || cs.iz_kind == SG_IZ_GAP) &&
-               !cs.fused_gol;
-    };
-    is_base_act = is_base;
-    bool any_base = false, any_other = false;
-    for (auto &lsp : state_->layers()) {
-        if (is_input(lsp.get()) || gol_fast.count(lsp.get()) ||
-            in_seq.count(lsp->layer))
-            continue;
-        for (auto *c : lsp->layer->inputs) {
-            const ConnState &cs = *state_->conn_state(c);
-            if (cs.fused_gol) continue;
-            (is_base(cs) ? any_base : any_other) = true;
-        }
+    for (auto &kv : in_keys_) {
+        if (!kv.second.count("input")) continue;
+        const Res r = res(RES_INTER, kv.first);
+        hipStream_t s = S.begin(io_, {}, {r});
+        check_hip(sg_hip_set_f32(C_->inter.at(kv.first), 0.0f,
+                                 kv.first->layer->size, 1, s), "clear");
+        S.end();
     }
-    // "concurrent activators": "false" keeps every activator on one stream.
-    const bool split_ok = any_base && any_other && !any_dendrites &&
-                          args.get_bool("concurrent activators", true);
 
-    // ---- main loop --------------------------------------------------------
-    auto &layers = state_->layers();
-    long done = 0;
+    // Runs without environment I/O capture their steady-state steps (from
+    // the second on, all but the last) into HIP graphs of `block` steps.
+    const bool graphs = modules_.empty() && !kt_.on && !any_second_order_ &&
+                        args_.get_bool("step graphs", true) &&
+                        iterations_ > 0;
+    int block = std::max(2, args_.get_int("graph steps", 8));
+    block += block & 1;
+
     const auto t0 = std::chrono::steady_clock::now();
-    bool interrupted = false;
-    for (long it = 0; iterations == 0 || it < iterations; ++it) {
-        // 1. pre-input: init values, then every activator.  Base activators
-        // (latency-bound) go to the second stream and run beside the
-        // conductance passes (HBM-bound): they write the layers' inputs, the
-        // conductance passes write conductances and synapse records, and
-        // both read only last step's spikes; the join below precedes the
-        // neuron launch.  With kernel timing on, one stream.
-        for (auto &lsp : layers) {
-            LayerState *ls = lsp.get();
-            if (is_input(ls) || gol_fast.count(ls) || in_seq.count(ls->layer))
-                continue;
-            if (!init_fold.count(ls)) init_layer(ls, true, it);
-        }
-        const bool split = split_ok && !kt.on;
-        if (split) {
-            check_hip(hipEventRecord(ev_fork, stream), "hipEventRecord");
-            check_hip(hipStreamWaitEvent(side, ev_fork, 0), "wait fork");
-        }
-        for (auto &lsp : layers) {
-            LayerState *ls = lsp.get();
-            if (is_input(ls) || gol_fast.count(ls) || in_seq.count(ls->layer))
-                continue;
-            activate_layer(ls, split);
-        }
-        if (split) {
-            check_hip(hipEventRecord(ev_join, side), "hipEventRecord");
-            check_hip(hipStreamWaitEvent(stream, ev_join, 0), "wait join");
-        }
-        // 2. environment input
-        bool any_input = false;
-        for (auto &m : modules) {
-            if (m.start_delay > 0) continue;
-            for (auto &ml : m.layers) {
-                if (!(ml.io & IO_INPUT)) continue;
-                if (m.cutoff == 0 || m.curr < m.cutoff) {
-                    if (m.curr % m.rate != 0) continue;
-                    for (auto &k : ml.in_keys) {
-                        if (!any_input) {
-                            check_hip(hipStreamSynchronize(stream), "sync");
-                            any_input = true;
-                        }
-                        HostBuf &b = *in_bufs.at({ml.ls, k});
-                        if (m.clear_first) std::memset(b.p, 0, (size_t)b.n * 4);
-                        ml.fn(ml.id, ml.ls->layer->size, b.p);
-                        b.dirty = true;
-                    }
-                } else if (m.curr == m.cutoff) {
-                    if (!any_input) {
-                        check_hip(hipStreamSynchronize(stream), "sync");
-                        any_input = true;
-                    }
-                    HostBuf &b = *in_bufs.at({ml.ls, "input"});
-                    std::memset(b.p, 0, (size_t)b.n * 4);
-                    b.dirty = true;
-                }
+    long it = 0, done = 0, graph_launches = 0;
+    bool interrupted = false, pre_issued = false, counter_set = false;
+    auto ctx = [&](long i) {
+        return StepCtx{global(i), nullptr,
+                       iterations_ != 0 && i + 1 == iterations_};
+    };
+    while (iterations_ == 0 || it < iterations_) {
+        if (graphs && it >= 1 && iterations_ - it > block) {
+            StepGraph *g = step_graph(block);
+            S.barrier();
+            if (!counter_set) {
+                *C_->h_step = global(it);
+                check_hip(hipMemcpyAsync(C_->d_step, C_->h_step, 8,
+                                         hipMemcpyHostToDevice, s0),
+                          "step counter");
+                counter_set = true;
             }
+            while (iterations_ - it > block) {
+                check_hip(hipGraphLaunch(g->exec, s0), "hipGraphLaunch");
+                ++graph_launches;
+                it += block;
+                done += block;
+                if (g_interrupt) break;
+            }
+            S.barrier();
+            check_hip(hipGetLastError(), "step");
+            if (g_interrupt) {
+                interrupted = true;
+                break;
+            }
+            continue;
         }
-        for (auto &kv : in_keys) {
-            LayerState *ls = kv.first;
-            const size_t bytes = (size_t)ls->layer->size * 4;
-            for (auto &k : kv.second) {
-                HostBuf &b = *in_bufs.at({ls, k});
-                if (k == "input") {
-                    if (b.dirty)
-                        check_hip(hipMemcpyAsync(inter[ls], b.p, bytes,
-                                                 hipMemcpyHostToDevice, stream),
-                                  "input H2D");
-                    b.dirty = false;
-                    check_hip(hipMemcpyAsync(ls->input.dev, inter[ls], bytes,
-                                             hipMemcpyDeviceToDevice, stream),
-                              "input D2D");
-                } else {
-                    check_hip(hipMemcpyAsync(ls->var(k).dev, b.p, bytes,
-                                             hipMemcpyHostToDevice, stream),
-                              "aux input H2D");
-                }
-            }
-        }
-        // 3. post-input, per structure in order: the input layers of a
-        // parallel structure activate; a sequential structure runs its
-        // layers one by one (init, activators, neuron update)
-        // (parallel_cluster.cpp:112-120, sequential_cluster.cpp:100-108).
-        const bool last = iterations != 0 && it + 1 == iterations;
-        auto update_neurons = [&](const std::vector<LayerState *> &lss) {
-            std::vector<sg_iz_layer> iz_layers;
-            std::vector<ConnState *> stepped;   // time since spike stepped
-            for (LayerState *ls : lss) {
-                const int n = ls->layer->size;
-                switch (ls->model) {
-                    case Model::IZHIKEVICH: {
-                        sg_iz_layer l;
-                        std::memset(&l, 0, sizeof(l));
-                        l.size = n; l.history = ls->history;
-                        l.input = ls->input.df(); l.output = ls->output.du();
-                        l.ampa = ls->var("ampa").df(); l.nmda = ls->var("nmda").df();
-                        l.gabaa = ls->var("gabaa").df();
-                        l.gabab = ls->var("gabab").df();
-                        l.mult = ls->var("mult").df();
-                        l.dopamine = ls->var("dopamine").df();
-                        l.voltage = ls->var("voltage").df();
-                        l.recovery = ls->var("recovery").df();
-                        l.post_trace = ls->var("post trace").df();
-                        l.time_since_spike = ls->var("time since spike").di();
-                        l.a = ls->var("a").df(); l.b = ls->var("b").df();
-                        l.c = ls->var("c").df(); l.d = ls->var("d").df();
-                        l.plane = ls->d_plane;
-                        for (ConnState *cs : tss_out[ls]) {
-                            stepped.push_back(cs);
-                            sg_tss_out &o = l.tss[l.ntss++];
-                            o.next = cs->d_tss[1 - cs->tss_cur];
-                            o.cur = cs->d_tss[cs->tss_cur];
-                            o.visits = cs->d_visits;
-                            o.rev_delays = cs->d_rev_delays;
-                            o.word = cs->conn->delay / 32;
-                            o.stp = (cs->compact && learning &&
-                                     cs->update_kind >= 0)
-                                ? cs->d_src_stp : nullptr;
-                            o.stp_sub = cs->update_kind == 1;
-                            o.stp_flag = cs->stp_flag;
-                        }
-                        auto fi = init_fold.find(ls);
-                        if (fi != init_fold.end()) {
-                            const InitFold &f = fi->second;
-                            l.noise = f.noise; l.noise_dist = f.dist;
-                            l.noise_op = f.op;
-                            l.noise_a = f.a; l.noise_b = f.b; l.noise_val = f.val;
-                            l.noise_seed = ls->layer->id;
-                            l.noise_step = (uint64_t)it;
-                        }
-                        iz_layers.push_back(l);
-                        break;
-                    }
-                    case Model::GAME_OF_LIFE: {
-                        auto gf = gol_fast.find(ls);
-                        if (gf != gol_fast.end()) {
-                            uint32_t *spare = gol_spare[ls];
-                            float k9[9];
-                            std::memcpy(k9, gf->second->arr("weights").f(), 36);
-                            kt.time("gol_step", [&] {
-                                check_hip(sg_hip_gol_step(
-                                              ls->output.du(), spare,
-                                              last ? ls->input.df() : nullptr,
-                                              ls->layer->rows, ls->layer->cols,
-                                              gf->second->conn->arbor.wrap, k9,
-                                              ls->survival_min, ls->survival_max,
-                                              ls->birth_min, ls->birth_max,
-                                              stream),
-                                          "gol step");
-                            });
-                            gol_spare[ls] = ls->output.du();
-                            ls->output.dev = spare;
-                        } else {
-                            check_hip(sg_hip_gol_neuron(
-                                          ls->input.df(), ls->output.du(), n,
-                                          ls->history, ls->survival_min,
-                                          ls->survival_max, ls->birth_min,
-                                          ls->birth_max, stream),
-                                      "gol neuron");
-                        }
-                        break;
-                    }
-                    case Model::RATE_ENCODING:
-                        check_hip(sg_hip_rate_neuron(ls->input.df(),
-                                                     ls->output.df(), n,
-                                                     ls->history, stream),
-                                  "rate neuron");
-                        break;
-                    case Model::RELAY:
-                        check_hip(sg_hip_relay_neuron(ls->input.df(),
-                                                      ls->output.df(), n,
-                                                      ls->history, ls->ramp,
-                                                      stream),
-                                  "relay neuron");
-                        break;
-                }
-            }
-            if (!iz_layers.empty()) {
-                kt.time("iz_neuron", [&] {
-                    check_hip(sg_hip_iz_neurons(iz_layers.data(),
-                                                (int)iz_layers.size(), stream),
-                              "iz neuron");
-                });
-                for (ConnState *cs : stepped) cs->tss_cur = 1 - cs->tss_cur;
-            }
-        };
-        for (size_t si = 0; si < net_->structures().size(); ++si) {
-            const Structure &st = *net_->structures()[si];
-            if (st.cluster == CLUSTER_PARALLEL) {
-                for (auto &l : st.layers) {
-                    LayerState *ls = state_->layer_state(l.get());
-                    if (!is_input(ls)) continue;
-                    // Only an "input" key makes the layer's input register
-                    // the environment's (cluster_node.cpp:20-44); auxiliary
-                    // input keys leave it to the usual clear.
-                    const auto ik = in_keys.find(ls);
-                    init_layer(ls, ik == in_keys.end() ||
-                                   !ik->second.count("input"), it);
-                    activate_layer(ls, false);
-                }
-                continue;
-            }
-            for (Layer *l : seq_order[si]) {
-                LayerState *ls = state_->layer_state(l);
-                if (is_input(ls)) {
-                    const auto ik = in_keys.find(ls);
-                    init_layer(ls, ik == in_keys.end() ||
-                                   !ik->second.count("input"), it);
-                } else if (!init_fold.count(ls)) {
-                    init_layer(ls, true, it);
-                }
-                activate_layer(ls, false);
-                update_neurons({ls});
-            }
-        }
-        // 4. neuron updates of the parallel structures (every Izhikevich
-        // layer in one launch)
-        {
-            std::vector<LayerState *> par;
-            for (auto &lsp : layers)
-                if (!in_seq.count(lsp->layer)) par.push_back(lsp.get());
-            update_neurons(par);
-        }
-        // 5. weight updates
-        if (learning) {
-            for (auto &csp : state_->conns()) {
-                ConnState &cs = *csp;
-                if (cs.update_kind < 0) continue;
-                if (fusable(cs)) cs.pending_update = true;
-                else update(cs);
-            }
-        }
-        // 6. output
-        if (it % env_rate == 0 && !suppress && !out_keys.empty()) {
-            for (auto &kv : out_keys) {
-                LayerState *ls = kv.first;
-                const size_t bytes = (size_t)ls->layer->size * 4;
-                for (auto &k : kv.second) {
-                    HostBuf &b = *out_bufs.at({ls, k});
-                    const void *src = k == "output" ? ls->output.dev
-                                    : k == "input" ? ls->input.dev
-                                    : ls->var(k).dev;
-                    check_hip(hipMemcpyAsync(b.p, src, bytes,
-                                             hipMemcpyDeviceToHost, stream),
-                              "output D2H");
-                }
-            }
-            check_hip(hipStreamSynchronize(stream), "output sync");
-            for (auto &m : modules) {
-                if (m.start_delay > 0) continue;
-                if (!(m.cutoff == 0 || m.curr < m.cutoff)) continue;
-                if (m.curr % m.rate != 0) continue;
-                for (auto &ml : m.layers) {
-                    if (!(ml.io & IO_OUTPUT)) continue;
-                    for (auto &k : ml.out_keys)
-                        ml.fn(ml.id, ml.ls->layer->size,
-                              out_bufs.at({ml.ls, k})->p);
-                }
-            }
-        }
-        // modules cycle (io/module.cpp:98-108)
-        for (auto &m : modules) {
-            if (m.start_delay <= 0) {
-                if (m.cutoff == 0 || m.curr < m.cutoff) ++m.curr;
-                else if (m.cutoff != 0 && m.curr == m.cutoff) ++m.curr;
-            } else {
-                --m.start_delay;
-            }
-        }
-        check_hip(hipGetLastError(), "step");
+        const StepCtx x = ctx(it);
+        if (!pre_issued) pre_input(x);
+        pre_issued = false;
+        input_phase();
+        post_input(x);
+        neurons(x);
+        updates();
+        const bool out = output_issue(it);
         ++done;
-        if (g_interrupt) {
+        // With environment I/O, the next step's pre-input launches go out
+        // before the host runs this step's output callbacks (and the next
+        // input callbacks), so the GPU works while the host does.
+        const bool more = (iterations_ == 0 || it + 1 < iterations_) &&
+                          !g_interrupt;
+        if (more && !modules_.empty()) {
+            pre_input(ctx(it + 1));
+            pre_issued = true;
+        }
+        if (out) output_deliver();
+        cycle_modules();
+        check_hip(hipGetLastError(), "step");
+        ++it;
+        if (g_interrupt && !pre_issued) {
             interrupted = true;
             break;
         }
     }
-    // Apply updates still deferred into a next activation.
-    for (auto &csp : state_->conns())
-        if (csp->pending_update) {
-            update(*csp);
-            csp->pending_update = false;
-        }
     // Leave double-buffered game-of-life words in the state's own buffer.
-    for (auto &kv : gol_fast) {
+    S.barrier();
+    for (auto &kv : gol_fast_) {
         LayerState *ls = kv.first;
-        if (ls->output.dev != gol_home[ls]) {
-            check_hip(hipMemcpyAsync(gol_home[ls], ls->output.dev,
+        if (ls->output.dev != gol_home_[ls]) {
+            check_hip(hipMemcpyAsync(gol_home_[ls], ls->output.dev,
                                      ls->output.bytes(),
-                                     hipMemcpyDeviceToDevice, stream),
+                                     hipMemcpyDeviceToDevice, s0),
                       "gol restore");
-            ls->output.dev = gol_home[ls];
+            C_->gol_spare[ls] = ls->output.du();
+            ls->output.dev = gol_home_[ls];
         }
     }
-    check_hip(hipStreamSynchronize(stream), "final sync");
+    S.join();
+    check_hip(hipStreamSynchronize(s0), "final sync");
+    S.reset();
+    st_->steps_done += (uint64_t)done;
     Props timing;
-    kt.report(timing);
+    kt_.report(timing);
     const double secs = std::chrono::duration<double>(
         std::chrono::steady_clock::now() - t0).count();
 
@@ -1111,27 +1697,53 @@ Props *Engine::run(const Props &args) {
     r->set("total time", std::to_string((float)secs));
     r->set("average time", std::to_string((float)(secs / (double)done)));
     r->set("fps", std::to_string((float)((double)done / secs)));
-    r->set("network bytes", std::to_string(state_->network_bytes()));
+    r->set("network bytes", std::to_string(st_->network_bytes()));
     size_t buf_bytes = 0;
-    for (auto &kv : in_bufs) buf_bytes += (size_t)kv.second->n * 4;
-    for (auto &kv : out_bufs) buf_bytes += (size_t)kv.second->n * 4;
-    r->set("state buffer bytes", std::to_string(inter.size() ? buf_bytes : 0));
+    for (auto &kv : in_bufs_) buf_bytes += (size_t)kv.second->b[0].n * 4;
+    for (auto &kv : out_bufs_) buf_bytes += (size_t)kv.second->n * 4;
+    r->set("state buffer bytes",
+           std::to_string(C_->inter.empty() ? 0 : buf_bytes));
     r->set("engine buffer bytes", std::to_string(buf_bytes));
-    r->set_child("args", args);
+    r->set_child("args", args_);
     r->set("interrupted", interrupted ? "true" : "false");
     // 2: the input-writing activators ran on a second stream beside the
-    // conductance passes; 1: every activator on one stream.
-    r->set("activator streams", split_ok && !kt.on ? "2" : "1");
+    // conductance passes; 1: every activator on its structure's stream.
+    r->set("activator streams", any_split_ ? "2" : "1");
+    // streams used in all (structures, activator streams, IO)
+    r->set("streams", std::to_string(nstreams_));
+    // captured graphs launched, and the steps they ran
+    r->set("step graphs", std::to_string(graph_launches));
+    r->set("graph steps", std::to_string(graph_launches * block));
     // Izhikevich connections whose records are compact (sg_hip.h SG_IZC_*).
     int compact = 0;
-    for (auto &csp : state_->conns()) compact += csp->compact ? 1 : 0;
+    for (auto &csp : st_->conns()) compact += csp->compact ? 1 : 0;
     r->set("compact records", std::to_string(compact));
     Props mem;
     mem.set("device", std::to_string(device));
-    mem.set("bytes", std::to_string(state_->network_bytes()));
+    mem.set("bytes", std::to_string(st_->network_bytes()));
     r->add_to_child_array("memory usage", mem);
-    if (kt.on) r->set_child("kernel timing", timing);
+    if (kt_.on) r->set_child("kernel timing", timing);
     return r;
+}
+
+}  // namespace
+
+Props *Engine::run(const Props &args) {
+    {
+        std::lock_guard<std::mutex> g(g_engine_lock);
+        if (g_running) throw Error("Cannot run more than one engine at once!");
+        g_running = true;
+        g_interrupt = false;
+    }
+    struct Release {
+        ~Release() {
+            std::lock_guard<std::mutex> g(g_engine_lock);
+            g_running = false;
+            g_interrupt = false;
+        }
+    } release;
+    Runner runner(net_, env_, state_, args);
+    return runner.run();
 }
 
 }  // namespace sg

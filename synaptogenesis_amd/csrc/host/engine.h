@@ -13,8 +13,17 @@
 //
 // The updater of a plastic Izhikevich conductance connection is deferred
 // into the next step's activation pass of the same connection (one fused
-// read/write of its per-synapse arrays); a run ends by applying any update
-// still deferred, so the state a caller sees is the reference's.
+// read/write of its per-synapse arrays), also across runs: the next run
+// starts with the fused pass, and any read of the state from the host
+// (getters, save) first applies what is still deferred, so the state a
+// caller sees is the reference's.
+//
+// Streams: each structure's work runs on a stream of its own (and the
+// activators that write a layer's input on a second stream of it); a
+// scheduler orders launches on different streams by the arrays they read
+// and write, with events only where one stream's launch depends on
+// another's.  Runs without per-step host I/O capture their steady-state
+// steps into HIP graphs (kept with the state for later runs).
 #ifndef SG_ENGINE_H
 #define SG_ENGINE_H
 
@@ -42,6 +51,12 @@ class Engine {
     const Props *env_;
     State *state_;
 };
+
+// Applies every update still deferred into a next activation (a plastic
+// conductance connection's update of the last step run), so that the
+// state's arrays are the reference's after that step.  State::to_host
+// calls it before any host read.
+void flush_pending_updates(State &state);
 
 }  // namespace sg
 

@@ -1,4 +1,5 @@
 #include "state.h"
+#include "engine.h"
 
 #include <hip/hip_runtime.h>
 
@@ -105,6 +106,8 @@ State::State(Network *net) : net_(net) {
 }
 
 State::~State() {
+    // engine streams and graphs go first: they may reference the buffers
+    engine_cache.reset();
     auto free_dev = [](void *p) { if (p) (void)hipFree(p); };
     for (auto &ls : layers_) {
         free_dev(ls->input.dev);
@@ -1062,6 +1065,7 @@ static std::vector<float> compact_dw(const ConnState &cs) {
 void State::to_device(bool allow_compact) {
     ensure_device();
     if (!on_host_) return;
+    ++device_epoch_;
     for (auto &ls : layers_) {
         upload(ls->input);
         upload(ls->output);
@@ -1173,6 +1177,7 @@ void State::download_compact(ConnState &cs) {
 
 void State::to_host() {
     if (on_host_) return;
+    flush_pending_updates(*this);
     check_hip(hipDeviceSynchronize(), "download sync");
     for (auto &ls : layers_) {
         download(ls->input);

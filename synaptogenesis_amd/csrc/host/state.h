@@ -101,6 +101,13 @@ struct ConnState {
     bool has(const std::string &k) const { return arrays.count(k) > 0; }
 };
 
+// Engine resources kept from one run to the next (streams, events, spare
+// buffers, captured step graphs): owned by the state they were made for,
+// built and used by the engine.
+struct EngineCache {
+    virtual ~EngineCache() {}
+};
+
 // Warnings of state construction (the reference's LOG_WARNING), shown
 // unless set_warnings(false) or set_suppress_output(true) turned them off.
 void set_host_messages(bool warnings, bool suppress);
@@ -116,6 +123,8 @@ class State {
     Array *neuron_data(Layer *layer, const std::string &key);
     Array *weight_matrix(Connection *conn, const std::string &key);
 
+    // Applies deferred updates (see pending_update), then moves every array
+    // to the host.
     void to_host();
     void download_compact(ConnState &cs);
     // allow_compact: tiled Izhikevich conductance connections whose
@@ -139,6 +148,15 @@ class State {
     const float *d_src_table() const { return d_tables_; }
     const float *d_dst_table() const { return d_tables_ + 32; }
     void ensure_device();
+    // Incremented whenever device buffers are (re)filled from the host:
+    // what the engine captured before is stale then.
+    uint64_t device_epoch() const { return device_epoch_; }
+
+    // Steps simulated on this state so far, over all runs: the step index
+    // that keys random init values.
+    uint64_t steps_done = 0;
+    // Engine resources of this state (see EngineCache).
+    std::unique_ptr<EngineCache> engine_cache;
 
   private:
     void build_layer(Layer *l);
@@ -159,6 +177,7 @@ class State {
     std::vector<std::unique_ptr<ConnState>> conns_;
     bool on_host_ = true;
     bool device_ready_ = false;
+    uint64_t device_epoch_ = 0;
     void *scratch_ = nullptr;
     size_t scratch_bytes_ = 0;
     float *d_tables_ = nullptr;

@@ -94,3 +94,44 @@ def _diff(a, b):
 
 def meta(fixture):
     return json.loads(str(fixture["meta"]))
+
+
+def compare_long(case, captures, finals, fixture):
+    """Compare a long run (per-step records of harness.long_record) with a
+    fixture of tests/golden/make_long_golden.py.
+
+    Returns (problems, stats): stats maps each captured key to the number of
+    steps whose array differs, the first such step, and the spike counts
+    of both sides summed over the run."""
+    problems = []
+    stats = {}
+    for k in case.capture:
+        key = "cap|" + "|".join(k)
+        want_d = fixture[key + "|d64"]
+        want_s = fixture[key + "|spikes"]
+        got = captures[k]
+        if len(got) != len(want_d):
+            problems.append("%s: %d steps, fixture has %d" %
+                            (key, len(got), len(want_d)))
+            continue
+        got_d = np.array([r[0] for r in got], np.uint64)
+        got_s = np.array([r[1] for r in got], np.int64)
+        bad = np.flatnonzero(got_d != want_d)
+        stats[key] = {"differing_steps": int(bad.size),
+                      "first_differing_step": int(bad[0]) if bad.size
+                      else None,
+                      "spikes": int(got_s.sum()),
+                      "reference_spikes": int(want_s.sum()),
+                      "spike_count_abs_diff": int(np.abs(
+                          got_s - want_s.astype(np.int64)).sum())}
+        if bad.size:
+            problems.append("%s: %d of %d steps differ, first at step %d"
+                            % (key, bad.size, len(got_d), bad[0]))
+    for k, arr in finals.items():
+        key = "fin|" + "|".join(k)
+        if harness.digest(arr).encode() != fixture[key + "|digest"][0]:
+            ref = fixture[key + "|data"]
+            got = arr[fixture[key + "|index"]] if key + "|index" in \
+                fixture.files else arr
+            problems.append("%s differs%s" % (key, _diff(got, ref)))
+    return problems, stats

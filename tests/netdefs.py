@@ -9,7 +9,7 @@ the same functions with larger arguments.
 import numpy as np
 
 from synaptogenesis_amd.inputs import Stream
-from harness import Case
+from harness import Case, long_record
 
 
 def S(x):
@@ -721,3 +721,31 @@ def all_cases():
             iz_ops(), rate(), iz_init(), history(), gol_rule(),
             relay_ramp()] + ragged_cases() + cluster_cases() + sparse_cases() + \
         [dendrites(), second_order()]
+
+
+# -- the benchmark configurations at their stated size and length ------------
+# Their fixtures keep one record per step (harness.long_record: a 64-bit
+# digest of each captured array and the layer's spike count), written by
+# tests/golden/make_long_golden.py from the reference's serial build.
+
+def sheet_full(steps=2000, name="sheet_full_2000"):
+    """The E/I sheet at its stated size: E 256x320 regular, I 128x160 fast,
+    noise N(0, 3) into both every step, seeded as workload 3; per step the
+    spike words and voltages of both layers."""
+    case = iz_sheet(256, 320, steps, name, noise=(0.0, 3.0), workload=3)
+    case.capture = [("sheet", l, k) for l in ("E", "I")
+                    for k in ("output", "voltage")]
+    case.reduce = long_record
+    return case
+
+
+def gol_full(steps=1000, name="gol_1024_1000"):
+    """Game of life on a 1024x1024 torus, cells alive with probability 0.5
+    (seeded as workload 2), every step's history words."""
+    case = gol(name, 1024, 1024, steps, 2, conway_init=True)
+    case.reduce = long_record
+    return case
+
+
+def long_cases():
+    return [gol_full(), sheet_full()]

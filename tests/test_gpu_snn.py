@@ -226,8 +226,49 @@ def test_rejects_bad_configurations():
     with pytest.raises(RuntimeError):
         adj.workspace_bytes(cfg)
     x = torch.zeros(10, 2, 8, dtype=torch.bfloat16)
-    with pytest.raises(RuntimeError):
+    with pytest.raises(ValueError):
         adj.snn_loss(x, torch.zeros(16, 8), torch.zeros(4, 16),
                      torch.zeros(2, dtype=torch.int32),
                      adj.Config.make(batch=2, steps=10, n_in=8, n_hid=16,
                                      n_out=4, checkpoint=5))
+
+
+@pytest.mark.gpu
+def test_rejects_mismatched_arguments_before_launch():
+    """Shapes, dtypes and labels that do not match the configuration raise
+    ValueError before any kernel runs (the kernels index by the
+    configuration and would read past a smaller buffer)."""
+    import torch
+    print("case snn argument checks", flush=True)
+    cfg = adj.Config.make(batch=4, steps=10, n_in=8, n_hid=16, n_out=5,
+                          checkpoint=5)
+    dev = "cuda"
+    x = torch.zeros(10, 4, 8, dtype=torch.bfloat16, device=dev)
+    w1 = torch.zeros(16, 8, device=dev)
+    w2 = torch.zeros(5, 16, device=dev)
+    labels = torch.zeros(4, dtype=torch.int32, device=dev)
+    bad = [
+        (x[:, :3], w1, w2, labels),                       # batch of x
+        (x, torch.zeros(16, 7, device=dev), w2, labels),  # w1 fan-in
+        (x, w1, torch.zeros(4, 16, device=dev), labels),  # w2 rows
+        (x, w1, torch.zeros(5, 15, device=dev), labels),  # w2 fan-in
+        (x, w1, w2, labels[:3]),                          # label count
+        (x, w1, w2, labels.long()),                       # label dtype
+        (x.float(), w1, w2, labels),                      # x dtype
+        (x, w1, w2, torch.tensor([0, 1, 5, 2], dtype=torch.int32,
+                                 device=dev)),            # label >= n_out
+        (x, w1, w2, torch.tensor([0, -1, 3, 2], dtype=torch.int32,
+                                 device=dev)),            # label < 0
+        (x, w1.cpu(), w2, labels),                        # device
+    ]
+    for args in bad:
+        with pytest.raises(ValueError):
+            adj.snn_loss(*args, cfg)
+    # a labels tensor that was valid and is then changed in place is
+    # checked again
+    ok = torch.tensor([0, 1, 2, 3], dtype=torch.int32, device=dev)
+    loss, _ = adj.snn_loss(x, w1, w2, ok, cfg)
+    assert torch.isfinite(loss)
+    ok[2] = 9
+    with pytest.raises(ValueError):
+        adj.snn_loss(x, w1, w2, ok, cfg)

@@ -110,3 +110,143 @@ def test_gap_and_base_stay_in_order(gpu_lib):
     print("two-stream gap + base", flush=True)
     state = _compare(gpu_lib, _gap_net(), "gapnet", ("A", "B"), ("AB",))
     assert np.unpackbits(state[("B", "output")].view(np.uint8)).sum() > 0
+
+
+def _state_after(lib, wl, structure, layers, plastic, parts, extra=None):
+    """Every neuron and plastic synapse array after runs of `parts` steps
+    each, and the reports."""
+    net = bench.build(lib, wl)
+    try:
+        reports = []
+        for n in parts:
+            _, rep = bench.run(net, n, extra)
+            reports.append(rep)
+        state = {}
+        for layer in layers:
+            for k in NEURON_KEYS:
+                state[(layer, k)] = _raw(net.get_neuron_data(structure,
+                                                             layer, k))
+        for c in plastic:
+            for k in SYN_KEYS:
+                state[(c, k)] = _raw(net.get_weight_matrix(c, k))
+        return state, reports
+    finally:
+        net.free()
+
+
+def _same(a, b):
+    bad = [k for k in a if not np.array_equal(a[k], b[k])]
+    assert not bad, "differ: %s" % bad
+
+
+@pytest.mark.gpu
+def test_step_graphs_match_eager_steps(gpu_lib):
+    """Steady-state steps captured into HIP graphs (of 2, 8 and 16 steps,
+    the step counter of the random draws read on the device) leave every
+    array as the eager steps do."""
+    print("step graphs vs eager", flush=True)
+    wl = workloads.sheet(32, 40)
+    eager, reps = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
+                               [STEPS], {"step graphs": "false"})
+    assert reps[0].get("step graphs") == "0"
+    for block in ("2", "8", "16"):
+        got, reps = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
+                                 [STEPS], {"graph steps": block})
+        assert int(reps[0].get("step graphs")) > 0, block
+        _same(eager, got)
+    assert np.unpackbits(eager[("E", "output")].view(np.uint8)).sum() > 0
+
+
+@pytest.mark.gpu
+def test_runs_in_parts_match_one_run(gpu_lib):
+    """A deferred update crosses from one run into the next, and the
+    random draws continue with the state's step count: 7 + 13 + 20 steps
+    leave the arrays of one 40-step run, graphs or not."""
+    print("runs in parts", flush=True)
+    wl = workloads.sheet(32, 40)
+    whole, _ = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
+                            [STEPS])
+    parts, reps = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
+                               [7, 13, 20])
+    _same(whole, parts)
+    eager, _ = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
+                            [7, 13, 20], {"step graphs": "false"})
+    _same(whole, eager)
+
+
+def _two_structures():
+    """Structure s1 (A, B) and structure s2 (C), with connections both
+    ways across them (conductance with STDP, direct), so each structure's
+    stream waits for the other's spikes of the last step and its neuron
+    launch for the other's activations."""
+    noise = {"type": "normal", "mean": "1", "std dev": "4"}
+    lay = lambda n, p: workloads._layer(n, "izhikevich", 10, 12, params=p,
+                                        **{"init config": noise})
+    conn = lambda name, a, b, **kw: dict(
+        {"name": name, "from layer": a, "to layer": b, "type": "convergent",
+         "opcode": "add", "plastic": "false", "delay": "0",
+         "arborized config": {"field size": "5", "wrap": "true"}}, **kw)
+    conns = [
+        conn("AB", "A", "B", plastic="true", **{"learning rate": "0.05"}),
+        conn("BA", "B", "A", direct="true", myelinated="true"),
+        dict(conn("AC", "A", "C", plastic="true",
+                  **{"learning rate": "0.05"}),
+             **{"from structure": "s1", "to structure": "s2"}),
+        dict(conn("CA", "C", "A", direct="true", myelinated="true",
+                  opcode="sub"),
+             **{"from structure": "s2", "to structure": "s1"}),
+        dict(conn("CB", "C", "B", direct="true", myelinated="true"),
+             **{"from structure": "s2", "to structure": "s1"}),
+    ]
+    st = Stream(78)
+    n = 120
+    weights = {c["name"]: (lambda i=i: st.uniform(n * 25, stream=10 + i,
+                                                  low=0.0, high=0.6))
+               for i, c in enumerate(conns)}
+    delays = {k: (lambda i=i: st.integers(n * 25, 0, 31, stream=20 + i))
+              for i, k in enumerate(("AB", "AC"))}
+    init = {}
+    for s, l in (("s1", "A"), ("s1", "B"), ("s2", "C")):
+        init[(s, l, "voltage")] = np.full(n, -65.0, np.float32)
+        init[(s, l, "recovery")] = np.full(
+            n, np.float32(0.2) * np.float32(-65.0), np.float32)
+    net = {"structures": [
+        {"name": "s1", "type": "parallel",
+         "layers": [lay("A", "regular"), lay("B", "fast")]},
+        {"name": "s2", "type": "parallel", "layers": [lay("C", "regular")]}],
+        "connections": conns}
+    return {"net": net, "weights": weights, "delays": delays, "init": init}
+
+
+def _two_state(lib, extra):
+    wl = _two_structures()
+    net = bench.build(lib, wl)
+    try:
+        _, rep = bench.run(net, STEPS, extra)
+        state = {}
+        for s, l in (("s1", "A"), ("s1", "B"), ("s2", "C")):
+            for k in NEURON_KEYS:
+                state[(l, k)] = _raw(net.get_neuron_data(s, l, k))
+        for c in ("AB", "AC"):
+            for k in SYN_KEYS:
+                state[(c, k)] = _raw(net.get_weight_matrix(c, k))
+        return state, rep
+    finally:
+        net.free()
+
+
+@pytest.mark.gpu
+def test_structure_streams_match_one_stream(gpu_lib):
+    """One stream per structure, ordered by the scheduler's events, leaves
+    every array as everything on one stream does, with and without step
+    graphs."""
+    print("structure streams", flush=True)
+    one, rep1 = _two_state(gpu_lib, {"structure streams": "false",
+                                     "concurrent activators": "false",
+                                     "step graphs": "false"})
+    assert rep1.get("streams") == "1"
+    for graphs in ("false", "true"):
+        many, rep = _two_state(gpu_lib, {"step graphs": graphs})
+        assert int(rep.get("streams")) >= 3
+        _same(one, many)
+    assert np.unpackbits(one[("C", "output")].view(np.uint8)).sum() > 0
