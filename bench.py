@@ -631,6 +631,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="sheet",
                     choices=sorted(_workloads()) + ["ensemble", "adjoint"])
+    ap.add_argument("--run-args", default="{}",
+                    help="extra engine run arguments as JSON, e.g. "
+                         "'{\"step graphs\": \"false\"}' (A/B runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference "
                          "for hardware byte counters) and exit")
@@ -660,9 +663,10 @@ def main():
     lib.set_suppress_output(True)
     lib.set_warnings(False)
     network = build(lib, wl)
-    run(network, a.warmup)
-    dt, report = run(network, a.steps)
-    _, kreport = run(network, a.steps, {"kernel timing": "true"})
+    extra = json.loads(a.run_args)
+    run(network, a.warmup, extra)
+    dt, report = run(network, a.steps, extra)
+    _, kreport = run(network, a.steps, dict(extra, **{"kernel timing": "true"}))
     timing = kreport.children["kernel timing"].properties
     network.free()
 
@@ -710,9 +714,10 @@ def main():
                                  traffic, tpath),
         "kernel_avg_ms": kernels,
         "report": {k: report.get(k) for k in ("activator streams",
-                                               "compact records",
-                                               "step graphs")
+                                               "compact records", "streams",
+                                               "step graphs", "graph steps")
                    if report.get(k) is not None},
+        "run_args": extra or None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

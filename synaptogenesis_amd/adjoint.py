@@ -43,14 +43,42 @@ class Config(ctypes.Structure):
 _lib = None
 
 
+def _foreign_hip_runtime():
+    """Path of a HIP runtime mapped into this process from outside
+    PyTorch's own library directory, or None."""
+    try:
+        import torch
+        torch_lib = os.path.realpath(os.path.join(
+            os.path.dirname(torch.__file__), "lib"))
+        with open("/proc/self/maps") as f:
+            for line in f:
+                path = line.split()[-1] if "/" in line else ""
+                if "libamdhip64" in path and \
+                        not os.path.realpath(path).startswith(torch_lib):
+                    return path
+    except (OSError, ImportError):
+        pass
+    return None
+
+
 def library():
     """libsg_hip.so with the adjoint entry points typed.  PyTorch is
     imported first: its bundled HIP runtime has the same soname as the one
-    the library links, and only the first one loaded serves the process."""
+    the library links, and only the first one loaded serves the process.
+    When another runtime was loaded before PyTorch (the library opened
+    before `import torch`), PyTorch sees no GPU; that raises here with the
+    remedy instead of failing later."""
     global _lib
     if _lib is not None:
         return _lib
-    import torch  # noqa: F401
+    import torch
+    if not torch.cuda.is_available():
+        other = _foreign_hip_runtime()
+        if other:
+            raise RuntimeError(
+                "PyTorch finds no GPU because the HIP runtime %s was loaded "
+                "before it; import torch before loading libsg_hip.so or "
+                "libsyngen.so in this process" % other)
     L = ctypes.CDLL(LIBRARY)
     P = ctypes.c_void_p
     cp = ctypes.POINTER(Config)

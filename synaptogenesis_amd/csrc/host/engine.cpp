@@ -651,6 +651,7 @@ class Runner {
 
     std::vector<uint64_t> graph_key(int block) const;
     StepGraph *step_graph(int block);
+    StepGraph *cached_graph(int block) const;
 
     Network *net_;
     const Props *env_;
@@ -1467,10 +1468,16 @@ std::vector<uint64_t> Runner::graph_key(int block) const {
 // (steps issued as in an eager run, every draw's step read from the device
 // counter, which the graph advances by `block` at its end) and kept with
 // the state.
-StepGraph *Runner::step_graph(int block) {
+StepGraph *Runner::cached_graph(int block) const {
     const std::vector<uint64_t> key = graph_key(block);
     for (auto &g : C_->graphs)
         if (g->key == key) return g.get();
+    return nullptr;
+}
+
+StepGraph *Runner::step_graph(int block) {
+    if (StepGraph *g = cached_graph(block)) return g;
+    const std::vector<uint64_t> key = graph_key(block);
     Sched &S = sched();
     S.barrier();
     hipStream_t s0 = S.stream(0);
@@ -1602,8 +1609,13 @@ Props *Runner::run() {
     }
 
     // Runs without environment I/O capture their steady-state steps (from
-    // the second on, all but the last) into HIP graphs of `block` steps.
+    // the second on, all but the last) into HIP graphs of `block` steps --
+    // when at most two streams take part (one structure and its activator
+    // stream): a capture over three streams (two structures) ended the
+    // process with a segmentation fault on the GPU host once, cause not
+    // found, so such runs stay eager.
     const bool graphs = modules_.empty() && !kt_.on && !any_second_order_ &&
+                        nstreams_ <= 2 &&
                         args_.get_bool("step graphs", true) &&
                         iterations_ > 0;
     int block = std::max(2, args_.get_int("graph steps", 8));
@@ -1612,12 +1624,20 @@ Props *Runner::run() {
     const auto t0 = std::chrono::steady_clock::now();
     long it = 0, done = 0, graph_launches = 0;
     bool interrupted = false, pre_issued = false, counter_set = false;
+    bool aligned = false;
     auto ctx = [&](long i) {
         return StepCtx{global(i), nullptr,
                        iterations_ != 0 && i + 1 == iterations_};
     };
     while (iterations_ == 0 || it < iterations_) {
-        if (graphs && it >= 1 && iterations_ - it > block) {
+        // A graph kept from an earlier run may have been captured one step
+        // off (the host state alternates from step to step): one more
+        // eager step aligns the run with it instead of capturing anew.
+        const bool graph_now = graphs && it >= 1 && iterations_ - it > block;
+        if (graph_now && !aligned && !C_->graphs.empty() &&
+            !cached_graph(block) && iterations_ - it > block + 1) {
+            aligned = true;
+        } else if (graph_now) {
             StepGraph *g = step_graph(block);
             S.barrier();
             if (!counter_set) {

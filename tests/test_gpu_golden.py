@@ -85,6 +85,10 @@ def test_golden_rate_encoding(gpu_lib, name):
                                     golden_check.load(name), exact=False,
                                     rtol=1e-5, atol=1e-6, loose=loose)
     assert not problems, "\n".join(problems)
+    if name == "mixed_clusters":
+        # one stream per structure (two) and the environment's IO stream,
+        # ordered by the engine's events
+        assert report.get("streams") == "3"
 
 
 @pytest.mark.gpu
