@@ -226,6 +226,19 @@ typedef struct sg_iz_layer {
  * extractor.cpp:8-15, aggregator.cpp:15-48). */
 int sg_hip_activate_base(const sg_conn *c, void *stream);
 
+/* The base activator for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  No counterpart in the reference.  Same kernels and results
+ * as sg_hip_activate_base; the plane path's spike plane (sg_conn.plane is
+ * ignored) lives in a fixed device array of the library,
+ * SG_NOALLOC_PLANE_WORDS 64-bit words (512 KiB, sources up to 4,194,304),
+ * one copy per GPU, held until the process ends.  A connection with more
+ * sources on the plane path returns SG_HIP_ERROR_NOALLOC_LIMIT before any
+ * launch.  Calls share their GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race on it.  Allocates nothing. */
+enum { SG_NOALLOC_PLANE_WORDS = 65536, SG_HIP_ERROR_NOALLOC_LIMIT = 20001 };
+int sg_hip_activate_base_noalloc(const sg_conn *c, void *stream);
+
 /* activate_base_second_order (synapse_kernel.h:918-935, kernel.cpp:220): a
  * second-order slave connection folds each synapse's activity into the
  * host's weight copy, sow[w] = aggregate(opcode, sow[w], extract(source) *

@@ -514,5 +514,8 @@ extern "C" int sg_hip_device_count(void) {
 }
 
 extern "C" const char *sg_hip_error_string(int code) {
+    if (code == SG_HIP_ERROR_NOALLOC_LIMIT)
+        return "connection has more sources than the no-allocation "
+               "activator's plane holds (SG_NOALLOC_PLANE_WORDS * 64)";
     return hipGetErrorString((hipError_t)code);
 }
