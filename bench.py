@@ -21,8 +21,8 @@ Izhikevich layers of 1000, fully connected) and gol1024 / gol16384 (game of
 life boards; the larger one does not fit the 256 MB Infinity Cache).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
-                    [--workload sheet|sparse|izfc|gol1024|gol16384|
-                                ensemble|adjoint]
+                    [--workload sheet|sparse|izfc|izfc_io|gol1024|gol16384|
+                                ensemble|adjoint] [--run-args JSON]
 
 Default --steps: the configuration's own run length (2000 steps for the
 sheets, 1000 for the others).
@@ -247,6 +247,65 @@ def run(network, steps, extra=None):
     return dt, report
 
 
+def bench_izfc_io(a, lib):
+    """The Izhikevich 1000 -> 1000 network with host I/O every step: layer
+    1's N(5, 2) input current from an input callback and layer 2's spike
+    words to an output callback, both native C functions
+    (libsg_io_native.so), against the same network with the input drawn on
+    the device and no callbacks.  The engine overlaps each step's output
+    callback and the next step's input callback with the next step's first
+    launches; the ratio says what the I/O costs per step."""
+    io = ctypes.CDLL(os.path.join(ROOT, "synaptogenesis_amd",
+                                  "libsg_io_native.so"))
+    io.sg_io_configure.argtypes = [ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double]
+    io.sg_io_checksum.restype = ctypes.c_uint64
+    for name in ("sg_io_normal_input", "sg_io_checksum_output"):
+        lib.add_io_callback(name.encode(), ctypes.cast(
+            getattr(io, name), ctypes.c_void_p).value)
+    steps = a.steps or 1000
+    # without I/O: the input is the layer's init config, drawn on the GPU
+    wl = workloads.izfc()
+    net = build(lib, wl)
+    run(net, a.warmup)
+    dt0, _ = run(net, steps)
+    net.free()
+    # with I/O
+    wl = workloads.izfc()
+    del wl["net"]["structures"][0]["layers"][0]["init config"]
+    net = build(lib, wl)
+    env = {"modules": [{"type": "callback", "layers": [
+        {"structure": "izfc", "layer": "L1", "input": "true",
+         "function": "sg_io_normal_input", "id": "0"},
+        {"structure": "izfc", "layer": "L2", "output": "true",
+         "function": "sg_io_checksum_output", "id": "1",
+         "output keys": ["output"]}]}]}
+    args = {"worker threads": "0", "multithreaded": "false",
+            "learning flag": "true", "verbose": "false"}
+    io.sg_io_configure(0, 5.0, 2.0)
+    io.sg_io_configure(1, 0.0, 0.0)
+    net.run(env, dict(args, iterations=str(a.warmup)))
+    t0 = time.perf_counter()
+    rep = net.run(env, dict(args, iterations=str(steps)))
+    dt1 = time.perf_counter() - t0
+    if rep.get("iterations") != str(steps):
+        raise RuntimeError("run with I/O failed")
+    net.free()
+    ms0, ms1 = dt0 * 1e3 / steps, dt1 * 1e3 / steps
+    print(json.dumps({
+        "metric": "synaptic updates/sec with per-step host I/O on 1 MI355X",
+        "value": wl["visits_per_step"] * steps / dt1,
+        "unit": "synaptic updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": ms1, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: native N(5,2) input callback, spike words to a "
+                "native checksum callback, every step",
+        "config": {"workload": wl["name"] + " with per-step input and "
+                                           "output callbacks"},
+        "no_io_ms_per_step": ms0, "io_over_no_io": ms1 / ms0,
+        "output_checksum": "%016x" % io.sg_io_checksum(1)}))
+
+
 def cpu_baseline(wl, steps):
     """The reference's serial build on the host: `steps` steps of the same
     network on one thread ("worker threads" 0), then on a pool of worker
@@ -336,13 +395,24 @@ def bench_ensemble(a):
     widths = synthetic_widths()
     e = RateEnsemble(widths, "bf16", leak=0.1)
     rt, ks, tune_ms = e.autotune(steps=5)
-    e.run(a.warmup, graph=True)
-    e.run(steps, graph=True)              # captures the timed graph
+    e.run(a.warmup, graph=True, persistent=True)
+    e.run(steps, graph=True, persistent=True)   # builds the timed graph
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.run(steps, graph=True, persistent=True)
+    e.synchronize()
+    dt = time.perf_counter() - t0
+    blocks, captured, timeouts = e.persistent_info()
+    if timeouts:
+        raise RuntimeError("persistent run: %d grid-barrier timeouts"
+                           % timeouts)
+    # the same steps as one launch per step, captured in one graph
+    e.run(steps, graph=True)
     e.synchronize()
     t0 = time.perf_counter()
     e.run(steps, graph=True)
     e.synchronize()
-    dt = time.perf_counter() - t0
+    dt_steps = time.perf_counter() - t0
     kernel_ms = e.time_steps(20)
     c = e.counts()
     e.close()
@@ -380,10 +450,14 @@ def bench_ensemble(a):
         "config": {"workload": "batched rate ensemble: 256 instances, "
                                "x0 -> h1 -> h2 fully connected, widths "
                                "U{128..4096}, leak 0.1, tanh, %d steps in "
-                               "one HIP graph launch" % steps,
+                               "one persistent launch%s" % (
+                                   steps, " (HIP graph node)" if captured
+                                   else ""),
                    "synapses_per_step": syn,
                    "neurons": c["neurons"],
-                   "launch_shape": {"row_tiles": rt, "k_split": ks}},
+                   "launch_shape": {"row_tiles": rt, "k_split": ks,
+                                    "persistent_blocks": blocks}},
+        "step_launches_ms_per_step": dt_steps * 1000.0 / steps,
         "neuron_steps_per_s": c["neurons"] * steps / dt,
         "roofline": roofline_hbm("ens_bf16_kernel", kernel_ms,
                                  c["algorithmic_bytes"],
@@ -630,10 +704,11 @@ def main():
                     help="CPU-baseline sample (0: the workload's default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="sheet",
-                    choices=sorted(_workloads()) + ["ensemble", "adjoint"])
+                    choices=sorted(_workloads()) + ["ensemble", "adjoint",
+                                                    "izfc_io"])
     ap.add_argument("--run-args", default="{}",
                     help="extra engine run arguments as JSON, e.g. "
-                         "'{\"step graphs\": \"false\"}' (A/B runs)")
+                         "'{\"step graphs\": \"true\"}' (A/B runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference "
                          "for hardware byte counters) and exit")
@@ -653,6 +728,12 @@ def main():
         return
     if a.workload == "adjoint":
         bench_adjoint(a)
+        return
+    if a.workload == "izfc_io":
+        lib = syngen.load_library()
+        lib.set_suppress_output(True)
+        lib.set_warnings(False)
+        bench_izfc_io(a, lib)
         return
     if a.steps is None:
         a.steps = _workloads()[a.workload].get("steps", 1000)

@@ -76,6 +76,24 @@ int sg_ens_autotune(sg_ens *e, int steps, int *row_tiles, int *k_split,
  * count and launch shape) and launch it once.  Asynchronous. */
 int sg_ens_run(sg_ens *e, int steps, int use_graph, void *stream);
 
+/* Run `steps` steps in ONE persistent launch (BF16 ensembles): a grid of
+ * as many blocks as stay resident takes each step's work items from an
+ * atomic queue and meets at a grid barrier between steps.  The launch is
+ * cooperative (the runtime keeps every block resident or refuses it).
+ * use_graph: the launch is the one kernel node, with the cooperative
+ * attribute, of a HIP graph (cached per step count and parity) that is
+ * then launched; where the runtime has no such node it is launched
+ * directly.  Same results as sg_ens_run (each row's sum is computed by the
+ * same code, in the same order).  Asynchronous. */
+int sg_ens_run_persistent(sg_ens *e, int steps, int use_graph, void *stream);
+
+/* Grid of the last persistent launch, whether it ran from a graph, and the
+ * barrier waits that gave up so far (0 unless a block was never resident;
+ * the results of such a run are wrong).  Reading the count waits for the
+ * runs in flight. */
+int sg_ens_persistent_info(sg_ens *e, int *blocks, int *captured,
+                           int *barrier_timeouts);
+
 /* Wait for everything queued on the ensemble's own stream and on the
  * stream of the last sg_ens_run. */
 int sg_ens_synchronize(sg_ens *e);

@@ -242,7 +242,10 @@ int sg_hip_activate_base_noalloc(const sg_conn *c, void *stream);
 /* activate_base_second_order (synapse_kernel.h:918-935, kernel.cpp:220): a
  * second-order slave connection folds each synapse's activity into the
  * host's weight copy, sow[w] = aggregate(opcode, sow[w], extract(source) *
- * weights[w]); sow has the slave's (= the host's) device layout. */
+ * weights[w]); sow has the slave's (= the host's) device layout.  A
+ * convolutional slave (its field the whole source layer, stride 0) folds
+ * one-to-one over its source layer, weight i with source i
+ * (kernel.cpp:223-229, synapse_kernel.h:943-970). */
 int sg_hip_activate_second_order(const sg_conn *slave, float *sow,
                                  void *stream);
 

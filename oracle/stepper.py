@@ -198,7 +198,17 @@ class Conn:
         self.delay = int(d.get("delay", "0"))
         self.max_weight = stof(d.get("max weight", "1.0"))
         self.convolutional = _bool(d.get("convolutional"), False)
-        self.sparse = _bool(d.get("sparse"), False)
+        # a randomized projection is sparse (connection.cpp:138-139)
+        self.randomized = _bool(d.get("randomized projection"), False)
+        self.sparse = _bool(d.get("sparse"), False) or self.randomized
+        if self.randomized and (self.convolutional or
+                                d.get("type") != "one to one"):
+            # only a one-to-one projection's sources can be injected whole
+            # (weight_matrix.cpp:871-890: the indices callback writes the
+            # from indices that sparsify keeps); a convergent one's random
+            # field corners come from the reference's unseeded generator
+            raise ValueError("oracle: randomized projection " +
+                             d.get("name", ""))
         node = self.dst.node(d.get("dendrite", "root"))
         self.reg = node["reg"]
         # second order (dendritic_node.cpp:118-139): the node's first
@@ -243,6 +253,18 @@ class Conn:
         self.geom = g
         self.arrays = {}
         self._enumerate()
+        if self.convolutional and not self.so_host and \
+                node.get("second order", False):
+            # a convolutional second-order slave folds like a one-to-one
+            # connection over its source layer: weight i with source i
+            # (engine/kernel/kernel.cpp:223-229, synapse_kernel.h:943-970
+            # CALC_ONE_TO_ONE)
+            n = self.src.n
+            self.dest = np.arange(n, dtype=np.int32)
+            self.first = np.arange(n + 1, dtype=np.int64)
+            self.srcs = np.arange(n, dtype=np.int32)
+            self.widx = np.arange(n, dtype=np.int64)
+            self.ndest = n
 
     def _enumerate(self):
         lib = load()
@@ -498,6 +520,15 @@ class Network:
         for c in self.conns:
             c.weights = _init_weights(c, case.weights.get(c.name))
             delays = case.delays.get(c.name)
+            injected = getattr(case, "indices", {}).get(c.name)
+            if c.randomized:
+                # randomize_projection (weight_matrix.cpp:380-393) draws a
+                # source per destination; the indices callback then writes
+                # the from indices (:871-890), which are what survive
+                if injected is None:
+                    raise ValueError("oracle: randomized projection %s "
+                                     "needs injected sources" % c.name)
+                c.srcs = np.asarray(injected, np.int32).copy()
             if c.sparse:
                 c.weights, delays = c.sparsify(c.weights, delays)
             c.iz = c.dst.model == "izhikevich"

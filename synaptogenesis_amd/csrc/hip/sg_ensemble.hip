@@ -72,14 +72,15 @@ __device__ __forceinline__ float rate(float y, int rectify) {
     return tanhf(y);
 }
 
+// One work item: RT row tiles of one layer, W . x and the state update,
+// every thread of the block taking part (it stages the input in `lds`).
 template <int RT, int KS>
-__global__ __launch_bounds__(64 * RT * KS) void ens_bf16_kernel(
-    const Layer *__restrict__ layers, const Item *__restrict__ items,
-    int parity, float leak, int rectify) {
-    extern __shared__ __align__(16) unsigned char lds[];
+__device__ __forceinline__ void bf16_item(const Layer *__restrict__ layers,
+                                          const Item it, int parity,
+                                          float leak, int rectify,
+                                          unsigned char *lds) {
     // Fields are read through the pointer: a local copy of the descriptor
     // indexed by the run-time parity would live in scratch memory.
-    const Item it = items[blockIdx.x];
     const Layer &L = layers[it.layer];
     const int Kp = L.Kp;
     uint16_t *xs = (uint16_t *)lds;   // [3][Kp]
@@ -156,6 +157,98 @@ __global__ __launch_bounds__(64 * RT * KS) void ens_bf16_kernel(
     for (int i = 0; i < 4; ++i) {
         const int row = rt * kTileRows + 4 * g + i;
         if (row < L.R) hd[row] = keep * hs[row] + leak * rate(y[i], rectify);
+    }
+}
+
+template <int RT, int KS>
+__global__ __launch_bounds__(64 * RT * KS) void ens_bf16_kernel(
+    const Layer *__restrict__ layers, const Item *__restrict__ items,
+    int parity, float leak, int rectify) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    bf16_item<RT, KS>(layers, items[blockIdx.x], parity, leak, rectify, lds);
+}
+
+// Grid barrier of the persistent kernel, on two words of device memory
+// (bar[0] arrivals, bar[1] generation): the last block to arrive clears
+// the arrivals and advances the generation; the others wait for the
+// generation to change.  The launch keeps every block resident (a
+// cooperative launch, or a graph node with the cooperative attribute), so
+// every block arrives; still, a wait gives up after about a second of
+// polling and counts a timeout in bar[2] (the run's results are then
+// wrong, and the host reports it) rather than spinning for ever.
+constexpr long kBarrierPolls = 1L << 24;
+
+__device__ __forceinline__ void grid_barrier(unsigned int *bar,
+                                             unsigned int blocks) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int gen = __hip_atomic_load(
+            bar + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned int arrived = __hip_atomic_fetch_add(
+            bar, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == blocks - 1) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long polls = 0;
+            while (__hip_atomic_load(bar + 1, __ATOMIC_ACQUIRE,
+                                     __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++polls > kBarrierPolls) {
+                    __hip_atomic_fetch_add(bar + 2, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// Every step of a run in one launch: the blocks, all resident, take the
+// step's items from an atomic queue (longest fan-in first, as the list is
+// ordered), then meet at the grid barrier, so that the next step reads
+// complete states.  Step t uses queue[t & 1]; block 0 clears it after the
+// barrier that ends step t, and the barrier that ends step t + 1 orders
+// the clear before its next use.  Every block runs the same number of
+// barriers.  sync = {queue[2], barrier[3]}.
+template <int RT, int KS>
+__global__ __launch_bounds__(64 * RT * KS) void ens_bf16_persistent(
+    const Layer *__restrict__ layers, const Item *__restrict__ items,
+    int n_items, int parity, int steps, float leak, int rectify,
+    unsigned int *sync) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int next;
+    unsigned int *queue = sync, *bar = sync + 2;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        queue[0] = 0u;
+        queue[1] = 0u;
+    }
+    grid_barrier(bar, gridDim.x);
+    for (int t = 0; t < steps; ++t) {
+        unsigned int *q = queue + (t & 1);
+        if (threadIdx.x == 0) next = (int)atomicAdd(q, 1u);
+        __syncthreads();
+        int i = next;
+        __syncthreads();
+        while (i < n_items) {
+            // the next item's index is fetched while this one is computed
+            // (thread 0 keeps it in a register until the item is done)
+            unsigned int ahead = 0u;
+            if (threadIdx.x == 0) ahead = atomicAdd(q, 1u);
+            bf16_item<RT, KS>(layers, items[i], parity ^ (t & 1), leak,
+                              rectify, lds);
+            __syncthreads();
+            if (threadIdx.x == 0) next = (int)ahead;
+            __syncthreads();
+            i = next;
+        }
+        grid_barrier(bar, gridDim.x);
+        if (blockIdx.x == 0 && threadIdx.x == 0) queue[t & 1] = 0u;
     }
 }
 
@@ -315,6 +408,13 @@ struct sg_ens {
     int parity = 0;
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, parity)
     std::map<std::pair<int, int>, hipGraph_t> graph_defs;
+    // persistent runs: the work queue, the grid of the last launch, and the
+    // graphs holding one persistent launch (by steps and parity)
+    unsigned int *d_queue = nullptr;
+    int persist_blocks = 0;
+    int persist_captured = 0;
+    std::map<std::pair<int, int>, hipGraphExec_t> pgraphs;
+    std::map<std::pair<int, int>, hipGraph_t> pgraph_defs;
 };
 
 namespace {
@@ -373,8 +473,80 @@ int launch_step(const sg_ens *e, int parity, hipStream_t s) {
 void drop_graphs(sg_ens *e) {
     for (auto &kv : e->graphs) (void)hipGraphExecDestroy(kv.second);
     for (auto &kv : e->graph_defs) (void)hipGraphDestroy(kv.second);
+    for (auto &kv : e->pgraphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : e->pgraph_defs) (void)hipGraphDestroy(kv.second);
     e->graphs.clear();
     e->graph_defs.clear();
+    e->pgraphs.clear();
+    e->pgraph_defs.clear();
+}
+
+// A persistent launch of `steps` steps from state `parity`: the grid is as
+// many blocks as fit resident at once (the occupancy the runtime reports
+// for this block shape and LDS size, times the compute units), at most one
+// per item.  With `graph` null it is launched at once as a cooperative
+// kernel (the runtime refuses one it cannot keep resident); otherwise it
+// becomes the one kernel node of *graph, with the cooperative attribute.
+template <int RT, int KS>
+hipError_t launch_persistent_t(sg_ens *e, int steps, int parity,
+                               hipStream_t s, hipGraph_t *graph) {
+    auto *k = ens_bf16_persistent<RT, KS>;
+    const size_t lds = lds_bytes(e);
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t err = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(k), 64 * RT * KS, lds);
+    if (err != hipSuccess) return err;
+    if ((err = hipGetDevice(&dev)) != hipSuccess) return err;
+    if ((err = hipDeviceGetAttribute(
+             &cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+        return err;
+    if (per_cu <= 0 || cus <= 0) return hipErrorLaunchOutOfResources;
+    const int n_items = (int)e->items.size();
+    const int blocks = std::max(1, std::min(n_items, per_cu * cus));
+    e->persist_blocks = blocks;
+    const Layer *layers = e->d_layers;
+    const Item *items = e->d_items;
+    float leak = e->leak;
+    int rectify = e->rectify;
+    unsigned int *sync = e->d_queue;
+    int n = n_items, p = parity, st = steps;
+    void *args[] = {(void *)&layers, (void *)&items, (void *)&n, (void *)&p,
+                    (void *)&st, (void *)&leak, (void *)&rectify,
+                    (void *)&sync};
+    if (!graph)
+        return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k),
+                                          dim3(blocks), dim3(64 * RT * KS),
+                                          args, (unsigned int)lds, s);
+    if ((err = hipGraphCreate(graph, 0)) != hipSuccess) return err;
+    hipKernelNodeParams kp;
+    std::memset(&kp, 0, sizeof(kp));
+    kp.func = reinterpret_cast<void *>(k);
+    kp.gridDim = dim3(blocks);
+    kp.blockDim = dim3(64 * RT * KS);
+    kp.sharedMemBytes = (unsigned int)lds;
+    kp.kernelParams = args;
+    hipGraphNode_t node = nullptr;
+    if ((err = hipGraphAddKernelNode(&node, *graph, nullptr, 0, &kp)) !=
+        hipSuccess)
+        return err;
+    hipKernelNodeAttrValue v;
+    std::memset(&v, 0, sizeof(v));
+    v.cooperative = 1;
+    return hipGraphKernelNodeSetAttribute(node, hipKernelNodeAttributeCooperative,
+                                          &v);
+}
+
+hipError_t launch_persistent(sg_ens *e, int steps, int parity,
+                             hipStream_t s, hipGraph_t *graph) {
+#define ENS_PCASE(R, K) \
+    if (e->rt == R && e->ks == K) \
+        return launch_persistent_t<R, K>(e, steps, parity, s, graph);
+    ENS_PCASE(1, 1) ENS_PCASE(1, 2) ENS_PCASE(1, 4) ENS_PCASE(1, 8)
+    ENS_PCASE(1, 16) ENS_PCASE(2, 1) ENS_PCASE(2, 2) ENS_PCASE(2, 4)
+    ENS_PCASE(2, 8) ENS_PCASE(4, 1) ENS_PCASE(4, 2) ENS_PCASE(4, 4)
+    ENS_PCASE(8, 1) ENS_PCASE(8, 2) ENS_PCASE(16, 1)
+#undef ENS_PCASE
+    return hipErrorInvalidValue;
 }
 
 // Waits for the work of the last run, wherever it was launched.
@@ -552,6 +724,7 @@ void sg_ens_destroy(sg_ens *e) {
     if (e->d_backup) (void)hipFree(e->d_backup);
     if (e->d_layers) (void)hipFree(e->d_layers);
     if (e->d_items) (void)hipFree(e->d_items);
+    if (e->d_queue) (void)hipFree(e->d_queue);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -620,6 +793,73 @@ int sg_ens_run(sg_ens *e, int steps, int use_graph, void *stream) {
     }
     ENS_CHECK(hipGraphLaunch(it->second, s), "hipGraphLaunch");
     if (steps & 1) e->parity ^= 1;
+    return 0;
+}
+
+int sg_ens_run_persistent(sg_ens *e, int steps, int use_graph,
+                          void *stream) {
+    if (!e) return fail("null ensemble");
+    if (steps < 0) return fail("negative step count");
+    if (e->precision != SG_ENS_BF16)
+        return fail("persistent runs: BF16 ensembles only");
+    hipStream_t s = stream ? (hipStream_t)stream : e->stream;
+    if (steps == 0 || e->items.empty()) return 0;
+    if (!e->d_queue) {
+        ENS_CHECK(hipMalloc(&e->d_queue, 8 * sizeof(unsigned int)),
+                  "hipMalloc queue");
+        ENS_CHECK(hipMemset(e->d_queue, 0, 8 * sizeof(unsigned int)),
+                  "hipMemset queue");
+    }
+    e->last = s;
+    if (!use_graph) {
+        ENS_CHECK(launch_persistent(e, steps, e->parity, s, nullptr),
+                  "persistent launch");
+        e->persist_captured = 0;
+        if (steps & 1) e->parity ^= 1;
+        return 0;
+    }
+    const std::pair<int, int> key(steps, e->parity);
+    auto it = e->pgraphs.find(key);
+    if (it == e->pgraphs.end()) {
+        hipGraph_t g = nullptr;
+        hipError_t err = launch_persistent(e, steps, e->parity, s, &g);
+        hipGraphExec_t ex = nullptr;
+        if (err == hipSuccess)
+            err = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (err != hipSuccess) {
+            // no cooperative graph node here: launch it directly (still
+            // one launch for every step)
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            ENS_CHECK(launch_persistent(e, steps, e->parity, s, nullptr),
+                      "persistent launch");
+            e->persist_captured = 0;
+            if (steps & 1) e->parity ^= 1;
+            return 0;
+        }
+        e->pgraph_defs[key] = g;
+        it = e->pgraphs.emplace(key, ex).first;
+    }
+    ENS_CHECK(hipGraphLaunch(it->second, s), "hipGraphLaunch");
+    e->persist_captured = 1;
+    if (steps & 1) e->parity ^= 1;
+    return 0;
+}
+
+int sg_ens_persistent_info(sg_ens *e, int *blocks, int *captured,
+                           int *barrier_timeouts) {
+    if (!e) return fail("null ensemble");
+    if (blocks) *blocks = e->persist_blocks;
+    if (captured) *captured = e->persist_captured;
+    if (barrier_timeouts) {
+        unsigned int words[8] = {0};
+        if (e->d_queue) {
+            if (wait_runs(e)) return 1;
+            ENS_CHECK(hipMemcpy(words, e->d_queue, sizeof(words),
+                                hipMemcpyDeviceToHost), "read barrier");
+        }
+        *barrier_timeouts = (int)words[4];
+    }
     return 0;
 }
 

@@ -410,10 +410,39 @@ static hipError_t launch_second_order_t(const sg_conn &c, float *sow,
     return hipGetLastError();
 }
 
+// A convolutional slave (kernel.cpp:223-229, synapse_kernel.h:943-970):
+// one-to-one over its source layer, sow[i] = aggregate(sow[i],
+// extract(source i) * weights[i]).
+template <int OUT>
+__global__ void second_order_conv_kernel(sg_conn c, float *sow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.from_size) return;
+    const float val = extract<OUT>(c.src[i], c.delay) * c.weights[i];
+    sow[i] = aggregate(c.opcode, sow[i], val);
+}
+
 extern "C" int sg_hip_activate_second_order(const sg_conn *c, float *sow,
                                             void *stream) {
-    if (c->num_dest <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
+    if (c->convolutional) {
+        if (c->from_size <= 0) return 0;
+        dim3 g((c->from_size + 255) / 256), b(256);
+        switch (c->out_type) {
+            case SG_OUT_BIT:
+                hipLaunchKernelGGL(second_order_conv_kernel<SG_OUT_BIT>, g, b,
+                                   0, s, *c, sow);
+                break;
+            case SG_OUT_INT:
+                hipLaunchKernelGGL(second_order_conv_kernel<SG_OUT_INT>, g, b,
+                                   0, s, *c, sow);
+                break;
+            default:
+                hipLaunchKernelGGL(second_order_conv_kernel<SG_OUT_FLOAT>, g,
+                                   b, 0, s, *c, sow);
+        }
+        return hipGetLastError();
+    }
+    if (c->num_dest <= 0) return 0;
     switch (c->type) {
         case SG_FULLY_CONNECTED:
             return launch_second_order_t<SG_FULLY_CONNECTED>(*c, sow, s);

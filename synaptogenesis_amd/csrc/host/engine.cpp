@@ -1608,15 +1608,18 @@ Props *Runner::run() {
         S.end();
     }
 
-    // Runs without environment I/O capture their steady-state steps (from
-    // the second on, all but the last) into HIP graphs of `block` steps --
-    // when at most two streams take part (one structure and its activator
-    // stream): a capture over three streams (two structures) ended the
-    // process with a segmentation fault on the GPU host once, cause not
-    // found, so such runs stay eager.
+    // With "step graphs": "true", runs without environment I/O capture their
+    // steady-state steps (from the second on, all but the last) into HIP
+    // graphs of `block` steps -- when at most two streams take part (one
+    // structure and its activator stream): a capture over three streams
+    // (two structures) ended the process with a segmentation fault on the
+    // GPU host once, cause not found, so such runs stay eager.  Off by
+    // default: on the benchmark networks the GPU, not the host's launches,
+    // bounds the step (the same step time with and without graphs), and a
+    // capture costs milliseconds.
     const bool graphs = modules_.empty() && !kt_.on && !any_second_order_ &&
                         nstreams_ <= 2 &&
-                        args_.get_bool("step graphs", true) &&
+                        args_.get_bool("step graphs", false) &&
                         iterations_ > 0;
     int block = std::max(2, args_.get_int("graph steps", 8));
     block += block & 1;

@@ -248,8 +248,27 @@ void Network::add_connection(const Props &cc) {
     if (c->delay < 0)
         throw Error("Attempted to construct ConnectionConfig with negative "
                     "delay!");
-    if (cc.get_bool("randomized projection", false))
-        throw Error("Randomized-projection connections are not supported");
+    // A randomized projection is sparse (connection.cpp:138-139); its
+    // sources are drawn when the weights are built
+    // (weight_matrix.cpp:356-412).
+    c->randomized = cc.get_bool("randomized projection", false);
+    if (c->randomized) {
+        c->sparse = true;
+        if (c->convolutional)
+            throw Error("Error intializing weight matrix for " + c->str() +
+                        "\n   Cannot randomize projection on convolutional "
+                        "weight matrix!");
+        if (c->type == SG_FULLY_CONNECTED || c->type == SG_SUBSET)
+            throw Error("Error intializing weight matrix for " + c->str() +
+                        "\n   Cannot randomize projection on fully "
+                        "connected or subset weight matrix!");
+        // the reference builds a divergent one by inverting a randomized
+        // convergent matrix into slots it never bounds-checks
+        // (weight_matrix.cpp:444-503)
+        if (c->type == SG_DIVERGENT)
+            throw Error("Randomized divergent projections are not supported "
+                        "(" + c->str() + ")");
+    }
     if (c->sparse && c->convolutional)   // connection_config.cpp:234-236
         throw Error("Convolutional connections cannot be sparse!");
     if (c->sparse && c->type == SG_SUBSET)
@@ -319,14 +338,23 @@ void Network::add_connection(const Props &cc) {
         if (c->sparse)
             throw Error("Error in " + c->str() + ":\n  Sparse second "
                         "order connections are not supported!");
-        if (c->convolutional)
-            throw Error("Convolutional second-order connections are not "
-                        "supported (" + c->str() + ")");
         if (c->so_slave &&
             (c->type != node->host->type ||
+             c->convolutional != node->host->convolutional ||
              c->num_weights != node->host->num_weights))
             throw Error("Error in " + c->str() + ":\n  Second order "
                         "connection does not match host connection!");
+        // A convolutional slave folds like a one-to-one connection over its
+        // source layer (kernel.cpp:223-229), so its field covers that layer
+        // with one visit per source.
+        if (c->so_slave && c->convolutional &&
+            (c->arbor.total() != from->size || c->arbor.row_spacing != 1 ||
+             c->arbor.col_spacing != 1 || c->arbor.row_stride != 0 ||
+             c->arbor.col_stride != 0))
+            throw Error("Error in " + c->str() + ":\n  Second order "
+                        "convolutional connections must have fields that "
+                        "are the size of the input layer, and must have 0 "
+                        "stride and 1 spacing!");
     }
     if (c->so_host) node->host = c.get();
     else node->conns.push_back(c.get());

@@ -87,6 +87,9 @@ struct Connection {
     float max_weight = 1.0f;
     bool plastic = true, convolutional = false, sparse = false;
     bool recurrent = false;
+    // randomized projection (connection_config.cpp:219): sources drawn at
+    // random, the connection sparse (connection.cpp:138-139)
+    bool randomized = false;
     int reg = 0;          // input register of its dendritic node
     std::string node = "root";                 // its dendritic node
     bool so_host = false, so_slave = false;    // second-order roles

@@ -277,6 +277,7 @@ void State::build_connection(Connection *c) {
                         " has no base aggregator");
     }
 
+    if (c->randomized) randomize_projection(s);
     init_weights(s);
     if (c->sparse) sparsify(s);
     if ((c->so_host || c->so_slave) && s.iz && !s.direct)
@@ -531,6 +532,43 @@ void State::compute_indices(ConnState &cs) {
         ti[wi] = to;
         used[wi] = 1;
     });
+}
+
+// WeightMatrix::randomize_projection (weight_matrix.cpp:356-507), before
+// the weights are initialised: every destination of a one-to-one
+// connection reads one random source neuron; every destination of a
+// convergent connection keeps its field's shape, moved so that its first
+// visit sits at a random source neuron, and its source indices are row *
+// rows + column as the reference writes them (with wrap, the sparse index
+// adjustment recomputes them from rows and columns).  The draws come from
+// the state's seeded generator, where the reference's is unseeded.
+void State::randomize_projection(ConnState &cs) {
+    Connection &c = *cs.conn;
+    compute_indices(cs);
+    int32_t *fr = cs.arr("from row indices").i();
+    int32_t *fc = cs.arr("from column indices").i();
+    int32_t *fi = cs.arr("from indices").i();
+    std::uniform_int_distribution<int> row_d(0, c.from->rows - 1);
+    std::uniform_int_distribution<int> col_d(0, c.from->cols - 1);
+    if (c.type == SG_ONE_TO_ONE) {
+        for (int i = 0; i < c.to->size; ++i) {
+            const int r = fr[i] = row_d(rng());
+            const int q = fc[i] = col_d(rng());
+            fi[i] = r * c.from->cols + q;
+        }
+        return;
+    }
+    const int K = cs.k_per_dest;
+    for (int i = 0; i < c.to->size; ++i) {
+        const int nr = row_d(rng()), nc = col_d(rng());
+        const long corner = (long)K * i;
+        const int dr = nr - fr[corner], dc = nc - fc[corner];
+        for (int k = 0; k < K; ++k) {
+            const int r = (fr[corner + k] += dr);
+            const int q = (fc[corner + k] += dc);
+            fi[corner + k] = r * c.from->rows + q;
+        }
+    }
 }
 
 // get_distances (weight_matrix.cpp:1199-1317).
@@ -901,10 +939,22 @@ void State::upload_conn_derived(ConnState &cs) {
         std::vector<int32_t> idx(tiled_elems(cs.num_dest, K), -1);
         const int32_t *cnt = cs.arr("nonzero counts").i();
         const int32_t *fi = cs.arr("from indices").i();
-        for (int slot = 0; slot < cs.num_dest; ++slot)
-            for (int k = 0; k < cnt[slot]; ++k)
-                idx[(((long)slot >> 6) * K + k) * 64 + (slot & 63)] =
-                    fi[(long)slot * K + k];
+        for (int slot = 0; slot < cs.num_dest; ++slot) {
+            if (cnt[slot] < 0 || cnt[slot] > K)
+                throw Error("Invalid nonzero count in " + c.str());
+            for (int k = 0; k < cnt[slot]; ++k) {
+                const int32_t f = fi[(long)slot * K + k];
+                // a source outside the layer would be read out of bounds
+                // (e.g. the reference's randomized convergent projection
+                // without wrap, whose index row * rows + column can leave
+                // a non-square layer, weight_matrix.cpp:436-437)
+                if (f < 0 || f >= c.from->size)
+                    throw Error("Error in " + c.str() + ":\n  sparse source "
+                                "index " + std::to_string(f) +
+                                " outside the source layer");
+                idx[(((long)slot >> 6) * K + k) * 64 + (slot & 63)] = f;
+            }
+        }
         if (cs.d_sparse_from) check_hip(hipFree(cs.d_sparse_from), "hipFree");
         check_hip(hipMalloc((void **)&cs.d_sparse_from, idx.size() * 4),
                   "hipMalloc sparse");

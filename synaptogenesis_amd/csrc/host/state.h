@@ -165,6 +165,7 @@ class State {
     void process_iz_matrix(ConnState &cs);
     void compute_indices(ConnState &cs);
     void sparsify(ConnState &cs);
+    void randomize_projection(ConnState &cs);
     void adjust_sparse_indices(ConnState &cs);
     void compute_distances(ConnState &cs, float from_spacing,
                            float to_spacing, float x_offset, float y_offset);

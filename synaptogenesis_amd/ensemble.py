@@ -2,8 +2,9 @@
 
 Many independent rate networks of uneven width -- an input layer x0 feeding
 two fully connected rate layers, h1 = layer 1 and h2 = layer 2 -- step
-together: one kernel launch per step over every row of every instance, a
-run of many steps captured once into a HIP graph.  Per step, every layer
+together: one kernel launch per step over every row of every instance (a
+run of many steps captured once into a HIP graph), or every step of a run
+in one persistent cooperative launch with a grid barrier between steps.  Per step, every layer
 reads the previous step's outputs (the parallel cluster's one-step lag,
 engine/cluster/parallel_cluster.cpp:107-131):
 
@@ -50,6 +51,8 @@ def library():
         "sg_ens_get_tile": (I, [P, ip, ip]),
         "sg_ens_autotune": (I, [P, I, ip, ip, fp]),
         "sg_ens_run": (I, [P, I, I, P]),
+        "sg_ens_run_persistent": (I, [P, I, I, P]),
+        "sg_ens_persistent_info": (I, [P, ip, ip, ip]),
         "sg_ens_synchronize": (I, [P]),
         "sg_ens_time_steps": (I, [P, I, fp]),
         "sg_ens_read_state": (I, [P, I, I, fp]),
@@ -119,9 +122,21 @@ class RateEnsemble:
         except Exception:
             pass
 
-    def run(self, steps, graph=True, stream=None):
-        self._check(self.lib.sg_ens_run(self.handle, steps, int(graph),
-                                        stream))
+    def run(self, steps, graph=True, stream=None, persistent=False):
+        """`steps` steps: one launch per step (in one HIP graph when
+        `graph`), or with `persistent` every step in one cooperative launch
+        (captured into a graph when `graph` and the runtime allows)."""
+        fn = (self.lib.sg_ens_run_persistent if persistent
+              else self.lib.sg_ens_run)
+        self._check(fn(self.handle, steps, int(graph), stream))
+
+    def persistent_info(self):
+        """(blocks of the last persistent launch, 1 if it ran from a graph,
+        grid-barrier waits that gave up so far -- 0 for a correct run)."""
+        b, c, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.sg_ens_persistent_info(
+            self.handle, ctypes.byref(b), ctypes.byref(c), ctypes.byref(t)))
+        return b.value, c.value, t.value
 
     def synchronize(self):
         self._check(self.lib.sg_ens_synchronize(self.handle))

@@ -12,9 +12,11 @@ regenerated from their seeds by whoever checks against the fixture.
 Usage:  python tests/golden/make_golden.py [case ...]
 """
 
+import ctypes
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -72,6 +74,54 @@ def fixture_arrays(case, captures, finals):
     return out
 
 
+def _std_hash(text):
+    """libstdc++'s std::hash<std::string>, the key hash of the state file's
+    records (state/state.cpp:365-447, util/pointer.h PointerKey): the
+    library's _Hash_bytes with its default seed 0xc70f6907."""
+    f = ctypes.CDLL("libstdc++.so.6")._ZSt11_Hash_bytesPKvmm
+    f.restype = ctypes.c_size_t
+    f.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_size_t]
+    b = text.encode()
+    return f(b, len(b), 0xc70f6907)
+
+
+def _records(path):
+    """(id, key hash) -> bytes of every record of a state file: a sequence
+    of (uint64 id, uint64 hash, uint64 bytes) headers, each followed by its
+    bytes."""
+    data = open(path, "rb").read()
+    out, pos = {}, 0
+    while pos < len(data):
+        ident, h, n = np.frombuffer(data, "<u8", 3, pos)
+        pos += 24
+        out[(int(ident), int(h))] = data[pos:pos + int(n)]
+        pos += int(n)
+    return out
+
+
+def drawn_sources(case, network):
+    """The sources the reference drew for each of ``case.drawn`` (one-to-one
+    randomized projections: one synapse per destination), read from its
+    saved state.  A connection's records are found by its weights, which
+    the case injects; destinations whose weight is zero (dropped by the
+    sparse matrix, source -1) get source 0, which no run reads."""
+    path = os.path.join(tempfile.mkdtemp(), "drawn.bin")
+    assert network.save_state(path)
+    recs = _records(path)
+    hw, hf = _std_hash("weights"), _std_hash("from indices")
+    out = {}
+    for cname in case.drawn:
+        want = case.weights[cname].astype(np.float32).tobytes()
+        ids = [i for (i, h), b in recs.items() if h == hw and b == want]
+        assert len(ids) == 1, (cname, len(ids))
+        src = np.frombuffer(recs[(ids[0], hf)], np.int32).copy()
+        assert src.size == case.weights[cname].size, cname
+        assert np.all((src >= 0) == (case.weights[cname] != 0)), cname
+        out[cname] = np.maximum(src, 0)
+    os.remove(path)
+    return out
+
+
 def main(names):
     lib = syngen.load_library(REF_LIB)
     lib.set_suppress_output(True)
@@ -81,8 +131,21 @@ def main(names):
         cases = [c for c in cases if c.name in names]
     for case in cases:
         t0 = time.time()
-        captures, finals, _ = harness.run_case(case, lib)
+        if case.drawn:
+            # the reference draws these sources itself
+            case.indices = {k: v for k, v in case.indices.items()
+                            if k not in case.drawn}
+            sess = harness.Session(case, lib)
+            sess.run(case.steps)
+            captures, finals = sess.captures, sess.finals()
+            drawn = drawn_sources(case, sess.network)
+            sess.free()
+        else:
+            captures, finals, _ = harness.run_case(case, lib)
+            drawn = {}
         arrays = fixture_arrays(case, captures, finals)
+        for cname, src in drawn.items():
+            arrays["drawn|" + cname] = src
         path = os.path.join(HERE, case.name + ".npz")
         np.savez_compressed(path, **arrays)
         print("%-14s %6.2fs  %7.1f KB" % (case.name, time.time() - t0,

@@ -31,12 +31,17 @@ class Case:
     ``final_neuron`` / ``final_conn`` list data read after the run.
     ``reduce``, when set, maps each captured array to what is kept of it
     (e.g. a digest), so that long runs need not hold every step's array.
+    ``indices`` maps connection name -> the source index of every synapse,
+    written through an indices callback (a randomized projection's
+    sources).  ``drawn`` names randomized connections whose sources the
+    reference draws itself when its fixture is written; the fixture keeps
+    those draws and the case injects them everywhere else.
     """
 
     def __init__(self, name, net, steps, learning=True, weights=None,
                  delays=None, init=None, inputs=None, capture=(),
                  final_neuron=(), final_conn=(), keep_steps=None,
-                 reduce=None):
+                 reduce=None, indices=None, drawn=()):
         self.name = name
         self.net = net
         self.steps = steps
@@ -50,6 +55,10 @@ class Case:
         self.final_conn = list(final_conn)
         self.keep_steps = keep_steps
         self.reduce = reduce
+        # connection name -> source index per synapse, written through an
+        # "indices callback" (state/weight_matrix.cpp:871-890)
+        self.indices = indices or {}
+        self.drawn = tuple(drawn)
 
 
 def digest(arr):
@@ -119,7 +128,8 @@ class Session:
         # Weight / delay injection through weight-config callbacks.
         conns = {c.get("name", ""): c for c in net.get("connections", [])}
         wlist = []
-        for cname in sorted(set(case.weights) | set(case.delays)):
+        for cname in sorted(set(case.weights) | set(case.delays) |
+                            set(case.indices)):
             c = conns[cname]
             wc = dict(c.get("weight config", {}))
             wc.setdefault("type", "flat")
@@ -129,6 +139,8 @@ class Session:
                 wc["callback"] = "W_" + tag
             if cname in case.delays:
                 wc["delay callback"] = "D_" + tag
+            if cname in case.indices:
+                wc["indices callback"] = "X_" + tag
             c["weight config"] = wc
             wlist.append(cname)
 
@@ -146,7 +158,17 @@ class Session:
                 syngen.cast(dptr, syngen.POINTER(syngen.c_int)),
                 shape=(size,))[:] = d
 
+        def indices_cb(ID, size, wptr, from_ptr, to_ptr, used_ptr):
+            f = case.indices[wlist[ID]]
+            assert f.size == size, (wlist[ID], f.size, size)
+            np.ctypeslib.as_array(
+                syngen.cast(from_ptr, syngen.POINTER(syngen.c_int)),
+                shape=(size,))[:] = f
+
         self.keep = []
+        if case.indices:
+            self.keep.append(lib.create_indices_weight_callback(
+                "X_" + tag, indices_cb))
         if case.weights:
             self.keep.append(lib.create_weight_callback("W_" + tag,
                                                         weight_cb))

@@ -56,7 +56,8 @@ def main():
                             final_conn=case.final_conn)
         sess = harness.Session(bare, lib, build=False)
         assert sess.network.build_state(path), case.name
-        for extra in ({}, {"step graphs": "false"}, {"graph steps": "2"},
+        for extra in ({}, {"step graphs": "true"},
+                      {"step graphs": "true", "graph steps": "2"},
                       {"structure streams": "false"},
                       {"concurrent activators": "false"},
                       {"kernel timing": "true"},
@@ -73,6 +74,7 @@ def main():
         net = bench.build(lib, make())
         for n in (5, 20, 3):
             bench.run(net, n)
+            bench.run(net, n, {"step graphs": "true"})
         net.free()
         print("%-16s ok" % ("workload " + name), flush=True)
     stub = ctypes.CDLL(LIB)

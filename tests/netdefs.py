@@ -6,6 +6,8 @@ CPU build finishes each case in seconds; the full benchmark sizes are built by
 the same functions with larger arguments.
 """
 
+import os
+
 import numpy as np
 
 from synaptogenesis_amd.inputs import Stream
@@ -701,6 +703,94 @@ def second_order(steps=20, workload=122, name="second_order"):
                 final_conn=[(c, "weights") for c in ("host", "fhost")])
 
 
+def second_order_conv(steps=20, workload=123, name="second_order_conv"):
+    """Convolutional second-order nodes (connection.cpp:163-182,
+    kernel.cpp:223-229, synapse_kernel.h:943-970): a relay layer whose
+    node hosts a convolutional 3x3 convergent connection, and whose slaves
+    -- convolutional with the whole 3x3 source layer as their field and
+    stride 0 -- fold one-to-one, weight i with source i, into the host's
+    shared kernel each step: a multiplying slave from a relay layer, an
+    adding one from a game-of-life layer (BIT, delay 1), a subtracting one
+    from the relay layer again.  (A divergent slave would need stride 0,
+    which divergent connections refuse, connection.cpp:38-44.)"""
+    st = Stream(workload)
+    tree = [{"name": "cc", "opcode": "add", "second order": "true"}]
+    L = [layer("in", "relay", 6, 6), layer("k", "relay", 3, 3),
+         layer("g", "game_of_life", 3, 3),
+         dict(layer("d", "relay", 6, 6), dendrites=tree)]
+    full = {"field size": 3, "stride": 0}
+    C = [conn("chost", "in", "d", "convergent", dendrite="cc",
+              convolutional=True, arbor={"field size": 3, "wrap": True}),
+         conn("cslave_m", "k", "d", "convergent", "mult", dendrite="cc",
+              convolutional=True, arbor=full),
+         conn("cslave_a", "g", "d", "convergent", dendrite="cc", delay=1,
+              convolutional=True, arbor=full),
+         conn("cslave_s", "k", "d", "convergent", "sub", dendrite="cc",
+              convolutional=True, arbor=full),
+         conn("root", "in", "d", "one to one")]
+    life = conn("life", "g", "g", "convergent", convolutional=True,
+                arbor={"field size": 3, "wrap": True},
+                weight={"type": "flat", "weight": 1, "diagonal": False})
+    weights = {c["name"]: st.normal(num_weights(c, L), stream=10 + i,
+                                    std=0.5)
+               for i, c in enumerate(C)}
+    gol0 = (st.bernoulli(9, 0.5, stream=5).astype(np.uint32) << 31)
+    inputs = {("so", "in"): lambda t: st.normal(36, step=t + 1, stream=2),
+              ("so", "k"): lambda t: st.uniform(9, step=t + 1, stream=3,
+                                                low=0.5, high=1.5)}
+    return Case(name, net("so", L, C + [life]), steps, weights=weights,
+                inputs=inputs, init={("so", "g", "output"): gol0},
+                capture=[("so", "d", "output")],
+                final_neuron=[("so", "d", "input")],
+                final_conn=[("chost", "weights")])
+
+
+def _drawn_sources(name, conns):
+    """The source of every synapse the reference drew for the randomized
+    connections ``conns`` of case ``name``, as its fixture records them
+    (tests/golden/make_golden.py); empty before the fixture exists."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "golden", name + ".npz")
+    if not os.path.exists(path):
+        return {}
+    with np.load(path) as f:
+        return {c: f["drawn|" + c].astype(np.int32) for c in conns
+                if "drawn|" + c in f.files}
+
+
+def random_proj(steps=20, workload=124, name="random_proj"):
+    """Randomized one-to-one projections (connection_config.cpp:219,
+    weight_matrix.cpp:356-393): each destination reads one randomly drawn
+    source.  The reference's generator is seeded from std::random_device
+    (util/parallel.cpp:22-26), so its fixture records the sources it drew
+    (read back from its saved state) and the other runs inject those through
+    an indices callback (weight_matrix.cpp:871-890).  From a relay layer
+    (FLOAT, add) and a game-of-life layer (BIT, sub) into a relay layer;
+    weights include zeros, which the sparse matrix drops."""
+    st = Stream(workload)
+    L = [layer("in", "relay", 6, 6), layer("g", "game_of_life", 6, 6),
+         layer("d", "relay", 6, 6)]
+    C = [conn("rf", "in", "d", "one to one", randomized_projection=True),
+         conn("rb", "g", "d", "one to one", "sub",
+              randomized_projection=True)]
+    life = conn("life", "g", "g", "convergent", convolutional=True,
+                arbor={"field size": 3, "wrap": True},
+                weight={"type": "flat", "weight": 1, "diagonal": False})
+    weights = {"rf": st.normal(36, stream=10, std=0.7),
+               "rb": np.where(st.uniform(36, stream=11) < 0.25, 0.0,
+                              st.uniform(36, stream=12, low=0.2,
+                                         high=1.0)).astype(np.float32)}
+    gol0 = (st.bernoulli(36, 0.5, stream=5).astype(np.uint32) << 31)
+    inputs = {("rp", "in"): lambda t: st.normal(36, step=t + 1, stream=2)}
+    return Case(name, net("rp", L, C + [life]), steps, weights=weights,
+                indices=_drawn_sources(name, ("rf", "rb")),
+                drawn=("rf", "rb"), inputs=inputs,
+                init={("rp", "g", "output"): gol0},
+                capture=[("rp", "d", "output"), ("rp", "d", "input")],
+                final_neuron=[("rp", "d", "input")],
+                final_conn=[(c, "weights") for c in ("rf", "rb")])
+
+
 def sparse_cases():
     return [sparse_base(), sparse_iz()]
 
@@ -720,7 +810,7 @@ def all_cases():
                      workload=107),
             iz_ops(), rate(), iz_init(), history(), gol_rule(),
             relay_ramp()] + ragged_cases() + cluster_cases() + sparse_cases() + \
-        [dendrites(), second_order()]
+        [dendrites(), second_order(), second_order_conv(), random_proj()]
 
 
 # -- the benchmark configurations at their stated size and length ------------

@@ -15,8 +15,8 @@ GPU, through the C-ABI.
   spike-divergence rate (differing spike bits over neurons x steps) is 0
   exactly when every step's words are identical; the test prints it.
 * The batched rate ensemble as stated: 256 seeded instances, widths
-  U{128..4096}, 500 steps in one graph launch with the autotuned launch
-  shape, against the FP64 model on 8 instances including the widest
+  U{128..4096}, 500 steps in one persistent launch (captured in a graph)
+  with the autotuned launch shape, against the FP64 model on 8 instances including the widest
   (max abs error 2e-5), and its leak-1 rectified subset at full width
   against the C restatement of the reference's serial kernels.
 """
@@ -123,8 +123,12 @@ def test_ensemble_config_256_instances_500_steps():
     e = RateEnsemble(widths, "bf16", leak=0.1)
     rt, ks, ms = e.autotune(steps=3)
     assert (rt, ks) == e.tile and ms > 0
-    e.run(500, graph=True)
+    e.run(500, graph=True, persistent=True)
     e.synchronize()
+    blocks, captured, timeouts = e.persistent_info()
+    print("500 steps in one persistent launch: %d blocks, from a graph: %d"
+          % (blocks, captured), flush=True)
+    assert timeouts == 0
     worst = 0.0
     picks = _config_instances(widths)
     assert len(picks) == 8

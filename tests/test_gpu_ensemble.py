@@ -154,3 +154,32 @@ def test_autotune_restores_state():
     for (a1, a2), (b1, b2) in zip(before, _states(e)):
         assert a1.tobytes() == b1.tobytes() and a2.tobytes() == b2.tobytes()
     e.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [False, True])
+def test_persistent_launch_matches_step_launches(graph):
+    """Every step in one cooperative launch (work queue, grid barrier
+    between steps) computes each row with the same code in the same order
+    as one launch per step: bit for bit, over runs of odd and even length
+    with host reads between them."""
+    print("case ensemble persistent graph=%s" % graph, flush=True)
+    p = RateEnsemble(RAGGED, "bf16", leak=0.1)
+    q = RateEnsemble(RAGGED, "bf16", leak=0.1)
+    for steps in (7, 12, 1, 30):
+        p.run(steps, graph=graph, persistent=True)
+        q.run(steps, graph=False)
+        for (a1, a2), (b1, b2) in zip(_states(p), _states(q)):
+            assert a1.tobytes() == b1.tobytes(), steps
+            assert a2.tobytes() == b2.tobytes(), steps
+    blocks, captured, timeouts = p.persistent_info()
+    assert blocks >= 1 and timeouts == 0
+    print("persistent grid %d blocks, from a graph: %d" % (blocks, captured),
+          flush=True)
+    assert np.abs(p.state(5, 2)).max() > 0.05
+    p.close()
+    q.close()
+    f = RateEnsemble(RAGGED[:2], "fp32")
+    with pytest.raises(Exception):
+        f.run(3, persistent=True)
+    f.close()

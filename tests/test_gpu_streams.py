@@ -151,7 +151,8 @@ def test_step_graphs_match_eager_steps(gpu_lib):
     assert reps[0].get("step graphs") == "0"
     for block in ("2", "8", "16"):
         got, reps = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
-                                 [STEPS], {"graph steps": block})
+                                 [STEPS], {"step graphs": "true",
+                                           "graph steps": block})
         assert int(reps[0].get("step graphs")) > 0, block
         _same(eager, got)
     assert np.unpackbits(eager[("E", "output")].view(np.uint8)).sum() > 0
@@ -167,10 +168,10 @@ def test_runs_in_parts_match_one_run(gpu_lib):
     whole, _ = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
                             [STEPS])
     parts, reps = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
-                               [7, 13, 20])
+                               [7, 13, 20], {"step graphs": "true"})
     _same(whole, parts)
     eager, _ = _state_after(gpu_lib, wl, "sheet", ("E", "I"), ("EE",),
-                            [7, 13, 20], {"step graphs": "false"})
+                            [7, 13, 20])
     _same(whole, eager)
 
 
