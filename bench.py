@@ -249,18 +249,21 @@ def run(network, steps, extra=None):
 
 def bench_izfc_io(a, lib):
     """The Izhikevich 1000 -> 1000 network with host I/O every step: layer
-    1's N(5, 2) input current from an input callback and layer 2's spike
-    words to an output callback, both native C functions
-    (libsg_io_native.so), against the same network with the input drawn on
-    the device and no callbacks.  The engine overlaps each step's output
-    callback and the next step's input callback with the next step's first
-    launches; the ratio says what the I/O costs per step."""
+    1's input current from an input callback that replays a recorded N(5,
+    2) stimulus (256 rows, cycled) and layer 2's spike words to a checksum
+    output callback, both native C functions (libsg_io_native.so), against
+    the same network with the input drawn on the device and no callbacks.
+    The engine issues each step's first launches before the previous
+    step's output callback and copies with kernels on the layers' own
+    streams; the ratio says what the I/O costs per step."""
     io = ctypes.CDLL(os.path.join(ROOT, "synaptogenesis_amd",
                                   "libsg_io_native.so"))
     io.sg_io_configure.argtypes = [ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double]
     io.sg_io_checksum.restype = ctypes.c_uint64
-    for name in ("sg_io_normal_input", "sg_io_checksum_output"):
+    io.sg_io_set_table.argtypes = [ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_int, ctypes.c_int]
+    for name in ("sg_io_table_input", "sg_io_checksum_output"):
         lib.add_io_callback(name.encode(), ctypes.cast(
             getattr(io, name), ctypes.c_void_p).value)
     steps = a.steps or 1000
@@ -276,13 +279,20 @@ def bench_izfc_io(a, lib):
     net = build(lib, wl)
     env = {"modules": [{"type": "callback", "layers": [
         {"structure": "izfc", "layer": "L1", "input": "true",
-         "function": "sg_io_normal_input", "id": "0"},
+         "function": "sg_io_table_input", "id": "0"},
         {"structure": "izfc", "layer": "L2", "output": "true",
          "function": "sg_io_checksum_output", "id": "1",
          "output keys": ["output"]}]}]}
     args = {"worker threads": "0", "multithreaded": "false",
             "learning flag": "true", "verbose": "false"}
-    io.sg_io_configure(0, 5.0, 2.0)
+    rows = 256
+    size = int(wl["net"]["structures"][0]["layers"][0]["rows"]) * \
+        int(wl["net"]["structures"][0]["layers"][0]["columns"])
+    table = np.ascontiguousarray(
+        np.random.default_rng(7).normal(5.0, 2.0, (rows, size)),
+        dtype=np.float32)
+    if io.sg_io_set_table(0, table.ctypes.data, rows, size) != 0:
+        raise RuntimeError("sg_io_set_table failed")
     io.sg_io_configure(1, 0.0, 0.0)
     net.run(env, dict(args, iterations=str(a.warmup)))
     t0 = time.perf_counter()
@@ -298,8 +308,9 @@ def bench_izfc_io(a, lib):
         "unit": "synaptic updates/s", "n_gpus": 1, "steps": steps,
         "warmup": a.warmup, "ms_per_step": ms1, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic: native N(5,2) input callback, spike words to a "
-                "native checksum callback, every step",
+        "data": "synthetic: a recorded N(5,2) stimulus replayed by a native "
+                "input callback, spike words to a native checksum callback, "
+                "every step",
         "config": {"workload": wl["name"] + " with per-step input and "
                                            "output callbacks"},
         "no_io_ms_per_step": ms0, "io_over_no_io": ms1 / ms0,
@@ -401,7 +412,7 @@ def bench_ensemble(a):
     t0 = time.perf_counter()
     e.run(steps, graph=True, persistent=True)
     e.synchronize()
-    dt = time.perf_counter() - t0
+    dt_pers = time.perf_counter() - t0
     blocks, captured, timeouts = e.persistent_info()
     if timeouts:
         raise RuntimeError("persistent run: %d grid-barrier timeouts"
@@ -413,6 +424,9 @@ def bench_ensemble(a):
     e.run(steps, graph=True)
     e.synchronize()
     dt_steps = time.perf_counter() - t0
+    # the headline is the faster of the two forms (same results)
+    persistent = dt_pers <= dt_steps
+    dt = dt_pers if persistent else dt_steps
     kernel_ms = e.time_steps(20)
     c = e.counts()
     e.close()
@@ -450,14 +464,18 @@ def bench_ensemble(a):
         "config": {"workload": "batched rate ensemble: 256 instances, "
                                "x0 -> h1 -> h2 fully connected, widths "
                                "U{128..4096}, leak 0.1, tanh, %d steps in "
-                               "one persistent launch%s" % (
-                                   steps, " (HIP graph node)" if captured
-                                   else ""),
+                               "%s" % (
+                                   steps,
+                                   ("one persistent launch" +
+                                    (" (HIP graph node)" if captured
+                                     else "")) if persistent else
+                                   "one HIP graph of one launch per step"),
                    "synapses_per_step": syn,
                    "neurons": c["neurons"],
                    "launch_shape": {"row_tiles": rt, "k_split": ks,
                                     "persistent_blocks": blocks}},
         "step_launches_ms_per_step": dt_steps * 1000.0 / steps,
+        "persistent_ms_per_step": dt_pers * 1000.0 / steps,
         "neuron_steps_per_s": c["neurons"] * steps / dt,
         "roofline": roofline_hbm("ens_bf16_kernel", kernel_ms,
                                  c["algorithmic_bytes"],

@@ -361,6 +361,13 @@ int sg_hip_record32(const void *src, void *dst, int slots, int K, int field,
  * byte counters). */
 int sg_hip_copy32(const void *src, void *dst, long n, void *stream);
 
+/* Copy of n 32-bit words by a kernel on `stream`, any n, 4-byte aligned
+ * pointers; either side may be pinned host memory mapped into the device's
+ * address space (hipHostGetDevicePointer).  The engine's per-step
+ * environment I/O: a kernel in the consuming stream's order instead of a
+ * DMA copy on a stream of its own and the events between them. */
+int sg_hip_copy_u32(const void *src, void *dst, long n, void *stream);
+
 /* Device properties: number of visible devices, -1 on error. */
 int sg_hip_device_count(void);
 

@@ -5,7 +5,7 @@
 // steps are one GEMM and each block of `checkpoint` steps (M = checkpoint *
 // batch rows, row = step * batch + b) runs as parallel passes with only
 // elementwise recurrences in time:
-//   forward   gemm_nt      A1[T*batch][n_hid] = X . W1^T for every step
+//   forward   gemm_fwd     A1[T*batch][n_hid] = X . W1^T for every step
 //                          (MFMA BF16), kept for the backward
 //             T <= 1024: scan1_fwd_all and out_scan2_fwd_all, the passes
 //             below over all steps in two launches; else per block:
@@ -30,7 +30,8 @@
 //             dw2_partial  D2^T . S1 over 16-row chunks, accumulated over
 //                          the blocks
 //             and after the last block
-//             gemm_tn      dW1 = D1^T . X over every step's rows (MFMA
+//             pack_x       x regrouped by eight rows for the product
+//             gemm_dw1     dW1 = D1^T . X over every step's rows (MFMA
 //                          BF16, D1 as hi + mid + lo BF16 terms), split
 //                          into row ranges
 // and add_parts / dw2_reduce add the ranges and chunks in a fixed order into dW1
@@ -96,152 +97,566 @@ __device__ __forceinline__ u32x4 load8(const uint16_t *p, int rows, int cols,
     return v;
 }
 
-// C[M][N] = A[M][K] . B[N][K]^T, BF16 operands, FP32 result.  64 x 64
-// tile per block, four wavefronts of 32 x 32 (2 x 2 MFMA 16x16x32 tiles),
-// 64 columns of K per barrier pair; the next slice's global loads are in
-// flight while the current one is multiplied.
-__global__ __launch_bounds__(256) void gemm_nt(const uint16_t *__restrict__ A,
-                                               const uint16_t *__restrict__ B,
-                                               float *__restrict__ C, int M,
-                                               int N, int K) {
-    __shared__ __align__(16) uint16_t As[64][72];
-    __shared__ __align__(16) uint16_t Bs[64][72];
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+// C[M][N] = X[M][K] . W[N][K]^T, BF16 operands, FP32 result (the input
+// currents of every step).  Block tile 128 (m) x 256 (n): four wavefronts
+// of 64 x 128, each 4 x 8 MFMA 16x16x32 tiles; K in slices of 32 through
+// double-buffered LDS (row pitch 40 BF16 = 80 B: the 16 rows a fragment
+// read touches start on 16 different 4-bank groups), the next slice's
+// global loads in flight during the current slice's MFMAs.  A wavefront
+// reads 12 fragments per 32 MFMAs.
+constexpr int kFm = 128, kFn = 256, kFk = 32, kFp = 40;
+
+__global__ __launch_bounds__(256) void gemm_fwd(const uint16_t *__restrict__ X,
+                                                const uint16_t *__restrict__ W,
+                                                float *__restrict__ C, int M,
+                                                int N, int K) {
+    __shared__ __align__(16) uint16_t As[2][kFm][kFp];
+    __shared__ __align__(16) uint16_t Bs[2][kFn][kFp];
+    const int n0 = blockIdx.x * kFn, m0 = blockIdx.y * kFm;
     const int t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
-    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int wm = (w & 1) * 64, wn = (w >> 1) * 128;
     const int li = lane & 15, lk = (lane >> 4) * 8;
-    // chunk i of this thread: row (t + 256 i) / 8, columns ((t + 256 i) % 8) * 8
-    const int r0 = t >> 3, r1 = (t + 256) >> 3, kc = (t & 7) * 8;
-    u32x4 a0 = load8(A, M, K, m0 + r0, kc), a1 = load8(A, M, K, m0 + r1, kc);
-    u32x4 b0 = load8(B, N, K, n0 + r0, kc), b1 = load8(B, N, K, n0 + r1, kc);
-    f32x4 acc[2][2] = {};
-    for (int k0 = 0; k0 < K; k0 += 64) {
-        *(u32x4 *)&As[r0][kc] = a0;
-        *(u32x4 *)&As[r1][kc] = a1;
-        *(u32x4 *)&Bs[r0][kc] = b0;
-        *(u32x4 *)&Bs[r1][kc] = b1;
-        __syncthreads();
-        if (k0 + 64 < K) {
-            a0 = load8(A, M, K, m0 + r0, k0 + 64 + kc);
-            a1 = load8(A, M, K, m0 + r1, k0 + 64 + kc);
-            b0 = load8(B, N, K, n0 + r0, k0 + 64 + kc);
-            b1 = load8(B, N, K, n0 + r1, k0 + 64 + kc);
+    // staging: 16-byte chunk q = t + 256 i is row q / 4, columns (q % 4) * 8
+    u32x4 ra[2], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = t + 256 * i;
+            ra[i] = load8(X, M, K, m0 + (q >> 2), k0 + (q & 3) * 8);
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            bf16x8 a[2], b[2];
-            for (int i = 0; i < 2; ++i)
-                a[i] = *(const bf16x8 *)&As[wm + i * 16 + li][h * 32 + lk];
-            for (int j = 0; j < 2; ++j)
-                b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + li][h * 32 + lk];
-            for (int i = 0; i < 2; ++i)
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < 4; ++i) {
+            const int q = t + 256 * i;
+            rb[i] = load8(W, N, K, n0 + (q >> 2), k0 + (q & 3) * 8);
         }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = t + 256 * i;
+            *(u32x4 *)&As[buf][q >> 2][(q & 3) * 8] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = t + 256 * i;
+            *(u32x4 *)&Bs[buf][q >> 2][(q & 3) * 8] = rb[i];
+        }
+    };
+    f32x4 acc[4][8] = {};
+    const int ns = (K + kFk - 1) / kFk;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int sl = 0; sl < ns; ++sl) {
+        const int buf = sl & 1;
+        if (sl + 1 < ns) load((sl + 1) * kFk);
+        bf16x8 a[4], b[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            a[i] = *(const bf16x8 *)&As[buf][wm + 16 * i + li][lk];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            b[j] = *(const bf16x8 *)&Bs[buf][wn + 16 * j + li][lk];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a[i], b[j], acc[i][j], 0, 0, 0);
+        if (sl + 1 < ns) store(buf ^ 1);
         __syncthreads();
     }
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn + j * 16 + li;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = n0 + wn + 16 * j + li;
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+                const int row = m0 + wm + 16 * i + (lane >> 4) * 4 + r;
                 if (row < M && col < N) C[(long)row * N + col] = acc[i][j][r];
             }
         }
 }
 
-// part[z][N][K] = L[rows of range z][N]^T . X[rows of range z][K]: FP32 L
-// as three BF16 terms, BF16 X.  64 (n) x 64 (k) tile per block, the M rows
-// split into gridDim.z contiguous ranges.
-__global__ __launch_bounds__(256) void gemm_tn(const float *__restrict__ L,
-                                               const uint16_t *__restrict__ X,
-                                               float *__restrict__ part, int M,
-                                               int N, int K) {
-    __shared__ __align__(16) uint16_t Ah[3][64][40];
-    __shared__ __align__(16) uint16_t Bs[64][40];
-    const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64;
-    const int per = (M + gridDim.z - 1) / gridDim.z;
-    const int rb = blockIdx.z * per, re = min(M, rb + per);
+// X [M][K] regrouped for the dW1 product, whose reduction runs over the
+// rows: xp[g][k][0..7] = X[8 g .. 8 g + 7][k], for g < gp (rows past M are
+// 0; gp = the groups rounded up to whole 32-row slices), so one 16-byte
+// read is the eight reduction values an MFMA operand lane holds.  Thread =
+// (group, 8 columns).
+__global__ __launch_bounds__(256) void pack_x(const uint16_t *__restrict__ X,
+                                              uint16_t *__restrict__ xp,
+                                              int M, int K, long gp) {
+    const int nc = (K + 7) / 8;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= gp * nc) return;
+    const int g = (int)(i / nc), c = (int)(i % nc);
+    u32x4 r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = load8(X, M, K, 8 * g + j, 8 * c);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int k = 8 * c + e;
+        if (k >= K) break;
+        u32x4 o;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const uint32_t lo = (r[2 * p][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+            const uint32_t hi = (r[2 * p + 1][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+            o[p] = lo | (hi << 16);
+        }
+        *(u32x4 *)&xp[((long)g * K + k) * 8] = o;
+    }
+}
+
+// part[z][H][K] = L[rows of range z][H]^T . X[rows of range z][K] (dW1):
+// L FP32 enters as three BF16 terms hi + mid + lo, so the product keeps
+// FP32 precision; X through xp (pack_x).  Block tile 128 (h) x 112 (k),
+// four wavefronts of 32 x 112 (2 x 7 MFMA 16x16x32 tiles, three MFMAs per
+// tile and slice), 32 rows per slice through double-buffered LDS.  An
+// operand lane needs eight consecutive rows of one column of L: it reads
+// them from the LDS tile with eight 4-byte reads, and the tile's row groups
+// of eight are stored with their columns XOR-swizzled by 16 per group, so
+// that the four groups a read touches fall in four different bank ranges.
+// Blocks are numbered so that each XCD (hardware block id modulo 8) gets
+// whole row ranges, all k tiles of an h tile in a row: a range's rows of L
+// and X are fetched into that XCD's L2 once and reused there.
+constexpr int kDh = 128, kDk = 112, kDr = 32;
+
+__device__ __forceinline__ void split3(const float (&v)[8], bf16x8 &hi,
+                                       bf16x8 &mid, bf16x8 &lo) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const __bf16 h = (__bf16)v[j];
+        const float r1 = v[j] - (float)h;
+        const __bf16 m = (__bf16)r1;
+        hi[j] = h;
+        mid[j] = m;
+        lo[j] = (__bf16)(r1 - (float)m);
+    }
+}
+
+__global__ __launch_bounds__(256) void gemm_dw1(const float *__restrict__ L,
+                                                const uint16_t *__restrict__ xp,
+                                                float *__restrict__ part,
+                                                int M, int H, int K, int per) {
+    __shared__ __align__(16) float Ls[2][kDr][kDh];
+    __shared__ __align__(16) uint16_t Xs[2][4][kDk][8];
+    // block numbering: logical = xcd * (blocks / 8) + local, k tile fastest
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int nb = nx * ny * gridDim.z;
+    const int hw = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const int lg = (nb % 8 == 0) ? (hw % 8) * (nb / 8) + hw / 8 : hw;
+    const int bx = lg % nx, by = (lg / nx) % ny, bz = lg / (nx * ny);
+    const int h0 = by * kDh, kb = bx * kDk;
+    const int rb = bz * per, re = min(M, rb + per);
+    const long G = (M + 7) / 8;
     const int t = threadIdx.x;
     const int w = t >> 6, lane = t & 63;
-    const int wn = (w >> 1) * 32, wk = (w & 1) * 32;
-    const int li = lane & 15, lk = (lane >> 4) * 8;
-    // staging: thread (row pair rp, column c) loads rows rp, rp + 1 of
-    // columns c, c + 16, c + 32, c + 48 of L and X (coalesced across the 16
-    // threads of a row) and stores each column's two rows as one 32-bit
-    // word of the transposed tiles; neighbouring threads take neighbouring
-    // tile rows, so the stores spread over the LDS banks
-    const int rp = (t >> 4) * 2, nc = t & 15;
-    f32x4 acc[2][2] = {};
-    float lv[2][4];
-    uint16_t xv[2][4];
-    // the staged values of rows r0 .. r0 + 31 (zeros past the range)
+    const int li = lane & 15, gq = lane >> 4;
+    const bool vec = (H & 3) == 0;
+    float4 rl[4];
+    u32x4 rx[2];
+    // staging: L chunk q = t + 256 i is row q / 32, columns 4 (q % 32);
+    // X chunk q = t + 256 i (q < 448) is group q / 112, column q % 112
     auto load = [&](int r0) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int r = r0 + rp + h;
-            const bool in = r < re;
-            const float *lrow = L + (long)r * N + n0 + nc;
-            const uint16_t *xrow = X + (long)r * K + k0 + nc;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                lv[h][e] = (in && n0 + nc + 16 * e < N) ? lrow[16 * e] : 0.0f;
-                xv[h][e] = (in && k0 + nc + 16 * e < K) ? xrow[16 * e]
-                                                        : (uint16_t)0;
+        for (int i = 0; i < 4; ++i) {
+            const int q = t + 256 * i;
+            const int row = r0 + (q >> 5), h = h0 + 4 * (q & 31);
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (row < re) {
+                const float *src = L + (long)row * H + h;
+                if (vec && h + 3 < H) {
+                    v = *(const float4 *)src;
+                } else {
+                    if (h < H) v.x = src[0];
+                    if (h + 1 < H) v.y = src[1];
+                    if (h + 2 < H) v.z = src[2];
+                    if (h + 3 < H) v.w = src[3];
+                }
             }
+            rl[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = t + 256 * i;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (q < 4 * kDk) {
+                const long g = r0 / 8 + q / kDk;
+                const int k = kb + q % kDk;
+                if (g < G && k < K) v = *(const u32x4 *)&xp[(g * K + k) * 8];
+            }
+            rx[i] = v;
         }
     };
-    load(rb);
-    for (int r0 = rb; r0 < re; r0 += 32) {
-        for (int e = 0; e < 4; ++e) {
-            uint32_t term[2][3];
-            for (int h = 0; h < 2; ++h) {
-                const float v = lv[h][e];
-                const uint32_t hi = bf16_rne(v);
-                const float rem = v - bf16_value(hi);
-                const uint32_t mid = bf16_rne(rem);
-                term[h][0] = hi;
-                term[h][1] = mid;
-                term[h][2] = bf16_rne(rem - bf16_value(mid));
-            }
-            for (int q = 0; q < 3; ++q)
-                *(uint32_t *)&Ah[q][nc + 16 * e][rp] =
-                    term[0][q] | (term[1][q] << 16);
-            *(uint32_t *)&Bs[nc + 16 * e][rp] =
-                (uint32_t)xv[0][e] | ((uint32_t)xv[1][e] << 16);
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = t + 256 * i;
+            const int row = q >> 5;
+            const int col = (4 * (q & 31)) ^ (16 * ((row >> 3) & 3));
+            *(float4 *)&Ls[buf][row][col] = rl[i];
         }
-        __syncthreads();
-        // the next rows' loads in flight during this tile's MFMAs
-        if (r0 + 32 < re) load(r0 + 32);
-        bf16x8 b[2];
-        for (int j = 0; j < 2; ++j)
-            b[j] = *(const bf16x8 *)&Bs[wk + j * 16 + li][lk];
-        for (int q = 0; q < 3; ++q)
-            for (int i = 0; i < 2; ++i) {
-                const bf16x8 a = *(const bf16x8 *)&Ah[q][wn + i * 16 + li][lk];
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        a, b[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = t + 256 * i;
+            if (q < 4 * kDk)
+                *(u32x4 *)&Xs[buf][q / kDk][q % kDk][0] = rx[i];
+        }
+    };
+    f32x4 acc[2][7] = {};
+    load(rb);
+    store(0);
+    __syncthreads();
+    for (int r0 = rb; r0 < re; r0 += kDr) {
+        const int buf = ((r0 - rb) / kDr) & 1;
+        const bool more = r0 + kDr < re;
+        if (more) load(r0 + kDr);
+        bf16x8 b[7];
+#pragma unroll
+        for (int n = 0; n < 7; ++n)
+            b[n] = *(const bf16x8 *)&Xs[buf][gq][16 * n + li][0];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int hl = (32 * w + 16 * i + li) ^ (16 * gq);
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = Ls[buf][8 * gq + j][hl];
+            bf16x8 hi, mid, lo;
+            split3(v, hi, mid, lo);
+#pragma unroll
+            for (int n = 0; n < 7; ++n) {
+                acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    hi, b[n], acc[i][n], 0, 0, 0);
+                acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    mid, b[n], acc[i][n], 0, 0, 0);
+                acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    lo, b[n], acc[i][n], 0, 0, 0);
             }
+        }
+        if (more) store(buf ^ 1);
         __syncthreads();
     }
-    float *P = part + (long)blockIdx.z * N * K;
+    float *P = part + (long)bz * H * K;
+#pragma unroll
     for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) {
-            const int col = k0 + wk + j * 16 + li;
-            for (int q = 0; q < 4; ++q) {
-                const int row = n0 + wn + i * 16 + (lane >> 4) * 4 + q;
-                if (row < N && col < K) P[(long)row * K + col] = acc[i][j][q];
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+            const int k = kb + 16 * n + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int h = h0 + 32 * w + 16 * i + 4 * gq + r;
+                if (h < H && k < K) P[(long)h * K + k] = acc[i][n][r];
             }
         }
 }
 
-// D[i] = part[0][i] + part[1][i] + ... in range order.
+// ---- LDS-DMA ring forms ------------------------------------------------------
+// The same two products with their operand slices brought into LDS by the
+// loads themselves (global_load_lds_dwordx4: each lane's 16 bytes land at
+// the wavefront's LDS base + 16 * lane), three slices in a ring, two in
+// flight while the third is multiplied: the slices' memory latency is
+// hidden behind two slices of MFMAs instead of one.  Each wavefront waits
+// for its own loads of a slice (vmcnt), then a barrier makes every
+// wavefront's part visible; a second barrier keeps a slot from being
+// refilled before every wavefront has read it.  The barriers are bare
+// s_barrier: the waits are explicit (a workgroup fence would wait for the
+// slices in flight too).  Out-of-range rows and columns are read from a
+// clamped in-range address; what they feed is either never stored or
+// multiplied by the zeros the other operand holds there.
+
+constexpr int kRing = 3;
+
+// s_waitcnt immediate (gfx9 encoding): vmcnt <= n, no LDS/scalar or export
+// wait.
+#define SG_WAIT_VM(n) \
+    __builtin_amdgcn_s_waitcnt(((n) & 15) | (7 << 4) | (15 << 8) | (((n) >> 4) << 14))
+// every LDS read of this wavefront has returned (before a slot it read may
+// be refilled)
+#define SG_WAIT_LDS() __builtin_amdgcn_s_waitcnt(15 | (7 << 4) | (3 << 14))
+// no memory access moves across it in the compiler (no code)
+#define SG_COMPILER_FENCE() asm volatile("" ::: "memory")
+#define SG_BARRIER()                   \
+    do {                               \
+        SG_COMPILER_FENCE();           \
+        __builtin_amdgcn_s_barrier();  \
+        SG_COMPILER_FENCE();           \
+    } while (0)
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// 16 bytes per lane from `src` into LDS at the wavefront's `lds_base` + 16
+// * lane.  Written as the instruction itself rather than the builtin: the
+// compiler tracks only a few such loads by destination and otherwise makes
+// every LDS read wait for all of them, which would drain the ring at each
+// slice; here the waits are the explicit vmcnt ones above.  The
+// instruction returns nothing to registers.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void dma16(const void *src, void *lds_base) {
+    const uint32_t off = (uint32_t)(uintptr_t)(lds_void *)lds_base;
+    const uint32_t m0v = __builtin_amdgcn_readfirstlane(off);
+    asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+                 :: "v"(src), "s"(m0v) : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+// Forward currents, K a multiple of 8 (16-byte rows).  Slices of 32
+// columns: a row of a slice is four 16-byte chunks, stored at chunk
+// c ^ ((row >> 2) & 3) so that the 16 rows a fragment read touches spread
+// over the banks.  Blocks that share an m tile run back to back on one XCD.
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
+        const uint16_t *__restrict__ X, const uint16_t *__restrict__ W,
+        float *__restrict__ C, int M, int N, int K) {
+    // one variable per ring slot: the compiler then sees that reading one
+    // slot does not wait for the loads still landing in the others
+    __shared__ __align__(16) uint16_t A0[kFm * kFk], A1[kFm * kFk],
+        A2[kFm * kFk];
+    __shared__ __align__(16) uint16_t B0[kFn * kFk], B1[kFn * kFk],
+        B2[kFn * kFk];
+    auto As = [&](int slot) { return slot == 0 ? A0 : slot == 1 ? A1 : A2; };
+    auto Bs = [&](int slot) { return slot == 0 ? B0 : slot == 1 ? B1 : B2; };
+    const int nx = gridDim.x, nb = gridDim.x * gridDim.y;
+    const int hw = blockIdx.x + nx * blockIdx.y;
+    const int lg = (nb % 8 == 0) ? (hw % 8) * (nb / 8) + hw / 8 : hw;
+    const int n0 = (lg % nx) * kFn, m0 = (lg / nx) * kFm;
+    const int t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const int wm = (w & 1) * 64, wn = (w >> 1) * 128;
+    const int li = lane & 15, lc = lane >> 4;
+    // this lane's DMA rows (fixed) and chunk within the row
+    const int dr = lane >> 2;
+    const int dc = 8 * ((lane & 3) ^ ((dr >> 2) & 3));
+    const uint16_t *asrc[2], *bsrc[4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        asrc[i] = X + (long)min(m0 + 16 * (2 * w + i) + dr, M - 1) * K + dc;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        bsrc[i] = W + (long)min(n0 + 16 * (4 * w + i) + dr, N - 1) * K + dc;
+    auto issue = [&](int sl, int slot) {
+        const int k0 = sl * kFk;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            dma16(asrc[i] + k0, As(slot) + (2 * w + i) * 512);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            dma16(bsrc[i] + k0, Bs(slot) + (4 * w + i) * 512);
+    };
+    f32x4 acc[4][8] = {};
+    auto compute = [&](int slot) {
+        const uint16_t *as = As(slot), *bs = Bs(slot);
+        bf16x8 a[4], b[8];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = wm + 16 * i + li;
+            a[i] = *(const bf16x8 *)&as[r * kFk + 8 * (lc ^ ((r >> 2) & 3))];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int r = wn + 16 * j + li;
+            b[j] = *(const bf16x8 *)&bs[r * kFk + 8 * (lc ^ ((r >> 2) & 3))];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+    const int ns = K / kFk;   // whole slices through the ring
+    auto stage = [&](int sl, int slot) {
+        if (sl + 2 < ns) {
+            if (sl > 0) {
+                SG_COMPILER_FENCE();
+                SG_WAIT_LDS();
+                SG_BARRIER();
+            }
+            issue(sl + 2, (slot + 2) % kRing);
+            SG_WAIT_VM(12);
+        } else {
+            SG_WAIT_VM(0);
+        }
+        SG_BARRIER();
+        compute(slot);
+    };
+    if (ns > 0) issue(0, 0);
+    if (ns > 1) issue(1, 1);
+    for (int sl = 0; sl < ns; sl += kRing) {
+        stage(sl, 0);
+        if (sl + 1 < ns) stage(sl + 1, 1);
+        if (sl + 2 < ns) stage(sl + 2, 2);
+    }
+    if (K % kFk) {
+        // the last partial slice through registers (columns past K are 0)
+        __syncthreads();
+        const int k0 = ns * kFk;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int q = t + 256 * i, r = q >> 2;
+            *(u32x4 *)&A0[r * kFk + 8 * ((q & 3) ^ ((r >> 2) & 3))] =
+                load8(X, M, K, m0 + r, k0 + (q & 3) * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int q = t + 256 * i, r = q >> 2;
+            *(u32x4 *)&B0[r * kFk + 8 * ((q & 3) ^ ((r >> 2) & 3))] =
+                load8(W, N, K, n0 + r, k0 + (q & 3) * 8);
+        }
+        __syncthreads();
+        compute(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = n0 + wn + 16 * j + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = m0 + wm + 16 * i + lc * 4 + r;
+                if (row < M && col < N) C[(long)row * N + col] = acc[i][j][r];
+            }
+        }
+}
+
+// dW1, H a multiple of 4.  A slice is 32 rows of L (row = 128 FP32 in
+// 4-float units stored at unit u ^ (4 * ((row >> 3) & 3))) and the xp
+// entries of its four groups ([4][112][8], 7 KB, padded to 8 KB so that
+// every wavefront loads two chunks).  Row ranges are whole slices (per is a
+// multiple of 32); rows past M read row M - 1 of L, against xp's zeros.
+__global__ __launch_bounds__(256) void gemm_dw1_dma(
+        const float *__restrict__ L, const uint16_t *__restrict__ xp,
+        float *__restrict__ part, int M, int H, int K, int per, long gp) {
+    __shared__ __align__(16) float L0[kDr * kDh], L1[kDr * kDh],
+        L2[kDr * kDh];
+    __shared__ __align__(16) uint16_t X0[8 * 512], X1[8 * 512], X2[8 * 512];
+    auto Ls = [&](int slot) { return slot == 0 ? L0 : slot == 1 ? L1 : L2; };
+    auto Xs = [&](int slot) { return slot == 0 ? X0 : slot == 1 ? X1 : X2; };
+    const int nx = gridDim.x, ny = gridDim.y;
+    const int nb = nx * ny * gridDim.z;
+    const int hw = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    const int lg = (nb % 8 == 0) ? (hw % 8) * (nb / 8) + hw / 8 : hw;
+    const int bx = lg % nx, by = (lg / nx) % ny, bz = lg / (nx * ny);
+    const int h0 = by * kDh, kb = bx * kDk;
+    const int rb = bz * per, re = min(M, rb + per);
+    const int t = threadIdx.x;
+    const int w = t >> 6, lane = t & 63;
+    const int li = lane & 15, gq = lane >> 4;
+    // DMA sources: L chunk j = 4 w + i holds rows 2 j, 2 j + 1 of the slice
+    int lrow[4], lcol[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = 2 * (4 * w + i) + (lane >> 5);
+        const int u = (lane & 31) ^ (4 * ((row >> 3) & 3));
+        lrow[i] = row;
+        lcol[i] = min(h0 + 4 * u, H - 4);
+    }
+    // xp chunk j = 2 w + i: entry q = 64 j + lane = group q / 112, column
+    // q % 112 (q >= 448: a spare slot, loaded from a clamped address)
+    int xg[2], xk[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int q = 64 * (2 * w + i) + lane;
+        xg[i] = q < 4 * kDk ? q / kDk : 3;
+        xk[i] = min(kb + (q < 4 * kDk ? q % kDk : 0), K - 1);
+    }
+    auto issue = [&](int r0, int slot) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long row = min(r0 + lrow[i], M - 1);
+            dma16(L + row * H + lcol[i], Ls(slot) + (4 * w + i) * 256);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const long g = min((long)(r0 / 8 + xg[i]), gp - 1);
+            dma16(xp + (g * K + xk[i]) * 8, Xs(slot) + (2 * w + i) * 512);
+        }
+    };
+    f32x4 acc[2][7] = {};
+    auto compute = [&](int slot) {
+        const float *ls = Ls(slot);
+        const uint16_t *xs = Xs(slot);
+        bf16x8 b[7];
+#pragma unroll
+        for (int n = 0; n < 7; ++n)
+            b[n] = *(const bf16x8 *)&xs[(gq * kDk + 16 * n + li) * 8];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int hl = (32 * w + 16 * i + li) ^ (16 * gq);
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = ls[(8 * gq + j) * kDh + hl];
+            bf16x8 hi, mid, lo;
+            split3(v, hi, mid, lo);
+#pragma unroll
+            for (int n = 0; n < 7; ++n) {
+                acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    hi, b[n], acc[i][n], 0, 0, 0);
+                acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    mid, b[n], acc[i][n], 0, 0, 0);
+                acc[i][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                    lo, b[n], acc[i][n], 0, 0, 0);
+            }
+        }
+    };
+    const int ns = re > rb ? (re - rb + kDr - 1) / kDr : 0;
+    auto stage = [&](int sl, int slot) {
+        if (sl + 2 < ns) {
+            if (sl > 0) {
+                SG_COMPILER_FENCE();
+                SG_WAIT_LDS();
+                SG_BARRIER();
+            }
+            issue(rb + (sl + 2) * kDr, (slot + 2) % kRing);
+            SG_WAIT_VM(12);
+        } else {
+            SG_WAIT_VM(0);
+        }
+        SG_BARRIER();
+        compute(slot);
+    };
+    if (ns > 0) issue(rb, 0);
+    if (ns > 1) issue(rb + kDr, 1);
+    for (int sl = 0; sl < ns; sl += kRing) {
+        stage(sl, 0);
+        if (sl + 1 < ns) stage(sl + 1, 1);
+        if (sl + 2 < ns) stage(sl + 2, 2);
+    }
+    float *P = part + (long)bz * H * K;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+            const int k = kb + 16 * n + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int h = h0 + 32 * w + 16 * i + 4 * gq + r;
+                if (h < H && k < K) P[(long)h * K + k] = acc[i][n][r];
+            }
+        }
+}
+
+// D[i] = part[0][i] + part[1][i] + ... in range order, four at a time
+// where n and D allow it.
 __global__ void add_parts(const float *__restrict__ part, float *__restrict__ D,
                           long n, int parts) {
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+    const long n4 = (n % 4 == 0 && ((uintptr_t)D & 15) == 0) ? n / 4 : 0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4;
+         i += (long)gridDim.x * blockDim.x) {
+        float4 acc = ((const float4 *)part)[i];
+        for (int z = 1; z < parts; ++z) {
+            const float4 v = ((const float4 *)(part + (long)z * n))[i];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+        }
+        ((float4 *)D)[i] = acc;
+    }
+    for (long i = 4 * n4 + blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
          i += (long)gridDim.x * blockDim.x) {
         float acc = part[i];
         for (int z = 1; z < parts; ++z) acc += part[(long)z * n + i];
@@ -819,7 +1234,7 @@ size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 // Workspace carve-up, in bytes from the base.
 struct Layout {
     size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, lam,
-        g2, part2, part1, w2t, total;
+        g2, part2, part1, w2t, xp, total;
 };
 
 // The whole backward runs in one launch (scan_bwd_fused) where each
@@ -859,6 +1274,8 @@ Layout layout(const sg_snn_cfg &c) {
     l.part2 = take(dw2_slots(c) * O * H);
     l.part1 = take((size_t)kTnSplit * H * c.n_in);
     l.w2t = take((H * kMaxOut + 1) / 2);
+    // x regrouped by 8 rows (BF16), groups padded to whole 32-row slices
+    l.xp = take(((MT + 7) / 8 + 3) / 4 * 4 * c.n_in * 4);
     l.total = off;
     return l;
 }
@@ -964,9 +1381,13 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     // input currents of every step in one GEMM (rows t * batch + b); the
     // backward reuses them and recomputes only the recurrences
     const int MT = c.steps * c.batch;
-    hipLaunchKernelGGL(gemm_nt, dim3((c.n_hid + 63) / 64, (MT + 63) / 64),
-                       dim3(256), 0, s, x, w1, (float *)(ws + l.a1), MT,
-                       c.n_hid, c.n_in);
+    const dim3 fgrid((c.n_hid + kFn - 1) / kFn, (MT + kFm - 1) / kFm);
+    if (c.n_in % 8 == 0)
+        hipLaunchKernelGGL(gemm_fwd_dma, fgrid, dim3(256), 0, s, x, w1,
+                           (float *)(ws + l.a1), MT, c.n_hid, c.n_in);
+    else
+        hipLaunchKernelGGL(gemm_fwd, fgrid, dim3(256), 0, s, x, w1,
+                           (float *)(ws + l.a1), MT, c.n_hid, c.n_in);
     hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)c.n_hid * kMaxOut)),
                        dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), c.n_hid,
                        c.n_out);
@@ -1044,11 +1465,21 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
                                dim3(256), 0, s, g2, s1, part2, M, H, O);
         }
     }
-    // dW1 over every step's rows in one GEMM
+    // dW1 over every step's rows in one GEMM, split into kTnSplit row
+    // ranges of whole 32-row slices
     const int MT = c.steps * B;
-    hipLaunchKernelGGL(gemm_tn, dim3((c.n_in + 63) / 64, (H + 63) / 64,
-                                     kTnSplit),
-                       dim3(256), 0, s, lam, x, part1, MT, H, c.n_in);
+    const long gp = ((MT + 7) / 8 + 3) / 4 * 4;
+    uint16_t *xp = (uint16_t *)(ws + l.xp);
+    hipLaunchKernelGGL(pack_x, dim3(blocks(gp * ((c.n_in + 7) / 8))),
+                       dim3(256), 0, s, x, xp, MT, c.n_in, gp);
+    const int per = ((MT + kTnSplit - 1) / kTnSplit + kDr - 1) / kDr * kDr;
+    const dim3 dgrid((c.n_in + kDk - 1) / kDk, (H + kDh - 1) / kDh, kTnSplit);
+    if (H % 4 == 0)
+        hipLaunchKernelGGL(gemm_dw1_dma, dgrid, dim3(256), 0, s, lam, xp,
+                           part1, MT, H, c.n_in, per, gp);
+    else
+        hipLaunchKernelGGL(gemm_dw1, dgrid, dim3(256), 0, s, lam, xp, part1,
+                           MT, H, c.n_in, per);
     hipLaunchKernelGGL(add_parts, dim3(1024), dim3(256), 0, s, part1, dw1,
                        (long)H * c.n_in, kTnSplit);
     hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 63) / 64), dim3(256), 0, s,

@@ -357,6 +357,21 @@ __global__ __launch_bounds__(kNeuronBlock) void copy32_kernel(
         dst[i] = src[i];
 }
 
+// Any n: 16-byte words where both sides allow it, then the tail.
+__global__ __launch_bounds__(kNeuronBlock) void copy_u32_kernel(
+        const uint32_t *src, uint32_t *dst, long n, int vec) {
+    const long stride = (long)gridDim.x * blockDim.x;
+    const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long done = 0;
+    if (vec) {
+        const long n4 = n / 4;
+        for (long i = i0; i < n4; i += stride)
+            ((uint4 *)dst)[i] = ((const uint4 *)src)[i];
+        done = n4 * 4;
+    }
+    for (long i = done + i0; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace sg
 
 using namespace sg;
@@ -504,6 +519,21 @@ extern "C" int sg_hip_copy32(const void *src, void *dst, long n,
     hipLaunchKernelGGL(copy32_kernel, dim3(blocks), dim3(kNeuronBlock), 0,
                        (hipStream_t)stream, (const uint4 *)src, (uint4 *)dst,
                        n4);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_copy_u32(const void *src, void *dst, long n,
+                               void *stream) {
+    if (n <= 0) return 0;
+    if (((uintptr_t)src | (uintptr_t)dst) % 4 != 0)
+        return (int)hipErrorInvalidValue;
+    const int vec = ((uintptr_t)src | (uintptr_t)dst) % 16 == 0;
+    const long units = vec ? n / 4 + 3 : n;
+    const int blocks = (int)std::min<long>((units + kNeuronBlock - 1) /
+                                           kNeuronBlock, 1024L);
+    hipLaunchKernelGGL(copy_u32_kernel, dim3(blocks), dim3(kNeuronBlock), 0,
+                       (hipStream_t)stream, (const uint32_t *)src,
+                       (uint32_t *)dst, n, vec);
     return hipGetLastError();
 }
 

@@ -41,7 +41,7 @@ enum { IO_INPUT = 1, IO_OUTPUT = 2 };
 // ---------------------------------------------------------------------------
 
 enum ResKind { RES_IN = 0, RES_OUT = 1, RES_STATE = 2, RES_SYN = 3,
-               RES_TSS = 4, RES_CTR = 5, RES_INTER = 6 };
+               RES_TSS = 4, RES_CTR = 5 };
 typedef uint64_t Res;
 
 Res res(ResKind k, const void *p) {
@@ -227,8 +227,6 @@ struct Cache : EngineCache {
     Sched sched;
     // second word buffer of each game-of-life layer on the fused step
     std::map<LayerState *, uint32_t *> gol_spare;
-    // last fed input of each input layer, on the device
-    std::map<LayerState *, float *> inter;
     // step counter that captured steps read their random draws' step from
     uint64_t *d_step = nullptr;
     uint64_t *h_step = nullptr;     // pinned staging for it
@@ -236,7 +234,6 @@ struct Cache : EngineCache {
     ~Cache() override {
         graphs.clear();
         for (auto &kv : gol_spare) (void)hipFree(kv.second);
-        for (auto &kv : inter) (void)hipFree(kv.second);
         if (d_step) (void)hipFree(d_step);
         if (h_step) (void)hipHostFree(h_step);
     }
@@ -247,10 +244,12 @@ Cache &cache_of(State &s) {
     return static_cast<Cache &>(*s.engine_cache);
 }
 
-// Pinned host staging buffer of one (layer, key), and the event of the
-// last copy that used it.
+// Pinned host staging buffer of one (layer, key), mapped into the device's
+// address space and coherent (the device reads and writes it over the
+// link, past its caches), and the event of the last copy that used it.
 struct HostBuf {
     float *p = nullptr;
+    void *dp = nullptr;    // the device's address of p
     int n = 0;
     bool dirty = false;
     hipEvent_t done = nullptr;
@@ -262,8 +261,11 @@ struct HostBuf {
     void alloc(int size) {
         n = size;
         check_hip(hipHostMalloc((void **)&p, (size_t)size * 4,
-                                hipHostMallocDefault), "hipHostMalloc");
+                                hipHostMallocMapped | hipHostMallocCoherent),
+                  "hipHostMalloc");
         std::memset(p, 0, (size_t)size * 4);
+        check_hip(hipHostGetDevicePointer(&dp, p, 0),
+                  "hipHostGetDevicePointer");
         check_hip(hipEventCreateWithFlags(&done, hipEventDisableTiming),
                   "hipEventCreate");
     }
@@ -615,6 +617,11 @@ class Runner {
         auto it = io_types_.find(ls);
         return it != io_types_.end() && (it->second & IO_INPUT);
     }
+    bool any_input_key() const {
+        for (auto &kv : in_keys_)
+            if (kv.second.count("input")) return true;
+        return false;
+    }
     bool is_base(const ConnState &cs) const {
         return (!cs.iz || cs.direct || cs.iz_kind == SG_IZ_GAP) &&
                !cs.fused_gol;
@@ -687,7 +694,7 @@ class Runner {
     bool any_dendrites_ = false, any_second_order_ = false;
 
     std::map<const Structure *, int> main_, side_;
-    int io_ = 0, nstreams_ = 1;
+    int nstreams_ = 1;
     bool any_split_ = false;
 };
 
@@ -762,11 +769,6 @@ void Runner::setup_modules() {
             const int n = kv.first->layer->size;
             in_bufs_[{kv.first, k}].reset(new InBuf());
             in_bufs_[{kv.first, k}]->alloc(n);
-            if (k == "input" && !C_->inter.count(kv.first)) {
-                float *d = nullptr;
-                check_hip(hipMalloc((void **)&d, (size_t)n * 4), "hipMalloc");
-                C_->inter[kv.first] = d;
-            }
         }
     for (auto &kv : out_keys_)
         for (auto &k : kv.second) {
@@ -922,8 +924,8 @@ void Runner::setup_plans() {
 // parallel structure whose layers have both kinds of activators, the ones
 // that write a layer's input (base activators, gap junctions) on a second
 // stream of their own, beside the conductance passes (which write
-// conductances and synapse records); one IO stream for the environment's
-// copies.  Kernel timing keeps everything on stream 0.
+// conductances and synapse records).  The environment's copies run on
+// the stream of their layer.  Kernel timing keeps everything on stream 0.
 void Runner::setup_streams() {
     const bool one = kt_.on;
     const bool per_structure = !one &&
@@ -953,7 +955,6 @@ void Runner::setup_streams() {
         side_[s] = split ? next++ : main_[s];
         any_split_ = any_split_ || split;
     }
-    io_ = (!modules_.empty() && !one) ? next++ : 0;
     nstreams_ = next;
     sched().ensure(nstreams_);
     sched().set_used(nstreams_);
@@ -1265,7 +1266,7 @@ void Runner::pre_input(const StepCtx &x) {
 }
 
 // 2. environment input: callbacks fill pinned buffers, which are copied to
-// the input layers on the IO stream.
+// the input layers on their structures' streams.
 void Runner::input_phase() {
     for (auto &m : modules_) {
         if (m.start_delay > 0) continue;
@@ -1282,37 +1283,24 @@ void Runner::input_phase() {
             }
         }
     }
+    // Each step the last buffer written (its values stay the input while a
+    // module skips steps) is copied by a kernel on the layer's stream,
+    // reading the pinned buffer over the link: no DMA engine, no stream
+    // and events of its own.
     for (auto &kv : in_keys_) {
         LayerState *ls = kv.first;
-        const size_t bytes = (size_t)ls->layer->size * 4;
+        const long n = ls->layer->size;
         for (auto &k : kv.second) {
             InBuf &b = *in_bufs_.at({ls, k});
-            if (k == "input") {
-                float *inter = C_->inter.at(ls);
-                const Res ri = res(RES_INTER, ls);
-                if (b.dirty) {
-                    hipStream_t s = sched().begin(io_, {}, {ri});
-                    check_hip(hipMemcpyAsync(inter, b.last().p, bytes,
-                                             hipMemcpyHostToDevice, s),
-                              "input H2D");
-                    sched().end();
-                    b.last().record(s);
-                    b.dirty = false;
-                }
-                hipStream_t s = sched().begin(io_, {ri}, {res(RES_IN, ls)});
-                check_hip(hipMemcpyAsync(ls->input.dev, inter, bytes,
-                                         hipMemcpyDeviceToDevice, s),
-                          "input D2D");
-                sched().end();
-            } else {
-                hipStream_t s = sched().begin(io_, {}, {res(RES_STATE, ls)});
-                check_hip(hipMemcpyAsync(ls->var(k).dev, b.last().p, bytes,
-                                         hipMemcpyHostToDevice, s),
-                          "aux input H2D");
-                sched().end();
-                b.last().record(s);
-                b.dirty = false;
-            }
+            const bool in = k == "input";
+            void *dst = in ? ls->input.dev : ls->var(k).dev;
+            const Res r = in ? res(RES_IN, ls) : res(RES_STATE, ls);
+            hipStream_t s = sched().begin(main_of(ls), {}, {r});
+            check_hip(sg_hip_copy_u32(b.last().dp, dst, n, s),
+                      in ? "input copy" : "aux input copy");
+            sched().end();
+            b.last().record(s);
+            b.dirty = false;
         }
     }
 }
@@ -1390,14 +1378,13 @@ void Runner::updates() {
     }
 }
 
-// 6. output: device -> host copies on the IO stream; the callbacks run once
-// the copies have landed (output_deliver).
+// 6. output: copies into pinned host buffers on the layers' streams; the
+// callbacks run once the copies have landed (output_deliver).
 bool Runner::output_issue(long it) {
     if (!(it % env_rate_ == 0 && !suppress_ && !out_keys_.empty()))
         return false;
     for (auto &kv : out_keys_) {
         LayerState *ls = kv.first;
-        const size_t bytes = (size_t)ls->layer->size * 4;
         for (auto &k : kv.second) {
             HostBuf &b = *out_bufs_.at({ls, k});
             const void *src = k == "output" ? ls->output.dev
@@ -1406,10 +1393,9 @@ bool Runner::output_issue(long it) {
             const Res r = k == "output" ? res(RES_OUT, ls)
                         : k == "input" ? res(RES_IN, ls)
                         : res(RES_STATE, ls);
-            hipStream_t s = sched().begin(io_, {r}, {});
-            check_hip(hipMemcpyAsync(b.p, src, bytes, hipMemcpyDeviceToHost,
-                                     s),
-                      "output D2H");
+            hipStream_t s = sched().begin(main_of(ls), {r}, {});
+            check_hip(sg_hip_copy_u32(src, b.dp, ls->layer->size, s),
+                      "output copy");
             sched().end();
             b.record(s);
         }
@@ -1599,14 +1585,6 @@ Props *Runner::run() {
             S.end();
         }
     }
-    for (auto &kv : in_keys_) {
-        if (!kv.second.count("input")) continue;
-        const Res r = res(RES_INTER, kv.first);
-        hipStream_t s = S.begin(io_, {}, {r});
-        check_hip(sg_hip_set_f32(C_->inter.at(kv.first), 0.0f,
-                                 kv.first->layer->size, 1, s), "clear");
-        S.end();
-    }
 
     // With "step graphs": "true", runs without environment I/O capture their
     // steady-state steps (from the second on, all but the last) into HIP
@@ -1725,7 +1703,7 @@ Props *Runner::run() {
     for (auto &kv : in_bufs_) buf_bytes += (size_t)kv.second->b[0].n * 4;
     for (auto &kv : out_bufs_) buf_bytes += (size_t)kv.second->n * 4;
     r->set("state buffer bytes",
-           std::to_string(C_->inter.empty() ? 0 : buf_bytes));
+           std::to_string(any_input_key() ? buf_bytes : 0));
     r->set("engine buffer bytes", std::to_string(buf_bytes));
     r->set_child("args", args_);
     r->set("interrupted", interrupted ? "true" : "false");

@@ -9,16 +9,21 @@
  *   sg_io_normal_input     fills `size` floats with N(mean, std) values of a
  *                          counter-based stream keyed by (id, step, index);
  *                          the step counts the calls for that id
+ *   sg_io_table_input      copies the next row of a table of recorded
+ *                          inputs (sg_io_set_table), cycling through its
+ *                          rows: a replayed stimulus, whose per-step cost is
+ *                          one copy
  *   sg_io_checksum_output  folds the `size` 32-bit words of the buffer into
- *                          a running checksum per id
+ *                          a running 64-bit FNV-1a checksum per id
  *
  * The ids index 64 slots; sg_io_configure sets an input slot's mean and
- * standard deviation and resets its step, sg_io_checksum reads an output
- * slot.  Not thread safe across calls for the same id (the engine calls
- * each callback from its one host thread).
+ * standard deviation and resets its step and checksum, sg_io_checksum reads
+ * an output slot.  Not thread safe across calls for the same id (the engine
+ * calls each callback from its one host thread).
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define SLOTS 64
@@ -26,6 +31,8 @@
 static uint64_t g_step[SLOTS];
 static double g_mean[SLOTS], g_std[SLOTS];
 static uint64_t g_sum[SLOTS];
+static float *g_table[SLOTS];
+static int g_rows[SLOTS], g_cols[SLOTS];
 
 /* splitmix64 (Steele, Lea, Flood, OOPSLA 2014) of a counter */
 static uint64_t mix(uint64_t x) {
@@ -40,7 +47,30 @@ void sg_io_configure(int id, double mean, double std) {
     g_mean[id] = mean;
     g_std[id] = std;
     g_step[id] = 0;
-    g_sum[id] = 0;
+    g_sum[id] = 0xCBF29CE484222325ull;   /* FNV-1a offset basis */
+}
+
+/* Keeps a copy of `rows` x `cols` floats for sg_io_table_input on slot
+ * `id` and resets its step; returns 0, or -1 (bad id or size, no memory). */
+int sg_io_set_table(int id, const float *table, int rows, int cols) {
+    if (id < 0 || id >= SLOTS || rows < 1 || cols < 1) return -1;
+    float *t = (float *)malloc((size_t)rows * cols * sizeof(float));
+    if (!t) return -1;
+    memcpy(t, table, (size_t)rows * cols * sizeof(float));
+    free(g_table[id]);
+    g_table[id] = t;
+    g_rows[id] = rows;
+    g_cols[id] = cols;
+    g_step[id] = 0;
+    return 0;
+}
+
+void sg_io_table_input(int id, int size, void *buffer) {
+    if (id < 0 || id >= SLOTS || !g_table[id]) return;
+    const int row = (int)(g_step[id]++ % (uint64_t)g_rows[id]);
+    const int n = size < g_cols[id] ? size : g_cols[id];
+    memcpy(buffer, g_table[id] + (size_t)row * g_cols[id],
+           (size_t)n * sizeof(float));
 }
 
 uint64_t sg_io_checksum(int id) {
@@ -65,6 +95,6 @@ void sg_io_checksum_output(int id, int size, void *buffer) {
     if (id < 0 || id >= SLOTS) return;
     const uint32_t *w = (const uint32_t *)buffer;
     uint64_t h = g_sum[id];
-    for (int i = 0; i < size; ++i) h = mix(h ^ w[i]);
+    for (int i = 0; i < size; ++i) h = (h ^ w[i]) * 0x100000001B3ull;
     g_sum[id] = h;
 }
