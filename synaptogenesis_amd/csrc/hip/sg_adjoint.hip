@@ -7,8 +7,9 @@
 // elementwise recurrences in time:
 //   forward   gemm_fwd     A1[T*batch][n_hid] = X . W1^T for every step
 //                          (MFMA BF16), kept for the backward
-//             T <= 1024: scan1_fwd_all and out_scan2_fwd_all, the passes
-//             below over all steps in two launches; else per block:
+//             T <= 1024: scan1_fwd_all, out_current_mfma (S1 . W2^T as
+//             one MFMA product) and scan2_fwd_all, the passes below over
+//             all steps; else per block:
 //             scan1_fwd    one thread per (b, hidden unit): v1, s1 over
 //                          the block's steps
 //             out_current  A2[M][n_out] = S1 . W2^T, one wavefront per row
@@ -76,6 +77,14 @@ __device__ __forceinline__ float act(float u, int mode, float slope) {
 __device__ __forceinline__ float act_grad(float u, int mode, float slope) {
     const float d = 1.0f + slope * fabsf(u);
     const float g = 1.0f / (d * d);
+    return mode == SG_SNN_SPIKE ? g : 0.5f * slope * g;
+}
+
+// The same derivative with the hardware reciprocal (about 1 ulp) in place
+// of the IEEE division: the backward passes' hot loop.
+__device__ __forceinline__ float act_grad_fast(float u, int mode, float slope) {
+    const float r = __builtin_amdgcn_rcpf(1.0f + slope * fabsf(u));
+    const float g = r * r;
     return mode == SG_SNN_SPIKE ? g : 0.5f * slope * g;
 }
 
@@ -404,9 +413,16 @@ __device__ __forceinline__ void dma16(const void *src, void *lds_base) {
 #pragma clang diagnostic pop
 
 // Forward currents, K a multiple of 8 (16-byte rows).  Slices of 32
-// columns: a row of a slice is four 16-byte chunks, stored at chunk
-// c ^ ((row >> 2) & 3) so that the 16 rows a fragment read touches spread
-// over the banks.  Blocks that share an m tile run back to back on one XCD.
+// columns: a slice's rows are stored in groups of 16, each group as its
+// four 16-byte column chunks one after the other, each chunk as the 16
+// rows' values in row order (byte (r / 16) * 1024 + c * 256 + (r % 16) *
+// 16): the 16 lanes of a fragment read take 256 contiguous bytes, and a
+// load instruction fills one group, 16 rows of 64 contiguous bytes.
+// Blocks that share an m tile run back to back on one XCD.
+__device__ __forceinline__ int tile_off(int r, int c) {
+    return (r >> 4) * 512 + c * 128 + (r & 15) * 8;   // in BF16 elements
+}
+
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
         const uint16_t *__restrict__ X, const uint16_t *__restrict__ W,
@@ -427,9 +443,8 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
     const int w = t >> 6, lane = t & 63;
     const int wm = (w & 1) * 64, wn = (w >> 1) * 128;
     const int li = lane & 15, lc = lane >> 4;
-    // this lane's DMA rows (fixed) and chunk within the row
-    const int dr = lane >> 2;
-    const int dc = 8 * ((lane & 3) ^ ((dr >> 2) & 3));
+    // this lane's DMA row within a 16-row group and its column chunk
+    const int dr = lane & 15, dc = 8 * (lane >> 4);
     const uint16_t *asrc[2], *bsrc[4];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -452,13 +467,11 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
         bf16x8 a[4], b[8];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int r = wm + 16 * i + li;
-            a[i] = *(const bf16x8 *)&as[r * kFk + 8 * (lc ^ ((r >> 2) & 3))];
+            a[i] = *(const bf16x8 *)&as[tile_off(wm + 16 * i + li, lc)];
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int r = wn + 16 * j + li;
-            b[j] = *(const bf16x8 *)&bs[r * kFk + 8 * (lc ^ ((r >> 2) & 3))];
+            b[j] = *(const bf16x8 *)&bs[tile_off(wn + 16 * j + li, lc)];
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -497,13 +510,13 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int q = t + 256 * i, r = q >> 2;
-            *(u32x4 *)&A0[r * kFk + 8 * ((q & 3) ^ ((r >> 2) & 3))] =
+            *(u32x4 *)&A0[tile_off(r, q & 3)] =
                 load8(X, M, K, m0 + r, k0 + (q & 3) * 8);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int q = t + 256 * i, r = q >> 2;
-            *(u32x4 *)&B0[r * kFk + 8 * ((q & 3) ^ ((r >> 2) & 3))] =
+            *(u32x4 *)&B0[tile_off(r, q & 3)] =
                 load8(W, N, K, n0 + r, k0 + (q & 3) * 8);
         }
         __syncthreads();
@@ -889,48 +902,134 @@ __global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
     }
 }
 
-// out_scan2_fwd_all: block = batch row b; wavefront w reduces W2 . s1 of
-// steps t = w, w + 16, ... exactly as out_scan2_fwd into LDS ([T][16]
-// floats), then threads o < O run layer 2's recurrence over every step: v2
-// to rec_v2, the state after each block of ck steps to that block's
-// checkpoint slot, the spike counts to counts.
-__global__ __launch_bounds__(1024) void out_scan2_fwd_all(
-        ScanArgs a, int T, int ck, long slot,
-        const float *__restrict__ s1_all, const uint16_t *__restrict__ w2t,
-        float *__restrict__ state, float *__restrict__ counts,
-        float *__restrict__ rec_v2) {
-    extern __shared__ float a2all[];
-    const int b = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nw = blockDim.x >> 6;
-    const int O = a.O;
-    for (int t = w; t < T; t += nw) {
-        float p[kMaxOut];
-        row_partials(s1_all, w2t, (long)t * a.B + b, a.H, lane, p);
+// The forward's output currents for every step as one MFMA product over
+// the whole grid: A2[row][o] = sum over u of S1[row][u] W2[o][u], rows =
+// t * B + b.  W2 enters as the B operand in fragment order (w2f, BF16,
+// exact); S1 as the A operand, exact in BF16 in spike mode (0 / 1) and as
+// three BF16 terms hi + mid + lo in rate mode, which keeps FP32 precision.
+// A block of four wavefronts takes 16 rows; wavefront w multiplies units
+// [w H / 4, (w + 1) H / 4) with all of its loads issued first, and the four
+// partial tiles are added in wavefront order.  Every sum runs in a fixed
+// order.
+constexpr int kOutWaves = 4;
+
+// w2f[(kk * 64 + lane) * 8 + e] = W2[lane % 16][32 kk + 8 (lane / 16) + e]
+// (0 past O or H): the B fragment of units 32 kk .. 32 kk + 31.
+__global__ void frag_w2(const uint16_t *__restrict__ w2,
+                        uint16_t *__restrict__ w2f, int H, int O, int nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nk * 64 * 8) return;
+    const int e = i & 7, lane = (i >> 3) & 63, kk = i >> 9;
+    const int o = lane & 15, u = 32 * kk + 8 * (lane >> 4) + e;
+    w2f[i] = (o < O && u < H) ? w2[o * H + u] : (uint16_t)0;
+}
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void out_current_mfma(
+        const float *__restrict__ s1, const uint16_t *__restrict__ w2f,
+        float *__restrict__ a2, long rows, int H, int O, int nk) {
+    __shared__ f32x4 part[kOutWaves - 1][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const long r0 = (long)blockIdx.x * 16;
+    const long row = min(r0 + li, rows - 1);   // past the end: never stored
+    const float *srow = s1 + row * H;
+    const int per = (nk + kOutWaves - 1) / kOutWaves;
+    const int k0 = w * per, k1 = min(nk, k0 + per);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bool vec = (H & 7) == 0;
+    constexpr int F = 8;   // fragments whose loads are in flight together
+    for (int kb = k0; kb < k1; kb += F) {
+        float v[F][8];
+        bf16x8 b[F];
 #pragma unroll
-        for (int o = 0; o < kMaxOut; ++o)
-            if (o < O) {
-                const float v = wave_sum(p[o]);
-                if (lane == 0) a2all[t * kMaxOut + o] = v;
+        for (int q = 0; q < F; ++q) {
+            const int kk = kb + q;
+            if (kk >= k1) break;
+            const int u0 = 32 * kk + 8 * lk;
+            if (vec && u0 + 8 <= H) {
+                const float4 x0 = *(const float4 *)&srow[u0];
+                const float4 x1 = *(const float4 *)&srow[u0 + 4];
+                v[q][0] = x0.x; v[q][1] = x0.y; v[q][2] = x0.z; v[q][3] = x0.w;
+                v[q][4] = x1.x; v[q][5] = x1.y; v[q][6] = x1.z; v[q][7] = x1.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    v[q][e] = u0 + e < H ? srow[u0 + e] : 0.0f;
             }
+            b[q] = *(const bf16x8 *)&w2f[((long)kk * 64 + lane) * 8];
+        }
+#pragma unroll
+        for (int q = 0; q < F; ++q) {
+            if (kb + q >= k1) break;
+            if (SPLIT) {
+                bf16x8 hi, mid, lo;
+                split3(v[q], hi, mid, lo);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, b[q], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mid, b[q], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, b[q], acc, 0, 0, 0);
+            } else {
+                bf16x8 a;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) a[e] = (__bf16)v[q][e];
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[q], acc, 0, 0, 0);
+            }
+        }
     }
+    if (w > 0) part[w - 1][lane] = acc;
     __syncthreads();
-    if (threadIdx.x >= O) return;
-    const int i = b * O + threadIdx.x;
+    if (w != 0) return;
+#pragma unroll
+    for (int p = 0; p < kOutWaves - 1; ++p) {
+        const f32x4 o4 = part[p][lane];
+        acc[0] += o4[0];
+        acc[1] += o4[1];
+        acc[2] += o4[2];
+        acc[3] += o4[3];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const long rr = r0 + 4 * lk + r;
+        if (rr < rows && li < O) a2[rr * O + li] = acc[r];
+    }
+}
+
+// Layer 2 over every step from the all-steps currents; thread = (b, o):
+// v2 of every step to rec_v2, the state after each block of ck steps to
+// that block's checkpoint slot, the spike counts to counts.  The currents
+// of 16 steps are loaded together.
+__global__ void scan2_fwd_all(ScanArgs a, int T, int ck, long slot,
+                              const float *__restrict__ a2,
+                              float *__restrict__ state,
+                              float *__restrict__ counts,
+                              float *__restrict__ rec_v2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int O = a.O;
+    if (i >= a.B * O) return;
     const long BH = (long)a.B * a.H, BO = (long)a.B * O;
     float v = state[2 * BH + i], s = state[2 * BH + BO + i];
     float cnt = 0.0f;
     int left = ck;
     float *next = state + slot;
-    for (int t = 0; t < T; ++t) {
-        v = a.beta * v - a.reset_theta * s + a2all[t * kMaxOut + threadIdx.x];
-        s = act(v - a.theta, a.mode, a.slope);
-        cnt += s;
-        rec_v2[(long)t * BO + i] = v;
-        if (--left == 0) {
-            left = ck;
-            next[2 * BH + i] = v;
-            next[2 * BH + BO + i] = s;
-            next += slot;
+    constexpr int W = 16;
+    for (int t0 = 0; t0 < T; t0 += W) {
+        float in[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+            if (t0 + j < T) in[j] = a2[(long)(t0 + j) * BO + i];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            if (t0 + j >= T) break;
+            v = a.beta * v - a.reset_theta * s + in[j];
+            s = act(v - a.theta, a.mode, a.slope);
+            cnt += s;
+            rec_v2[(long)(t0 + j) * BO + i] = v;
+            if (--left == 0) {
+                left = ck;
+                next[2 * BH + i] = v;
+                next[2 * BH + BO + i] = s;
+                next += slot;
+            }
         }
     }
     counts[i] = cnt;
@@ -1003,31 +1102,48 @@ __global__ __launch_bounds__(256) void scan_bwd_fused(
     float tot[kMaxOut];
 #pragma unroll
     for (int o = 0; o < kMaxOut; ++o) tot[o] = 0.0f;
-    for (int blk = nblk - 1; blk >= 0; --blk) {
+    // the loads of block blk - 1 (currents, layer 2's potentials, the
+    // checkpoint) are issued at the start of block blk and land while it
+    // is computed
+    float in_n[16] = {}, v2_n[16] = {}, ckv_n = 0.0f, cks_n = 0.0f;
+    auto fetch = [&](int blk) {
+        const float *ab = a1 + blk * M * H;
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if (t < a.steps) in_n[t] = ab[(long)t * BH + i];
+        const float *ck = state + blk * slot;
+        ckv_n = ck[i];
+        cks_n = ck[BH + i];
         if (out_thread) {
             const float *rv = rec_v2 + blk * M * O;
-            float v2[16];
 #pragma unroll
             for (int t = 0; t < 16; ++t)
-                if (t < a.steps) v2[t] = rv[(long)t * BO + i2];
+                if (t < a.steps) v2_n[t] = rv[(long)t * BO + i2];
+        }
+    };
+    fetch(nblk - 1);
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+        float in[16], v2[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            in[t] = in_n[t];
+            v2[t] = v2_n[t];
+        }
+        float v = ckv_n, s = cks_n;
+        if (blk > 0) fetch(blk - 1);
+        if (out_thread) {
 #pragma unroll
             for (int t = 15; t >= 0; --t) {
                 if (t >= a.steps) continue;
                 const float d = (gb - a.reset_theta * dn2) *
-                                act_grad(v2[t] - a.theta, a.mode, a.slope) +
+                                act_grad_fast(v2[t] - a.theta, a.mode, a.slope) +
                                 a.beta * dn2;
                 d2s[t][threadIdx.x] = d;
                 dn2 = d;
             }
         }
-        // layer 1 forward from the checkpoint, all loads issued first
-        const float *ab = a1 + blk * M * H;
-        const float *ck = state + blk * slot;
-        float in[16], vr[16];
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (t < a.steps) in[t] = ab[(long)t * BH + i];
-        float v = ck[i], s = ck[BH + i];
+        // layer 1 forward from the checkpoint
+        float vr[16];
         __syncthreads();   // d2s of this block complete
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
@@ -1048,11 +1164,11 @@ __global__ __launch_bounds__(256) void scan_bwd_fused(
 #pragma unroll
             for (int o = 0; o < kMaxOut; ++o)
                 if (o < O) {
-                    gs += wc[o] * d2s[t][o];
-                    acc[o] += d2s[t][o] * sv;
+                    gs = fmaf(wc[o], d2s[t][o], gs);
+                    acc[o] = fmaf(d2s[t][o], sv, acc[o]);
                 }
             const float d = (gs - a.reset_theta * dn) *
-                            act_grad(vr[t] - a.theta, a.mode, a.slope) +
+                            act_grad_fast(vr[t] - a.theta, a.mode, a.slope) +
                             a.beta * dn;
             lb[(long)t * BH + i] = d;
             dn = d;
@@ -1234,7 +1350,7 @@ size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 // Workspace carve-up, in bytes from the base.
 struct Layout {
     size_t state, counts, grad, a1, s1, a2, rv1, rv2, carry1, carry2, lam,
-        g2, part2, part1, w2t, xp, total;
+        g2, part2, part1, w2t, xp, a2all, w2f, total;
 };
 
 // The whole backward runs in one launch (scan_bwd_fused) where each
@@ -1274,6 +1390,8 @@ Layout layout(const sg_snn_cfg &c) {
     l.part2 = take(dw2_slots(c) * O * H);
     l.part1 = take((size_t)kTnSplit * H * c.n_in);
     l.w2t = take((H * kMaxOut + 1) / 2);
+    l.a2all = take(MT * O);         // output currents of every step
+    l.w2f = take((H + 31) / 32 * 64 * 8 / 2);   // W2 in B-fragment order
     // x regrouped by 8 rows (BF16), groups padded to whole 32-row slices
     l.xp = take(((MT + 7) / 8 + 3) / 4 * 4 * c.n_in * 4);
     l.total = off;
@@ -1398,11 +1516,27 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
         hipLaunchKernelGGL(scan1_fwd_all, dim3(blocks((long)c.batch * c.n_hid)),
                            dim3(256), 0, s, a, c.steps, c.checkpoint, slot,
                            (const float *)(ws + l.a1), state, s1_all);
-        hipLaunchKernelGGL(out_scan2_fwd_all, dim3(c.batch), dim3(1024),
-                           (size_t)c.steps * kMaxOut * sizeof(float), s, a,
-                           c.steps, c.checkpoint, slot, s1_all,
-                           (const uint16_t *)(ws + l.w2t), state, cnt,
-                           (float *)(ws + l.rv2));
+        {
+            // S1 . W2^T over the whole grid (MFMA), then layer 2's scan
+            float *a2all = (float *)(ws + l.a2all);
+            uint16_t *w2f = (uint16_t *)(ws + l.w2f);
+            const int nk = (c.n_hid + 31) / 32;
+            hipLaunchKernelGGL(frag_w2, dim3(blocks((long)nk * 512)),
+                               dim3(256), 0, s, w2, w2f, c.n_hid, c.n_out, nk);
+            const dim3 og((unsigned)((MT + 15) / 16));
+            if (c.mode == SG_SNN_SPIKE)
+                hipLaunchKernelGGL(out_current_mfma<false>, og, dim3(256), 0,
+                                   s, s1_all, w2f, a2all, (long)MT, c.n_hid,
+                                   c.n_out, nk);
+            else
+                hipLaunchKernelGGL(out_current_mfma<true>, og, dim3(256), 0,
+                                   s, s1_all, w2f, a2all, (long)MT, c.n_hid,
+                                   c.n_out, nk);
+            hipLaunchKernelGGL(scan2_fwd_all,
+                               dim3(blocks((long)c.batch * c.n_out)), dim3(256),
+                               0, s, a, c.steps, c.checkpoint, slot, a2all,
+                               state, cnt, (float *)(ws + l.rv2));
+        }
     } else {
         for (int blk = 0; blk < c.steps / c.checkpoint; ++blk)
             forward_block(c, a, blk, l, ws, state + blk * slot,

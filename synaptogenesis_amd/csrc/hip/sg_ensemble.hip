@@ -217,8 +217,7 @@ __device__ __forceinline__ void grid_barrier(unsigned int *bar,
 // the clear before its next use.  Every block runs the same number of
 // barriers.  sync = {queue[2], barrier[3]}.
 template <int RT, int KS>
-__global__ __launch_bounds__(64 * RT * KS)
-__attribute__((amdgpu_waves_per_eu(8, 8))) void ens_bf16_persistent(
+__global__ __launch_bounds__(64 * RT * KS) void ens_bf16_persistent(
     const Layer *__restrict__ layers, const Item *__restrict__ items,
     int n_items, int parity, int steps, float leak, int rectify,
     unsigned int *sync) {

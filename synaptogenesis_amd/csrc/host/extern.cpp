@@ -153,6 +153,7 @@ ARRAY get_child_array(PROPS p, char *key) {
 }
 
 ENVIRONMENT create_environment(PROPS properties) {
+    if (!properties) return nullptr;
     try {
         EnvHandle *e = new EnvHandle();
         e->props = *P(properties);
@@ -173,6 +174,7 @@ ENVIRONMENT load_env(char *filename) {
 }
 
 bool save_env(ENVIRONMENT env, char *filename) {
+    if (!env || !filename) return false;
     try {
         std::ofstream f(filename);
         f << write_json(static_cast<EnvHandle *>(env)->props);
@@ -183,6 +185,7 @@ bool save_env(ENVIRONMENT env, char *filename) {
 }
 
 NETWORK create_network(PROPS properties) {
+    if (!properties) return nullptr;
     try {
         std::unique_ptr<NetHandle> n(new NetHandle());
         n->net.reset(new Network(*P(properties)));
@@ -206,6 +209,7 @@ NETWORK load_net(char *filename) {
 }
 
 bool save_net(NETWORK network, char *filename) {
+    if (!network || !filename) return false;
     try {
         std::ofstream f(filename);
         f << write_json(static_cast<NetHandle *>(network)->net->config());
@@ -216,6 +220,7 @@ bool save_net(NETWORK network, char *filename) {
 }
 
 STATE build_state(NETWORK net) {
+    if (!net) return nullptr;
     try {
         std::unique_ptr<StateHandle> s(new StateHandle());
         s->state.reset(new State(static_cast<NetHandle *>(net)->net.get()));
@@ -226,6 +231,7 @@ STATE build_state(NETWORK net) {
 }
 
 STATE build_load_state(NETWORK net, char *filename) {
+    if (!net || !filename) return nullptr;
     try {
         std::unique_ptr<StateHandle> s(new StateHandle());
         s->state.reset(new State(static_cast<NetHandle *>(net)->net.get()));
@@ -237,6 +243,7 @@ STATE build_load_state(NETWORK net, char *filename) {
 }
 
 bool load_state(STATE state, char *filename) {
+    if (!state || !filename) return false;
     try {
         static_cast<StateHandle *>(state)->state->load(filename);
         return true;
@@ -246,6 +253,7 @@ bool load_state(STATE state, char *filename) {
 }
 
 bool save_state(STATE state, char *filename) {
+    if (!state || !filename) return false;
     try {
         static_cast<StateHandle *>(state)->state->save(filename);
         return true;
@@ -256,6 +264,7 @@ bool save_state(STATE state, char *filename) {
 
 ARRAY get_neuron_data(STATE state, char *structure_name, char *layer_name,
                       char *key) {
+    if (!state || !structure_name || !layer_name || !key) return null_array();
     try {
         State *st = static_cast<StateHandle *>(state)->state.get();
         Structure *s = st->network()->structure(structure_name);
@@ -273,6 +282,7 @@ ARRAY get_layer_data(STATE, char *, char *, char *) { return null_array(); }
 ARRAY get_connection_data(STATE, char *, char *) { return null_array(); }
 
 ARRAY get_weight_matrix(STATE state, char *conn_name, char *key) {
+    if (!state || !conn_name) return null_array();
     try {
         State *st = static_cast<StateHandle *>(state)->state.get();
         Connection *c = st->network()->connection(conn_name);

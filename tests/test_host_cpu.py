@@ -300,3 +300,94 @@ def test_state_files_cross_between_implementations(lib, ref_lib, tmp_path,
     assert not bad, bad
     src.free()
     dst.free()
+
+
+def _randomized_net(frm, to, ctype, arbor=None, wrap=False):
+    layers = [netdefs.layer("a", "relay", *frm), netdefs.layer("b", "relay", *to)]
+    a = None
+    if arbor is not None:
+        a = {"field size": arbor, "stride": 1, "wrap": wrap}
+    c = netdefs.conn("rp", "a", "b", ctype, arbor=a,
+                     weight={"type": "flat", "weight": 0.5},
+                     randomized_projection=True)
+    return netdefs.net("s", layers, [c])
+
+
+def _conn_array(network, key):
+    return harness._view(network.get_weight_matrix("rp", key), key).view(np.int32)
+
+
+def test_randomized_one_to_one_draws_one_source_per_destination(lib):
+    """weight_matrix.cpp:380-393: each destination of a randomized
+    one-to-one projection reads one source, drawn uniformly; its index is
+    row * columns + column of the drawn row and column."""
+    network = syngen.Network(_randomized_net((9, 11), (9, 11), "one to one"),
+                             lib)
+    assert network.build_state()
+    rows = _conn_array(network, "from row indices")
+    cols = _conn_array(network, "from column indices")
+    src = _conn_array(network, "from indices")
+    counts = _conn_array(network, "nonzero counts")
+    assert src.size == 99 and np.all(counts == 1)
+    assert np.all((rows >= 0) & (rows < 9)) and np.all((cols >= 0) & (cols < 11))
+    assert np.array_equal(src, rows * 11 + cols)
+    assert len(np.unique(src)) > 40          # drawn, not the identity
+    assert not np.array_equal(src, np.arange(99))
+    network.free()
+
+
+def test_null_handles_fail_without_crashing(lib):
+    """Entry points given a null handle report failure the reference's way
+    (null / false / a null ARRAY) instead of dereferencing it."""
+    L = lib.lib if hasattr(lib, "lib") else lib
+    assert not L.build_state(None)
+    assert not L.build_load_state(None, b"/nonexistent")
+    assert not L.save_state(None, b"/tmp/x")
+    assert not L.load_state(None, b"/tmp/x")
+    assert not L.save_net(None, b"/tmp/x")
+    assert not L.create_network(None)
+    assert not L.run(None, None, None, None)
+    a = L.get_weight_matrix(None, b"c", b"weights")
+    assert a.size == 0
+
+
+@pytest.mark.parametrize("wrap", [False, True])
+def test_randomized_convergent_moves_whole_fields(lib, wrap):
+    """weight_matrix.cpp:394-438 and 253-343: every destination's 3x3 field
+    keeps its shape, moved so that its first visit sits at a randomly drawn
+    source; the reference writes the index as row * rows + column (kept, on
+    a square source layer where it stays in range); without wrap the visits
+    that leave the layer are dropped, with wrap they wrap and the index is
+    recomputed as row * columns + column.  The draws themselves are the
+    library's (the reference's generator is unseeded)."""
+    network = syngen.Network(_randomized_net((12, 12), (12, 12), "convergent",
+                                             arbor=3, wrap=wrap), lib)
+    assert network.build_state()
+    rows = _conn_array(network, "from row indices").reshape(144, -1)
+    cols = _conn_array(network, "from column indices").reshape(144, -1)
+    src = _conn_array(network, "from indices").reshape(144, -1)
+    counts = _conn_array(network, "nonzero counts")
+    corners = set()
+    for d in range(144):
+        n = counts[d]
+        r, c, s = rows[d, :n], cols[d, :n], src[d, :n]
+        assert np.all((r >= 0) & (r < 12)) and np.all((c >= 0) & (c < 12))
+        assert np.array_equal(s, r * 12 + c)
+        if wrap:
+            assert n == 9
+            # relative offsets of a 3x3 field, modulo the layer
+            dr = (r - r[0]) % 12
+            dc = (c - c[0]) % 12
+            assert sorted(zip(dr.tolist(), dc.tolist())) == \
+                sorted((i, j) for i in range(3) for j in range(3))
+        else:
+            # the visits still in the layer (possibly none: a field is
+            # moved by its first visit, which for an edge destination lies
+            # outside the layer before the move)
+            assert 0 <= n <= 9
+            if n == 0:
+                continue
+            assert r.max() - r.min() <= 2 and c.max() - c.min() <= 2
+        corners.add((int(r[0]), int(c[0])))
+    assert len(corners) > 40                 # drawn, not the regular grid
+    network.free()
