@@ -5,8 +5,9 @@
 // steps are one GEMM and each block of `checkpoint` steps (M = checkpoint *
 // batch rows, row = step * batch + b) runs as parallel passes with only
 // elementwise recurrences in time:
-//   forward   gemm_fwd     A1[T*batch][n_hid] = X . W1^T for every step
-//                          (MFMA BF16), kept for the backward
+//   forward   gemm_fwd_dma A1[T*batch][n_hid] = X . W1^T for every step
+//                          (MFMA BF16; gemm_fwd where n_in is not a
+//                          multiple of 8), kept for the backward
 //             T <= 1024: scan1_fwd_all, out_current_mfma (S1 . W2^T as
 //             one MFMA product) and scan2_fwd_all, the passes below over
 //             all steps; else per block:
@@ -32,8 +33,9 @@
 //                          the blocks
 //             and after the last block
 //             pack_x       x regrouped by eight rows for the product
-//             gemm_dw1     dW1 = D1^T . X over every step's rows (MFMA
-//                          BF16, D1 as hi + mid + lo BF16 terms), split
+//             gemm_dw1_dma dW1 = D1^T . X over every step's rows (MFMA
+//                          BF16, D1 as hi + mid + lo BF16 terms; gemm_dw1
+//                          where n_hid is not a multiple of 4), split
 //                          into row ranges
 // and add_parts / dw2_reduce add the ranges and chunks in a fixed order into dW1
 // and dW2.
@@ -427,8 +429,7 @@ __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
         const uint16_t *__restrict__ X, const uint16_t *__restrict__ W,
         float *__restrict__ C, int M, int N, int K) {
-    // one variable per ring slot: the compiler then sees that reading one
-    // slot does not wait for the loads still landing in the others
+    // one variable per ring slot, chosen by a constant at every use
     __shared__ __align__(16) uint16_t A0[kFm * kFk], A1[kFm * kFk],
         A2[kFm * kFk];
     __shared__ __align__(16) uint16_t B0[kFn * kFk], B1[kFn * kFk],
@@ -489,6 +490,8 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
                 SG_BARRIER();
             }
             issue(sl + 2, (slot + 2) % kRing);
+            // six loads per slice per wavefront: slices sl + 1 and sl + 2
+            // may stay in flight
             SG_WAIT_VM(12);
         } else {
             SG_WAIT_VM(0);
@@ -624,6 +627,8 @@ __global__ __launch_bounds__(256) void gemm_dw1_dma(
                 SG_BARRIER();
             }
             issue(rb + (sl + 2) * kDr, (slot + 2) % kRing);
+            // six loads per slice per wavefront: slices sl + 1 and sl + 2
+            // may stay in flight
             SG_WAIT_VM(12);
         } else {
             SG_WAIT_VM(0);
