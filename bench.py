@@ -17,8 +17,13 @@ reference's sources) on a bounded number of steps of the same workload.
 
 --workload also measures the other single-GPU configurations: sparse (the
 sheet with a SPARSE E->E connection), izfc (two
-Izhikevich layers of 1000, fully connected) and gol1024 / gol16384 (game of
-life boards; the larger one does not fit the 256 MB Infinity Cache).
+Izhikevich layers of 1000, fully connected), izfc_io (the same with a host
+input and output callback every step), gol1024 / gol16384 (game of life
+boards; the larger one does not fit the 256 MB Infinity Cache), ensemble
+(256 batched rate networks) and adjoint (the differentiable two-layer
+network).  The default run measures each of them too, in a child process
+before the sheet, and adds a summary of each one's line under "configs"
+(--no-configs skips them).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-steps S]
                     [--workload sheet|sparse|izfc|izfc_io|gol1024|gol16384|
@@ -32,6 +37,7 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -711,6 +717,56 @@ def ensemble_cpu_baseline(widths, seconds=10.0):
                       "NumPy model, one thread (%.1f s)" % (done, t)}
 
 
+# The other single-GPU configurations, measured by the default run so that
+# one invocation times every configuration: each is this script with
+# --workload <name> --no-cpu in a child process (started before this
+# process touches the GPU), and the summary of its line goes into the
+# headline line's "configs".
+CONFIG_WORKLOADS = ("izfc", "izfc_io", "gol1024", "gol16384", "sparse",
+                    "ensemble", "adjoint")
+CONFIG_TIMEOUT_S = 300
+
+
+def _config_summary(line):
+    keep = ("ms_per_step", "value", "unit", "dtype", "steps")
+    out = {k: line[k] for k in keep if k in line}
+    out["workload"] = line.get("config", {}).get("workload")
+    roof = line.get("roofline")
+    if isinstance(roof, dict):
+        out["roofline"] = {k: roof[k] for k in ("kernel", "bound", "achieved",
+                                                "peak", "unit", "frac")
+                           if k in roof}
+    for k in ("no_io_ms_per_step", "io_over_no_io", "persistent_ms_per_step",
+              "fp64_oracle_max_abs_error", "fp32_torch_path"):
+        if k in line:
+            out[k] = line[k]
+    return out
+
+
+def run_configs(names):
+    """Each workload of `names` in a child process; returns name -> summary
+    (or {"error": ...} when the child failed)."""
+    res = {}
+    for name in names:
+        print("bench: configuration %s" % name, file=sys.stderr, flush=True)
+        cmd = [sys.executable, os.path.abspath(__file__), "--workload", name,
+               "--no-cpu", "--no-configs"]
+        try:
+            p = subprocess.run(cmd, capture_output=True, text=True,
+                               timeout=CONFIG_TIMEOUT_S, cwd=ROOT)
+        except subprocess.TimeoutExpired:
+            res[name] = {"error": "timed out after %d s" % CONFIG_TIMEOUT_S}
+            continue
+        lines = [ln for ln in p.stdout.splitlines() if ln.startswith("{")]
+        if p.returncode != 0 or not lines:
+            tail = (p.stderr or "").strip().splitlines()[-3:]
+            res[name] = {"error": "exit status %d: %s"
+                                  % (p.returncode, " | ".join(tail))}
+            continue
+        res[name] = _config_summary(json.loads(lines[-1]))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -727,6 +783,9 @@ def main():
     ap.add_argument("--run-args", default="{}",
                     help="extra engine run arguments as JSON, e.g. "
                          "'{\"step graphs\": \"true\"}' (A/B runs)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="the default (sheet) run: skip measuring the other "
+                         "configurations beside it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run a plain device copy of MB MiB (reference "
                          "for hardware byte counters) and exit")
@@ -740,6 +799,10 @@ def main():
     if a.copy_calibration:
         copy_calibration(a.copy_calibration)
         return
+
+    configs = None
+    if a.workload == "sheet" and not a.no_configs:
+        configs = run_configs(CONFIG_WORKLOADS)
 
     if a.workload == "ensemble":
         bench_ensemble(a)
@@ -819,6 +882,8 @@ def main():
         "run_args": extra or None,
         "cpu_baseline": cpu,
     }
+    if configs is not None:
+        out["configs"] = configs
     print(json.dumps(out))
 
 
