@@ -60,6 +60,17 @@ __device__ __forceinline__ void ordered_sum_n(const float (*vals)[64],
     }
 }
 
+// The same sum where every skipped visit wrote +0.0f (SPARSE: a slot's
+// visits past its own nonzero count).  Adding +0.0f leaves every value but
+// -0.0f unchanged (quiet NaNs keep their payload), and a sum that starts at
+// +0.0f is never -0.0f (x + y rounds to -0.0f only when both are -0.0f), so
+// the result is bit for bit the sum over the valid visits alone -- without
+// reading each visit's source index again.
+__device__ __forceinline__ void dense_sum_n(const float (*vals)[64], int n,
+                                            int lane, float &sum) {
+    for (int j = 0; j < n; ++j) sum += vals[j][lane];
+}
+
 template <int TYPE, int OUT, int WAVES, int CHUNK>
 __global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
     constexpr int PER = CHUNK / WAVES;
@@ -90,6 +101,8 @@ __global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
                 from[b] = it.k < count ? it.source() : -1;
                 if (it.k < count) it.advance();
                 if (from[b] >= 0) x[b] = c.src[from[b]];
+                else if (TYPE == SG_SPARSE)
+                    vals[buf][wave * PER + b][lane] = 0.0f;
             }
             if (skip) {
 #pragma unroll
@@ -112,8 +125,13 @@ __global__ __launch_bounds__(64 * WAVES) void tile_base_kernel(sg_conn c) {
             }
         }
         __syncthreads();
-        if (live && wave == 0)
-            ordered_sum_n(vals[buf], walk, min(CHUNK, count - k0), lane, sum);
+        if (live && wave == 0) {
+            if (TYPE == SG_SPARSE)
+                dense_sum_n(vals[buf], min(CHUNK, count - k0), lane, sum);
+            else
+                ordered_sum_n(vals[buf], walk, min(CHUNK, count - k0), lane,
+                              sum);
+        }
     }
     if (live && wave == 0) {
         const int to = slot_to_index(c, slot);
@@ -264,7 +282,15 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
             }
 #pragma unroll
             for (int b = 0; b < kSub; ++b) {
-                if (CHECK && from[b] < 0) continue;
+                if (CHECK && from[b] < 0) {
+                    // SPARSE: the ordered sum adds every visit of the chunk
+                    if (TYPE == SG_SPARSE && ACT) {
+                        vals[buf][0][wave * kPerWave + h + b][lane] = 0.0f;
+                        if (kLong)
+                            vals[buf][1][wave * kPerWave + h + b][lane] = 0.0f;
+                    }
+                    continue;
+                }
                 float *q = rec + (h + b) * F * 64;
                 // The updater leaves a synapse of weight below 1e-4 alone
                 // and never writes a weight below it: the test is the same
@@ -322,15 +348,11 @@ __global__ __launch_bounds__(64 * WAVES) void iz_tile_kernel(
             __syncthreads();
             if (live && (wave == 0 || (kLong && wave == 1))) {
                 const int n = min(CHUNK, count - k0);
-                if (k0 + n <= sure) {
-                    for (int j = 0; j < n; ++j) sum += vals[buf][wave][j][lane];
-                    if (TYPE == SG_SPARSE) {   // keep the walk in step
-                        walk.k += n;
-                        walk.wi += (long)n * walk.wstep;
-                    }
-                } else {
+                // SPARSE: skipped visits wrote +0.0f (see dense_sum_n)
+                if (k0 + n <= sure || TYPE == SG_SPARSE)
+                    dense_sum_n(vals[buf][wave], n, lane, sum);
+                else
                     ordered_sum_n(vals[buf][wave], walk, n, lane, sum);
-                }
             }
         }
     }
