@@ -318,6 +318,13 @@ int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
                     float *input, int rows, int cols, int wrap,
                     const float *kernel9, int survival_min, int survival_max,
                     int birth_min, int birth_max, void *stream);
+/* The same for `steps` (1 to 4) consecutive steps in one pass, for a layer
+ * whose words nothing else reads between them: words_out holds the words
+ * after the last step, `input` (if not NULL) that step's neighbour sums. */
+int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
+                     float *input, int rows, int cols, int wrap,
+                     const float *kernel9, int survival_min, int survival_max,
+                     int birth_min, int birth_max, int steps, void *stream);
 
 /* Random init configs (engine/kernel/kernel.cpp:43-158 via
  * engine/instruction.h:208-282): dist 0 = uniform[a,b), 1 = normal(a, b),

@@ -148,6 +148,293 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
     }
 }
 
+// x within K <= 4 of [0, n) wrapped into it: one add or subtract, a
+// division only for boards narrower than the halo.
+__device__ __forceinline__ int gol_wrap_k(int x, int n) {
+    if (x < 0) x += n;
+    else if (x >= n) x -= n;
+    if ((unsigned)x >= (unsigned)n) x = ((x % n) + n) % n;
+    return x;
+}
+
+// K steps (2 <= K <= 4) in one pass, for boards whose words nothing else
+// reads between those steps.  The block stages its tile plus a K-cell halo
+// of words; step s = 1 .. K then computes the newest bit of every cell of
+// the tile and of a (K - s)-cell ring around it from step s - 1's bits
+// (ring cells are computed again by each block that needs them), with the
+// same sum, order and rule as the one-step kernel: the survival test of
+// step s reads bit s - 1 of the staged word (bit 0 after s - 1 shifts),
+// the neighbours' newest bits are step s - 1's.  The tile's words are
+// written once, shifted by K with the K new bits on top, and the input
+// (neighbour sum) of step K only.  Cells outside a non-wrapping board are
+// no cells: their bits stay 0 and the float sum skips them, as in the
+// one-step kernel.
+//
+// Step S of K: each thread takes one column of the step's region and a
+// run of R consecutive rows (NR runs per column, as many as one pass of
+// the block's threads covers for every column), reading the (R + 2) x 3
+// neighbourhood bits once into registers.
+template <bool INTW, int TY, int K, int S>
+__device__ __forceinline__ void gol_multi_step(
+        const uint32_t (&w0)[TY + 2 * K][kGolTx + 2 * K],
+        uint8_t (&bits)[K + 1][TY + 2 * K][kGolTx + 2 * K],
+        uint32_t *__restrict__ out, float *__restrict__ input, int rows,
+        int cols, int wrap, int row0, int col0, const float (&kw)[9],
+        const int (&ki)[9], int smin, int smax, int bmin, int bmax) {
+    constexpr int RY = TY + 2 * K;
+    constexpr int m = K - S;                   // ring of this step
+    constexpr int ny = TY + 2 * m, nx = kGolTx + 2 * m;
+    constexpr int NR = kGolTx * kGolWy / nx;   // row runs per column
+    constexpr int R = (ny + NR - 1) / NR;
+    const int t = threadIdx.y * kGolTx + threadIdx.x;
+    if (t < nx * NR) {
+        const int cx = t % nx;
+        const int lr0 = S + (t / nx) * R;      // this thread's first row
+        const int lc = S + cx;
+        const int c = col0 + lc;
+        uint32_t win[R + 2][3];
+#pragma unroll
+        for (int j = 0; j < R + 2; ++j) {
+            const int lr = lr0 - 1 + j;
+#pragma unroll
+            for (int kc = 0; kc < 3; ++kc)
+                win[j][kc] = lr < RY ? bits[S - 1][lr][lc - 1 + kc] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int lr = lr0 + j;
+            if (lr >= RY - S) break;           // past the region
+            const int r = row0 + lr;
+            if (!wrap && (r < 0 || r >= rows || c < 0 || c >= cols)) {
+                bits[S][lr][lc] = 0;
+                continue;
+            }
+            int n;
+            float sum;
+            if (INTW) {
+                n = 0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    n += win[j + q / 3][q % 3] ? ki[q] : 0;
+                sum = (float)n;
+            } else {
+                sum = 0.0f;
+#pragma unroll
+                for (int kr = 0; kr < 3; ++kr) {
+#pragma unroll
+                    for (int kc = 0; kc < 3; ++kc) {
+                        if (!wrap) {
+                            const int rr = r + kr - 1, cc = c + kc - 1;
+                            if (rr < 0 || rr >= rows || cc < 0 || cc >= cols)
+                                continue;
+                        }
+                        sum += (float)win[j + kr][kc] * kw[kr * 3 + kc];
+                    }
+                }
+                n = (int)sum;
+            }
+            const uint32_t w = w0[lr][lc];
+            const bool prev_live = (w >> (S - 1)) & 1u;
+            const bool live = prev_live ? (n >= smin && n <= smax)
+                                        : (n >= bmin && n <= bmax);
+            bits[S][lr][lc] = live ? 1 : 0;
+            if (S == K && r < rows && c < cols) {
+                // m == 0: a tile cell (r, c >= 0 there)
+                uint32_t word = w >> K;
+#pragma unroll
+                for (int q = 1; q < K; ++q)
+                    word |= (uint32_t)bits[q][lr][lc] << (31 - K + q);
+                word |= (live ? 1u : 0u) << 31;
+                const long idx = (long)r * cols + c;
+                __builtin_nontemporal_store(word, &out[idx]);
+                if (input) input[idx] = sum;
+            }
+        }
+    }
+}
+
+template <bool INTW, int TY, int K>
+__global__ __launch_bounds__(kGolTx * kGolWy) void gol_multi_kernel(
+        const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+        float *__restrict__ input, int rows, int cols, int wrap, float k0,
+        float k1, float k2, float k3, float k4, float k5, float k6, float k7,
+        float k8, int smin, int smax, int bmin, int bmax) {
+    constexpr int RY = TY + 2 * K, RX = kGolTx + 2 * K;
+    __shared__ uint32_t w0[RY][RX];
+    __shared__ uint8_t bits[K + 1][RY][RX];   // newest bit after step s
+    const int tid = threadIdx.y * kGolTx + threadIdx.x;
+    // board coordinates of the region's first row and column
+    const int row0 = blockIdx.y * TY - K, col0 = blockIdx.x * kGolTx - K;
+    for (int i = tid; i < RY * RX; i += kGolTx * kGolWy) {
+        const int lr = i / RX, lc = i - lr * RX;
+        int r = row0 + lr, c = col0 + lc;
+        uint32_t v = 0;
+        if (wrap) {
+            r = gol_wrap_k(r, rows);
+            c = gol_wrap_k(c, cols);
+            v = in[(long)r * cols + c];
+        } else if (r >= 0 && r < rows && c >= 0 && c < cols) {
+            v = in[(long)r * cols + c];
+        }
+        w0[lr][lc] = v;
+        bits[0][lr][lc] = (uint8_t)(v >> 31);
+    }
+    __syncthreads();
+    const float kw[9] = {k0, k1, k2, k3, k4, k5, k6, k7, k8};
+    int ki[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) ki[q] = (int)kw[q];
+    gol_multi_step<INTW, TY, K, 1>(w0, bits, out, input, rows, cols, wrap,
+                                   row0, col0, kw, ki, smin, smax, bmin, bmax);
+    __syncthreads();
+    gol_multi_step<INTW, TY, K, 2>(w0, bits, out, input, rows, cols, wrap,
+                                   row0, col0, kw, ki, smin, smax, bmin, bmax);
+    if constexpr (K >= 3) {
+        __syncthreads();
+        gol_multi_step<INTW, TY, K, 3>(w0, bits, out, input, rows, cols,
+                                       wrap, row0, col0, kw, ki, smin, smax,
+                                       bmin, bmax);
+    }
+    if constexpr (K >= 4) {
+        __syncthreads();
+        gol_multi_step<INTW, TY, K, 4>(w0, bits, out, input, rows, cols,
+                                       wrap, row0, col0, kw, ki, smin, smax,
+                                       bmin, bmax);
+    }
+}
+
+// Kernels whose nine weights are each 0 or 1 (the game of life's own:
+// ones around a cleared centre): K steps (1 <= K <= 4) on rows of 64 cells
+// packed into 64-bit masks.  A block owns a 64 x 64 region of cells (rows
+// row0 .., columns col0 ..) and writes its inner (64 - 2K) x (64 - 2K)
+// tile: each step's new bits are exact one cell further in from the
+// region's edge than the previous step's.  Its four wavefronts stage 16
+// region rows each, lane l = column col0 + l, and pack each row's newest
+// bits, the K survival bits (bit s - 1 of the word for step s) and, on a
+// bounded board, the cells inside it, by ballot.  One wavefront then takes
+// all K steps, lane i = region row i, with the rows above and below from
+// its neighbour lanes: the neighbour count of 64 cells at once as four bit
+// planes (a ripple add of each weighted neighbour mask), the rule as the
+// union of the count's equality masks for the values the survival and
+// birth ranges admit.  The count equals the integer sum of the
+// one-weighted bits, which is the float sum of the one-step kernel
+// exactly; cells outside a bounded board hold 0 at every step.  Finally
+// each wavefront writes its rows' inner words: the staged word shifted by
+// K with the K new bits on top, and the last step's count as the input.
+template <int K>
+__global__ __launch_bounds__(256) void gol_packed_kernel(
+        const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+        float *__restrict__ input, int rows, int cols, int wrap,
+        uint32_t wmask, uint32_t surv, uint32_t birth) {
+    constexpr int TI = 64 - 2 * K;           // inner tile
+    __shared__ uint64_t newest[64], inside[64], keep[K][64];
+    __shared__ uint64_t step_bits[K][64], planes[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = blockIdx.y * TI - K, col0 = blockIdx.x * TI - K;
+    const int c = col0 + lane;
+    // the 16 rows' addresses first (cells outside a bounded board read
+    // cell 0 and are cleared), then all 16 loads at once
+    const int cw = wrap ? gol_wrap_k(c, cols) : c;
+    const bool col_ok = wrap || (c >= 0 && c < cols);
+    uint32_t words[16];
+    bool okv[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int r = row0 + wave * 16 + j;
+        const int rw = wrap ? gol_wrap_k(r, rows) : r;
+        okv[j] = col_ok && (wrap || (r >= 0 && r < rows));
+        words[j] = in[okv[j] ? (long)rw * cols + cw : 0];
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int i = wave * 16 + j;
+        const bool ok = okv[j];
+        const uint32_t v = ok ? words[j] : 0u;
+        words[j] = v;
+        const uint64_t nb = __ballot((v >> 31) != 0u);
+        const uint64_t ib = __ballot(ok);
+        uint64_t kb[K];
+#pragma unroll
+        for (int s = 0; s < K; ++s) kb[s] = __ballot(((v >> s) & 1u) != 0u);
+        if (lane == 0) {
+            newest[i] = nb;
+            inside[i] = ib;
+#pragma unroll
+            for (int s = 0; s < K; ++s) keep[s][i] = kb[s];
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint64_t b = newest[lane];
+        const uint64_t ib = inside[lane];
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            // rows above and below, every lane taking part in both
+            // shuffles (a lane left out of one supplies 0 to its
+            // neighbour); the edge rows get their own row instead, which
+            // only touches the rows no output depends on
+            const uint64_t a = __shfl_up(b, 1);
+            const uint64_t d = __shfl_down(b, 1);
+            const uint64_t nbr[9] = {a << 1, a, a >> 1, b << 1, b, b >> 1,
+                                     d << 1, d, d >> 1};
+            uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                if (!((wmask >> q) & 1u)) continue;
+                const uint64_t x = nbr[q];
+                const uint64_t k0 = c0 & x;
+                c0 ^= x;
+                const uint64_t k1 = c1 & k0;
+                c1 ^= k0;
+                const uint64_t k2 = c2 & k1;
+                c2 ^= k1;
+                c3 |= k2;
+            }
+            uint64_t sv = 0, bv = 0;
+#pragma unroll
+            for (int v = 0; v <= 9; ++v) {
+                if (!(((surv | birth) >> v) & 1u)) continue;
+                const uint64_t eq = ((v & 1) ? c0 : ~c0) & ((v & 2) ? c1 : ~c1) &
+                                    ((v & 4) ? c2 : ~c2) & ((v & 8) ? c3 : ~c3);
+                if ((surv >> v) & 1u) sv |= eq;
+                if ((birth >> v) & 1u) bv |= eq;
+            }
+            const uint64_t p = keep[s][lane];
+            b = ((p & sv) | (~p & bv)) & ib;
+            step_bits[s][lane] = b;
+            if (s == K - 1) {
+                planes[0][lane] = c0;
+                planes[1][lane] = c1;
+                planes[2][lane] = c2;
+                planes[3][lane] = c3;
+            }
+        }
+    }
+    __syncthreads();
+    if (lane < K || lane >= 64 - K) return;
+    if (c >= cols) return;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int i = wave * 16 + j;
+        const int r = row0 + i;
+        if (i < K || i >= 64 - K || r >= rows) continue;
+        uint32_t word = words[j] >> K;
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+            word |= (uint32_t)((step_bits[s][i] >> lane) & 1ull)
+                    << (32 - K + s);
+        const long idx = (long)r * cols + c;
+        __builtin_nontemporal_store(word, &out[idx]);
+        if (input) {
+            const int n = (int)((planes[0][i] >> lane) & 1ull) |
+                          (int)((planes[1][i] >> lane) & 1ull) << 1 |
+                          (int)((planes[2][i] >> lane) & 1ull) << 2 |
+                          (int)((planes[3][i] >> lane) & 1ull) << 3;
+            input[idx] = (float)n;
+        }
+    }
+}
+
 }  // namespace sg
 
 extern "C" int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
@@ -169,6 +456,69 @@ extern "C" int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
     auto *kern = ty == 32
         ? (intw ? gol_step_kernel<true, 32> : gol_step_kernel<false, 32>)
         : (intw ? gol_step_kernel<true, 16> : gol_step_kernel<false, 16>);
+    hipLaunchKernelGGL(kern, g, dim3(kGolTx, kGolWy), 0, (hipStream_t)stream,
+                       words_in, words_out, input, rows, cols, wrap, k[0],
+                       k[1], k[2], k[3], k[4], k[5], k[6], k[7], k[8], smin,
+                       smax, bmin, bmax);
+    return hipGetLastError();
+}
+
+extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
+                                float *input, int rows, int cols, int wrap,
+                                const float *k, int smin, int smax, int bmin,
+                                int bmax, int steps, void *stream) {
+    using namespace sg;
+    if (steps < 1 || steps > 4) return (int)hipErrorInvalidValue;
+    if (rows <= 0 || cols <= 0) return 0;
+    // weights of 0 and 1 only: the packed form
+    uint32_t wmask = 0;
+    bool bitw = true;
+    for (int q = 0; q < 9; ++q) {
+        if (k[q] == 1.0f) wmask |= 1u << q;
+        else if (k[q] != 0.0f) bitw = false;   // -0.0f counts as 0
+    }
+    if (bitw) {
+        // rule ranges as flags over the counts 0 .. 9
+        uint32_t surv = 0, birth = 0;
+        for (int v = 0; v <= 9; ++v) {
+            if (v >= smin && v <= smax) surv |= 1u << v;
+            if (v >= bmin && v <= bmax) birth |= 1u << v;
+        }
+        const int ti = 64 - 2 * steps;
+        dim3 g((cols + ti - 1) / ti, (rows + ti - 1) / ti);
+        typedef void (*PK)(const uint32_t *, uint32_t *, float *, int, int,
+                           int, uint32_t, uint32_t, uint32_t);
+        const PK kern = steps == 1 ? (PK)gol_packed_kernel<1>
+                      : steps == 2 ? (PK)gol_packed_kernel<2>
+                      : steps == 3 ? (PK)gol_packed_kernel<3>
+                                   : (PK)gol_packed_kernel<4>;
+        hipLaunchKernelGGL(kern, g, dim3(256), 0, (hipStream_t)stream,
+                           words_in, words_out, input, rows, cols, wrap,
+                           wmask, surv, birth);
+        return hipGetLastError();
+    }
+    if (steps == 1)
+        return sg_hip_gol_step(words_in, words_out, input, rows, cols, wrap,
+                               k, smin, smax, bmin, bmax, stream);
+    const long tiles32 = (long)((cols + kGolTx - 1) / kGolTx) *
+                         ((rows + 31) / 32);
+    const int ty = tiles32 >= 2048 ? 32 : 16;
+    dim3 g((cols + kGolTx - 1) / kGolTx, (rows + ty - 1) / ty);
+    bool intw = true;
+    for (int q = 0; q < 9; ++q)
+        intw = intw && std::fabs(k[q]) <= 1048576.0f &&
+               k[q] == (float)(int)k[q];
+    typedef void (*Kern)(const uint32_t *, uint32_t *, float *, int, int, int,
+                         float, float, float, float, float, float, float,
+                         float, float, int, int, int, int);
+#define SG_GOL_K(KS)                                                         \
+    (ty == 32 ? (intw ? (Kern)gol_multi_kernel<true, 32, KS>                 \
+                      : (Kern)gol_multi_kernel<false, 32, KS>)               \
+              : (intw ? (Kern)gol_multi_kernel<true, 16, KS>                 \
+                      : (Kern)gol_multi_kernel<false, 16, KS>))
+    const Kern kern = steps == 2 ? SG_GOL_K(2)
+                    : steps == 3 ? SG_GOL_K(3) : SG_GOL_K(4);
+#undef SG_GOL_K
     hipLaunchKernelGGL(kern, g, dim3(kGolTx, kGolWy), 0, (hipStream_t)stream,
                        words_in, words_out, input, rows, cols, wrap, k[0],
                        k[1], k[2], k[3], k[4], k[5], k[6], k[7], k[8], smin,

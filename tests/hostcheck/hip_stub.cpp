@@ -146,6 +146,8 @@ SG_STUB(sg_hip_rate_neuron, float *, float *, int, int, void *)
 SG_STUB(sg_hip_relay_neuron, float *, float *, int, int, int, void *)
 SG_STUB(sg_hip_gol_step, const uint32_t *, uint32_t *, float *, int, int, int,
         const float *, int, int, int, int, void *)
+SG_STUB(sg_hip_gol_steps, const uint32_t *, uint32_t *, float *, int, int,
+        int, const float *, int, int, int, int, int, void *)
 SG_STUB(sg_hip_init_random, float *, int, int, float, float, float, int,
         uint64_t, uint64_t, const uint64_t *, void *)
 SG_STUB(sg_hip_add_u64, uint64_t *, uint64_t, void *)

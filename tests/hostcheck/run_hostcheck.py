@@ -77,6 +77,16 @@ def main():
             bench.run(net, n, {"step graphs": "true"})
         net.free()
         print("%-16s ok" % ("workload " + name), flush=True)
+    # A game-of-life layer that nothing else reads takes up to four steps
+    # per launch in eager runs (one per step when captured into graphs).
+    net = bench.build(lib, workloads.gol(64))
+    for n, launches in ((10, 3), (7, 2), (1, 1), (4, 1), (13, 4)):
+        rep = net.run(None, {"iterations": str(n)})
+        assert rep.get("iterations") == str(n), rep
+        got = int(rep.get("game of life launches"))
+        assert got == launches, (n, got, launches)
+    net.free()
+    print("%-16s ok" % "gol grouping", flush=True)
     stub = ctypes.CDLL(LIB)
     stub.hip_stub_bad_waits.restype = ctypes.c_long
     bad = stub.hip_stub_bad_waits()

@@ -102,6 +102,54 @@ def test_full_size_game_of_life_captured(gpu_lib):
     _check(case, gpu_lib)
 
 
+GOL_MULTI = [
+    # rows, cols, wrap, steps, kernel, seed; kernel "std" (ones around a
+    # cleared centre: the packed form), "int" (integers -2 .. 3) or "float"
+    (17, 23, True, 37, "std", 311),     # boards smaller than a tile + halo
+    (9, 11, False, 13, "std", 312),
+    (24, 20, True, 6, "float", 313),
+    (70, 130, False, 11, "float", 314),  # ragged edge tiles
+    (2050, 2100, True, 7, "std", 315),  # several tiles in each direction
+    (2048, 2112, False, 5, "float", 316),
+    (130, 70, True, 9, "int", 317),
+    (300, 260, False, 10, "int", 318),
+    (1024, 1024, True, 21, "std", 319),
+    (5, 3, True, 8, "std", 320),        # narrower than the halo
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows,cols,wrap,steps,kern,seed", GOL_MULTI,
+                         ids=["%dx%d%s_%d_%s" % (r, c, "w" if w else "", n, k)
+                              for r, c, w, n, k, _ in GOL_MULTI])
+def test_game_of_life_several_steps_per_launch(gpu_lib, rows, cols, wrap,
+                                               steps, kern, seed):
+    """A board that nothing else reads runs up to four steps per launch
+    (groups of 4, then the remainder): final history words and the last
+    step's neighbour sums bit for bit against the oracle, on wrapping and
+    bounded boards, with the standard kernel (the bit-packed form), other
+    integer kernels and a float kernel (the float sum in row-major order,
+    bounded boards skipping outside cells)."""
+    print("game of life %dx%d wrap=%s steps=%d kernel=%s" %
+          (rows, cols, wrap, steps, kern), flush=True)
+    case = netdefs.gol("gol_multi_%d" % seed, rows, cols, steps, seed,
+                       wrap=wrap)
+    case.capture = []
+    rng = np.random.default_rng(seed)
+    if kern == "float":
+        k = rng.normal(0.0, 1.2, 9).astype(np.float32)
+    elif kern == "int":
+        k = rng.integers(-2, 4, 9).astype(np.float32)
+    if kern != "std":
+        k[4] = 0.0   # the centre, cleared by "diagonal": false
+        case.weights = {"life": k}
+    captures, finals, report = harness.run_case(case, gpu_lib)
+    assert report.get("iterations") == str(steps)
+    assert report.get("game of life launches") == str((steps + 3) // 4)
+    problems = stepper.compare(case, captures, finals)
+    assert not problems, "\n".join(problems)
+
+
 @pytest.mark.gpu
 def test_learning_flag_off_freezes_weights(gpu_lib):
     """With "learning flag" false no updater runs: weights and every
