@@ -179,15 +179,20 @@ def _workloads():
                      step_bytes=4228000, cpu_steps=300, detail="",
                      data="synthetic (Philox-seeded weights U[0,0.5); "
                           "device-drawn N(5,2) input into L1)"),
+        # launch_bytes: one launch reads and writes every word once, for
+        # the several steps it takes (8 on the 1024 board, 4 on the 16384
+        # one; "game of life launches" in the report)
         "gol1024": dict(make=lambda: workloads.gol(1024), kernel="gol_step",
-                        label="gol_step (fused 3x3 count + rule + shift)",
+                        label="gol_step (8 steps per launch on bit-packed "
+                              "rows: 3x3 count + rule + shift)",
                         launch_bytes=8 * 1024 * 1024,
                         step_bytes=8 * 1024 * 1024, cpu_steps=40,
                         detail=" (3x3 convolutional, wrap, self excluded)",
                         data="synthetic (Philox Bernoulli(0.5) cells)"),
         "gol16384": dict(make=lambda: workloads.gol(16384),
                          kernel="gol_step",
-                         label="gol_step (fused 3x3 count + rule + shift)",
+                         label="gol_step (4 steps per launch on bit-packed "
+                               "rows: 3x3 count + rule + shift)",
                          launch_bytes=8 * 16384 * 16384,
                          step_bytes=8 * 16384 * 16384, cpu_steps=1,
                          detail=" (3x3 convolutional, wrap, self excluded)",
@@ -877,7 +882,8 @@ def main():
         "kernel_avg_ms": kernels,
         "report": {k: report.get(k) for k in ("activator streams",
                                                "compact records", "streams",
-                                               "step graphs", "graph steps")
+                                               "step graphs", "graph steps",
+                                               "game of life launches")
                    if report.get(k) is not None},
         "run_args": extra or None,
         "cpu_baseline": cpu,

@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_multi_kernel(
 }
 
 // Kernels whose nine weights are each 0 or 1 (the game of life's own:
-// ones around a cleared centre): K steps (1 <= K <= 4) on rows of 64 cells
+// ones around a cleared centre): K steps (1 <= K <= 8) on rows of 64 cells
 // packed into 64-bit masks.  A block owns a 64 x 64 region of cells (rows
 // row0 .., columns col0 ..) and writes its inner (64 - 2K) x (64 - 2K)
 // tile: each step's new bits are exact one cell further in from the
@@ -463,12 +463,23 @@ extern "C" int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
     return hipGetLastError();
 }
 
+extern "C" int sg_hip_gol_max_steps(const float *k, int rows, int cols) {
+    for (int q = 0; q < 9; ++q)
+        if (k[q] != 0.0f && k[q] != 1.0f) return 4;
+    // the packed form: 8 steps per launch where the launch, not the board's
+    // traffic, bounds the step (a 48 x 48 tile of 8 steps reads 1.78 words
+    // per cell, a 56 x 56 tile of 4 steps 1.31)
+    const long tiles8 = (long)((cols + 47) / 48) * ((rows + 47) / 48);
+    return tiles8 < 2048 ? 8 : 4;
+}
+
 extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
                                 float *input, int rows, int cols, int wrap,
                                 const float *k, int smin, int smax, int bmin,
                                 int bmax, int steps, void *stream) {
     using namespace sg;
-    if (steps < 1 || steps > 4) return (int)hipErrorInvalidValue;
+    if (steps < 1 || steps > sg_hip_gol_max_steps(k, rows, cols))
+        return (int)hipErrorInvalidValue;
     if (rows <= 0 || cols <= 0) return 0;
     // weights of 0 and 1 only: the packed form
     uint32_t wmask = 0;
@@ -488,10 +499,11 @@ extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
         dim3 g((cols + ti - 1) / ti, (rows + ti - 1) / ti);
         typedef void (*PK)(const uint32_t *, uint32_t *, float *, int, int,
                            int, uint32_t, uint32_t, uint32_t);
-        const PK kern = steps == 1 ? (PK)gol_packed_kernel<1>
-                      : steps == 2 ? (PK)gol_packed_kernel<2>
-                      : steps == 3 ? (PK)gol_packed_kernel<3>
-                                   : (PK)gol_packed_kernel<4>;
+        static const PK kerns[8] = {
+            gol_packed_kernel<1>, gol_packed_kernel<2>, gol_packed_kernel<3>,
+            gol_packed_kernel<4>, gol_packed_kernel<5>, gol_packed_kernel<6>,
+            gol_packed_kernel<7>, gol_packed_kernel<8>};
+        const PK kern = kerns[steps - 1];
         hipLaunchKernelGGL(kern, g, dim3(256), 0, (hipStream_t)stream,
                            words_in, words_out, input, rows, cols, wrap,
                            wmask, surv, birth);

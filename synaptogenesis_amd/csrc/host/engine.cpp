@@ -691,8 +691,8 @@ class Runner {
     std::map<LayerState *, ConnState *> gol_fast_;
     std::map<LayerState *, uint32_t *> gol_home_;
     // game-of-life layers that nothing else reads (their only outgoing
-    // connection is the self-connection): stepped up to 4 steps per launch
-    // in eager runs; gol_ahead_ counts the steps an earlier launch already
+    // connection is the self-connection): stepped up to
+    // sg_hip_gol_max_steps steps per launch in eager runs; gol_ahead_ counts the steps an earlier launch already
     // took beyond the current one, gol_rem_ the steps left in the run from
     // the current one (1 while steps are captured into graphs)
     std::set<LayerState *> gol_multi_;
@@ -1208,16 +1208,19 @@ void Runner::update_neurons(const std::vector<LayerState *> &lss, int sidx,
                         --ahead;
                         break;
                     }
+                    float k9[9];
+                    std::memcpy(k9, gf->second->arr("weights").f(), 36);
                     const int k = gol_multi_.count(ls)
-                        ? (int)std::min<long>(4, std::max<long>(1, gol_rem_))
+                        ? (int)std::min<long>(
+                              sg_hip_gol_max_steps(k9, ls->layer->rows,
+                                                   ls->layer->cols),
+                              std::max<long>(1, gol_rem_))
                         : 1;
                     // the input (neighbour sums) is written at the run's
                     // last step only
                     const bool last = k == 1 ? x.last
                                              : iterations_ != 0 && k == gol_rem_;
                     uint32_t *spare = C_->gol_spare.at(ls);
-                    float k9[9];
-                    std::memcpy(k9, gf->second->arr("weights").f(), 36);
                     hipStream_t s = sched().begin(
                         sidx, {rout, rin, res(RES_SYN, gf->second)},
                         {rout, rin});

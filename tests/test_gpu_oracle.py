@@ -8,6 +8,8 @@ bit for bit, except rate encoding (tanh, see test_gpu_golden.py).
 """
 
 import copy
+import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -15,6 +17,8 @@ import pytest
 import harness
 import netdefs
 from oracle import stepper
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 FRESH = [
     lambda: netdefs.conn_zoo(steps=10, workload=201, name="conn_zoo_201"),
@@ -102,6 +106,20 @@ def test_full_size_game_of_life_captured(gpu_lib):
     _check(case, gpu_lib)
 
 
+_STD_KERNEL = np.float32([1, 1, 1, 1, 0, 1, 1, 1, 1])
+
+
+def _gol_max_steps(kernel9, rows, cols):
+    """sg_hip_gol_max_steps of the MI355X library (include/sg_hip.h)."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "synaptogenesis_amd",
+                                   "libsg_hip.so"))
+    lib.sg_hip_gol_max_steps.restype = ctypes.c_int
+    lib.sg_hip_gol_max_steps.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.c_int]
+    k = np.ascontiguousarray(kernel9, dtype=np.float32)
+    return lib.sg_hip_gol_max_steps(k.ctypes.data, rows, cols)
+
+
 GOL_MULTI = [
     # rows, cols, wrap, steps, kernel, seed; kernel "std" (ones around a
     # cleared centre: the packed form), "int" (integers -2 .. 3) or "float"
@@ -115,6 +133,7 @@ GOL_MULTI = [
     (300, 260, False, 10, "int", 318),
     (1024, 1024, True, 21, "std", 319),
     (5, 3, True, 8, "std", 320),        # narrower than the halo
+    (4096, 4096, True, 6, "std", 321),  # large: 4 steps per launch
 ]
 
 
@@ -124,8 +143,9 @@ GOL_MULTI = [
                               for r, c, w, n, k, _ in GOL_MULTI])
 def test_game_of_life_several_steps_per_launch(gpu_lib, rows, cols, wrap,
                                                steps, kern, seed):
-    """A board that nothing else reads runs up to four steps per launch
-    (groups of 4, then the remainder): final history words and the last
+    """A board that nothing else reads runs several steps per launch
+    (groups of 8 with the standard kernel, of 4 with others, then the
+    remainder): final history words and the last
     step's neighbour sums bit for bit against the oracle, on wrapping and
     bounded boards, with the standard kernel (the bit-packed form), other
     integer kernels and a float kernel (the float sum in row-major order,
@@ -145,7 +165,11 @@ def test_game_of_life_several_steps_per_launch(gpu_lib, rows, cols, wrap,
         case.weights = {"life": k}
     captures, finals, report = harness.run_case(case, gpu_lib)
     assert report.get("iterations") == str(steps)
-    assert report.get("game of life launches") == str((steps + 3) // 4)
+    per = _gol_max_steps(case.weights.get("life", _STD_KERNEL), rows, cols)
+    assert per == (4 if kern != "std" or rows * cols > 2048 * 48 * 48
+                   else 8)
+    assert report.get("game of life launches") == str((steps + per - 1) //
+                                                      per)
     problems = stepper.compare(case, captures, finals)
     assert not problems, "\n".join(problems)
 
