@@ -481,7 +481,7 @@ sg_conn conn_desc(ConnState &cs, bool store_sum) {
     // sources are packed per activation into the connection's own.
     if (cs.from->model == Model::IZHIKEVICH && c.delay == 0 &&
         cs.from->d_plane) {
-        d.plane = cs.from->plane_read();
+        d.plane = cs.from->d_plane;
         d.plane_ready = 1;
     } else {
         d.plane = cs.d_plane;
@@ -692,10 +692,9 @@ class Runner {
     std::map<LayerState *, uint32_t *> gol_home_;
     // game-of-life layers that nothing else reads (their only outgoing
     // connection is the self-connection): stepped up to
-    // sg_hip_gol_max_steps steps per launch in eager runs; gol_ahead_
-    // counts the steps an earlier launch already took beyond the current
-    // one, gol_rem_ the steps left in the run from the current one (1
-    // while steps are captured into graphs)
+    // sg_hip_gol_max_steps steps per launch in eager runs; gol_ahead_ counts the steps an earlier launch already
+    // took beyond the current one, gol_rem_ the steps left in the run from
+    // the current one (1 while steps are captured into graphs)
     std::set<LayerState *> gol_multi_;
     std::map<LayerState *, int> gol_ahead_;
     long gol_rem_ = 1;
@@ -1145,7 +1144,6 @@ void Runner::update_neurons(const std::vector<LayerState *> &lss, int sidx,
     std::vector<sg_iz_layer> iz_layers;
     std::vector<Res> reads, writes;
     std::vector<ConnState *> stepped;   // time since spike stepped
-    std::vector<LayerState *> planed;   // spike planes written
     for (LayerState *ls : lss) {
         const int n = ls->layer->size;
         const Res rin = res(RES_IN, ls), rout = res(RES_OUT, ls),
@@ -1167,8 +1165,7 @@ void Runner::update_neurons(const std::vector<LayerState *> &lss, int sidx,
                 l.time_since_spike = ls->var("time since spike").di();
                 l.a = ls->var("a").df(); l.b = ls->var("b").df();
                 l.c = ls->var("c").df(); l.d = ls->var("d").df();
-                l.plane = ls->plane_write();
-                planed.push_back(ls);
+                l.plane = ls->d_plane;
                 reads.insert(reads.end(), {rin, rout, rst});
                 writes.insert(writes.end(), {rin, rout, rst});
                 for (ConnState *cs : tss_out_[ls]) {
@@ -1281,7 +1278,6 @@ void Runner::update_neurons(const std::vector<LayerState *> &lss, int sidx,
         });
         sched().end();
         for (ConnState *cs : stepped) cs->tss_cur = 1 - cs->tss_cur;
-        for (LayerState *ls : planed) ls->plane_cur = 1 - ls->plane_cur;
     }
 }
 
@@ -1486,8 +1482,6 @@ std::vector<uint64_t> Runner::graph_key(int block) const {
                     ((uint64_t)csp->fused_gol << 2));
     for (auto &kv : gol_fast_)
         k.push_back((uint64_t)(uintptr_t)kv.first->output.dev);
-    for (auto &lsp : st_->layers())
-        if (lsp->d_plane) k.push_back((uint64_t)lsp->plane_cur);
     return k;
 }
 
@@ -1613,7 +1607,7 @@ Props *Runner::run() {
             const Res r = res(RES_OUT, ls);
             hipStream_t s = S.begin(0, {r}, {r});
             check_hip(sg_hip_pack_plane(ls->output.du(), ls->layer->size, 0,
-                                        ls->plane_read(), s), "pack plane");
+                                        ls->d_plane, s), "pack plane");
             S.end();
         }
         const int n = ls->layer->size;

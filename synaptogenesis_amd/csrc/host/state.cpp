@@ -1122,8 +1122,8 @@ void State::to_device(bool allow_compact) {
         for (auto &kv : ls->vars) upload(kv.second);
         if (ls->model == Model::IZHIKEVICH && !ls->d_plane)
             check_hip(hipMalloc((void **)&ls->d_plane,
-                                ls->plane_words() * 8 * 2),
-                      "hipMalloc layer planes");
+                                (size_t)(ls->layer->size + 63) / 64 * 8),
+                      "hipMalloc layer plane");
     }
     for (auto &cs : conns_) {
         cs->packed = record_packed(*cs);

@@ -59,19 +59,9 @@ struct LayerState {
     int survival_min = 2, survival_max = 3, birth_min = 3, birth_max = 3;
     bool ramp = false;
 
-    // Izhikevich layers: planes of the newest spike bits (one bit per
-    // neuron) for delay-0 base activators, two of them: the neuron launch
-    // writes plane_write() while base activators it runs itself read the
-    // sources' plane_read(); the engine swaps them after each launch.
+    // Izhikevich layers: plane of the newest spike bits (one bit per
+    // neuron), written by the neuron kernel for delay-0 base activators.
     uint64_t *d_plane = nullptr;
-    int plane_cur = 0;
-    size_t plane_words() const { return (size_t)(layer->size + 63) / 64; }
-    uint64_t *plane_read() const {
-        return d_plane ? d_plane + plane_cur * plane_words() : nullptr;
-    }
-    uint64_t *plane_write() const {
-        return d_plane ? d_plane + (1 - plane_cur) * plane_words() : nullptr;
-    }
 
     Array &var(const std::string &k) { return vars.at(k); }
 };
