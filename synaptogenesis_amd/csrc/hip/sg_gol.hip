@@ -468,7 +468,9 @@ extern "C" int sg_hip_gol_max_steps(const float *k, int rows, int cols) {
         if (k[q] != 0.0f && k[q] != 1.0f) return 4;
     // the packed form: 8 steps per launch where the launch, not the board's
     // traffic, bounds the step (a 48 x 48 tile of 8 steps reads 1.78 words
-    // per cell, a 56 x 56 tile of 4 steps 1.31)
+    // per cell, a 56 x 56 tile of 4 steps 1.31); 16 steps (32 x 32 tiles)
+    // measured no faster on a 1024 board: one wavefront per block steps the
+    // rows, and the step's instructions, not the launch, then bound it
     const long tiles8 = (long)((cols + 47) / 48) * ((rows + 47) / 48);
     return tiles8 < 2048 ? 8 : 4;
 }

@@ -78,8 +78,8 @@ def main():
         net.free()
         print("%-16s ok" % ("workload " + name), flush=True)
     # A game-of-life layer that nothing else reads takes up to eight steps
-    # per launch in eager runs with its 0/1 kernel (one per step when
-    # captured into graphs).
+    # per launch in eager runs with its 0/1 kernel on a small board (one per
+    # step when captured into graphs).
     net = bench.build(lib, workloads.gol(64))
     for n, launches in ((10, 2), (7, 1), (1, 1), (8, 1), (17, 3)):
         rep = net.run(None, {"iterations": str(n)})
