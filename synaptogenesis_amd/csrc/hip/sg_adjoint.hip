@@ -90,6 +90,18 @@ __device__ __forceinline__ float act_grad_fast(float u, int mode, float slope) {
     return mode == SG_SNN_SPIKE ? g : 0.5f * slope * g;
 }
 
+// act and act_grad_fast for a mode fixed at compile time (MODE < 0: the
+// run's mode): the same operations, without the other mode's.
+template <int MODE>
+__device__ __forceinline__ float act_m(float u, int mode, float slope) {
+    return act(u, MODE < 0 ? mode : MODE, slope);
+}
+template <int MODE>
+__device__ __forceinline__ float act_grad_fast_m(float u, int mode,
+                                                 float slope) {
+    return act_grad_fast(u, MODE < 0 ? mode : MODE, slope);
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
@@ -883,7 +895,10 @@ __global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
     const int H = a.H;
     if (i >= a.B * H) return;
     const long BH = (long)a.B * H;
-    float v = state[i], s = state[BH + i];
+    // the run starts from v1 = s1 = 0, written as checkpoint 0
+    float v = 0.0f, s = 0.0f;
+    state[i] = 0.0f;
+    state[BH + i] = 0.0f;
     int left = ck;
     float *next = state + slot;
     for (int t0 = 0; t0 < T; t0 += kWin) {
@@ -1002,7 +1017,10 @@ __global__ __launch_bounds__(256) void out_current_mfma(
 // Layer 2 over every step from the all-steps currents; thread = (b, o):
 // v2 of every step to rec_v2, the state after each block of ck steps to
 // that block's checkpoint slot, the spike counts to counts.  The currents
-// of 16 steps are loaded together.
+// of 16 steps are loaded together, the next 16 while these are stepped
+// (the grid is only batch x outputs threads: each thread's chain of loads
+// and steps is the launch).  MODE as in scan_bwd_fused.
+template <int MODE>
 __global__ void scan2_fwd_all(ScanArgs a, int T, int ck, long slot,
                               const float *__restrict__ a2,
                               float *__restrict__ state,
@@ -1012,21 +1030,30 @@ __global__ void scan2_fwd_all(ScanArgs a, int T, int ck, long slot,
     const int O = a.O;
     if (i >= a.B * O) return;
     const long BH = (long)a.B * a.H, BO = (long)a.B * O;
-    float v = state[2 * BH + i], s = state[2 * BH + BO + i];
+    // the run starts from v2 = s2 = 0, written as checkpoint 0
+    float v = 0.0f, s = 0.0f;
+    state[2 * BH + i] = 0.0f;
+    state[2 * BH + BO + i] = 0.0f;
     float cnt = 0.0f;
     int left = ck;
     float *next = state + slot;
     constexpr int W = 16;
+    float in_n[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j)
+        in_n[j] = j < T ? a2[(long)j * BO + i] : 0.0f;
     for (int t0 = 0; t0 < T; t0 += W) {
         float in[W];
 #pragma unroll
+        for (int j = 0; j < W; ++j) in[j] = in_n[j];
+#pragma unroll
         for (int j = 0; j < W; ++j)
-            if (t0 + j < T) in[j] = a2[(long)(t0 + j) * BO + i];
+            if (t0 + W + j < T) in_n[j] = a2[(long)(t0 + W + j) * BO + i];
 #pragma unroll
         for (int j = 0; j < W; ++j) {
             if (t0 + j >= T) break;
             v = a.beta * v - a.reset_theta * s + in[j];
-            s = act(v - a.theta, a.mode, a.slope);
+            s = act_m<MODE>(v - a.theta, a.mode, a.slope);
             cnt += s;
             rec_v2[(long)(t0 + j) * BO + i] = v;
             if (--left == 0) {
@@ -1084,29 +1111,34 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
 //    and the block's share of dW2, the sum over its steps (latest first)
 //    of d2[o] * s1[u], added to a running sum in block order.
 // The dW2 sums go to part2[b][o][u] (dw2_reduce adds the batch rows in a
-// fixed order).  No scan2_bwd, dw2_partial or recomputing launch.
+// fixed order).  No scan2_bwd, dw2_partial or recomputing launch.  MODE
+// (>= 0) and OUT (> 0) fix the mode and the number of outputs at compile
+// time: the per-step loops then carry no guard or select for the other
+// mode or for outputs past O.
+template <int MODE, int OUT>
 __global__ __launch_bounds__(256) void scan_bwd_fused(
         ScanArgs a, int nblk, long slot, const uint16_t *__restrict__ w2,
         const float *__restrict__ a1, const float *__restrict__ state,
         const float *__restrict__ rec_v2, const float *__restrict__ g,
         float *__restrict__ lam, float *__restrict__ part2) {
+    constexpr int NO = OUT > 0 ? OUT : kMaxOut;   // output slots walked
     __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int H = a.H, O = a.O;
+    const int H = a.H, O = OUT > 0 ? OUT : a.O;
     const long BH = (long)a.B * H, BO = (long)a.B * O;
     const long M = (long)a.steps * a.B;   // rows of a block of steps
     const int b = i / H, u = i % H;
     const bool out_thread = threadIdx.x < O;
     const int i2 = b * O + (out_thread ? threadIdx.x : 0);
     const float gb = out_thread ? g[i2] : 0.0f;
-    float wc[kMaxOut];
+    float wc[NO];
 #pragma unroll
-    for (int o = 0; o < kMaxOut; ++o)
+    for (int o = 0; o < NO; ++o)
         wc[o] = o < O ? bf16_value(w2[o * H + u]) : 0.0f;
     float dn = 0.0f, dn2 = 0.0f;
-    float tot[kMaxOut];
+    float tot[NO];
 #pragma unroll
-    for (int o = 0; o < kMaxOut; ++o) tot[o] = 0.0f;
+    for (int o = 0; o < NO; ++o) tot[o] = 0.0f;
     // the loads of block blk - 1 (currents, layer 2's potentials, the
     // checkpoint) are issued at the start of block blk and land while it
     // is computed
@@ -1141,7 +1173,8 @@ __global__ __launch_bounds__(256) void scan_bwd_fused(
             for (int t = 15; t >= 0; --t) {
                 if (t >= a.steps) continue;
                 const float d = (gb - a.reset_theta * dn2) *
-                                act_grad_fast(v2[t] - a.theta, a.mode, a.slope) +
+                                act_grad_fast_m<MODE>(v2[t] - a.theta, a.mode,
+                                                      a.slope) +
                                 a.beta * dn2;
                 d2s[t][threadIdx.x] = d;
                 dn2 = d;
@@ -1154,37 +1187,38 @@ __global__ __launch_bounds__(256) void scan_bwd_fused(
         for (int t = 0; t < 16; ++t) {
             if (t >= a.steps) break;
             v = a.beta * v - a.reset_theta * s + in[t];
-            s = act(v - a.theta, a.mode, a.slope);
+            s = act_m<MODE>(v - a.theta, a.mode, a.slope);
             vr[t] = v;
         }
-        float acc[kMaxOut];
+        float acc[NO];
 #pragma unroll
-        for (int o = 0; o < kMaxOut; ++o) acc[o] = 0.0f;
+        for (int o = 0; o < NO; ++o) acc[o] = 0.0f;
         float *lb = lam + blk * M * H;
 #pragma unroll
         for (int t = 15; t >= 0; --t) {
             if (t >= a.steps) continue;
-            const float sv = act(vr[t] - a.theta, a.mode, a.slope);
+            const float sv = act_m<MODE>(vr[t] - a.theta, a.mode, a.slope);
             float gs = 0.0f;
 #pragma unroll
-            for (int o = 0; o < kMaxOut; ++o)
-                if (o < O) {
+            for (int o = 0; o < NO; ++o)
+                if (OUT > 0 || o < O) {
                     gs = fmaf(wc[o], d2s[t][o], gs);
                     acc[o] = fmaf(d2s[t][o], sv, acc[o]);
                 }
             const float d = (gs - a.reset_theta * dn) *
-                            act_grad_fast(vr[t] - a.theta, a.mode, a.slope) +
+                            act_grad_fast_m<MODE>(vr[t] - a.theta, a.mode,
+                                                  a.slope) +
                             a.beta * dn;
             lb[(long)t * BH + i] = d;
             dn = d;
         }
 #pragma unroll
-        for (int o = 0; o < kMaxOut; ++o)
+        for (int o = 0; o < NO; ++o)
             tot[o] = blk == nblk - 1 ? acc[o] : tot[o] + acc[o];
         __syncthreads();   // d2s read by every thread before it changes
     }
 #pragma unroll
-    for (int o = 0; o < kMaxOut; ++o)
+    for (int o = 0; o < NO; ++o)
         if (o < O) part2[((long)b * O + o) * H + u] = tot[o];
 }
 
@@ -1500,7 +1534,9 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     float *cnt = (float *)(ws + l.counts);
     const long slot = slot_floats(c.batch, c.n_hid, c.n_out);
     const ScanArgs a = scan_args(c);
-    fill(state, slot, 0.0f, s);
+    // the initial state (checkpoint 0) is zero: written by the all-steps
+    // scans themselves, cleared here for the per-block path
+    if (c.steps > kAllSteps) fill(state, slot, 0.0f, s);
     // input currents of every step in one GEMM (rows t * batch + b); the
     // backward reuses them and recomputes only the recurrences
     const int MT = c.steps * c.batch;
@@ -1511,9 +1547,12 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
     else
         hipLaunchKernelGGL(gemm_fwd, fgrid, dim3(256), 0, s, x, w1,
                            (float *)(ws + l.a1), MT, c.n_hid, c.n_in);
-    hipLaunchKernelGGL(transpose_w2, dim3(blocks((long)c.n_hid * kMaxOut)),
-                       dim3(256), 0, s, w2, (uint16_t *)(ws + l.w2t), c.n_hid,
-                       c.n_out);
+    // W2 transposed for the per-block passes (the per-block forward, and
+    // the backward's recomputation when it is not the fused one)
+    if (c.steps > kAllSteps || !fused_backward(c))
+        hipLaunchKernelGGL(transpose_w2,
+                           dim3(blocks((long)c.n_hid * kMaxOut)), dim3(256), 0,
+                           s, w2, (uint16_t *)(ws + l.w2t), c.n_hid, c.n_out);
     if (c.steps <= kAllSteps) {
         // two passes over all steps; s1 of every step goes to the adjoint
         // states' buffer, which only the backward writes
@@ -1537,7 +1576,9 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
                 hipLaunchKernelGGL(out_current_mfma<true>, og, dim3(256), 0,
                                    s, s1_all, w2f, a2all, (long)MT, c.n_hid,
                                    c.n_out, nk);
-            hipLaunchKernelGGL(scan2_fwd_all,
+            hipLaunchKernelGGL(c.mode == SG_SNN_SPIKE
+                                   ? scan2_fwd_all<SG_SNN_SPIKE>
+                                   : scan2_fwd_all<SG_SNN_RATE>,
                                dim3(blocks((long)c.batch * c.n_out)), dim3(256),
                                0, s, a, c.steps, c.checkpoint, slot, a2all,
                                state, cnt, (float *)(ws + l.rv2));
@@ -1583,8 +1624,17 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     const int chunks = (int)dw2_slots(c);
     const int nblk = c.steps / c.checkpoint;
     if (fused_backward(c)) {
-        hipLaunchKernelGGL(scan_bwd_fused, dim3((unsigned)((long)B * H / 256)),
-                           dim3(256), 0, s, a, nblk, slot, w2,
+        // the mode and, for 10 outputs, their number fixed at compile time
+        typedef void (*BK)(ScanArgs, int, long, const uint16_t *,
+                           const float *, const float *, const float *,
+                           const float *, float *, float *);
+        const bool spike = c.mode == SG_SNN_SPIKE;
+        const BK k = spike ? (O == 10 ? (BK)scan_bwd_fused<SG_SNN_SPIKE, 10>
+                                      : (BK)scan_bwd_fused<SG_SNN_SPIKE, 0>)
+                           : (O == 10 ? (BK)scan_bwd_fused<SG_SNN_RATE, 10>
+                                      : (BK)scan_bwd_fused<SG_SNN_RATE, 0>);
+        hipLaunchKernelGGL(k, dim3((unsigned)((long)B * H / 256)), dim3(256),
+                           0, s, a, nblk, slot, w2,
                            (const float *)(ws + l.a1), state, rv2, g, lam,
                            part2);
     } else {
