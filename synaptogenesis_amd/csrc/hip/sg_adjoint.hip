@@ -19,8 +19,9 @@
 //                          launch, out_scan2_fwd, a block per batch row);
 //                          v2 of every step is kept for the backward
 //   backward  n_hid a multiple of 256 and checkpoints of at most 16
-//             steps: scan_bwd_fused, all of the passes below for every
-//             block of steps in one launch (dW2 summed per batch row);
+//             steps: scan_bwd_fused, the loss gradient and all of the
+//             passes below for every block of steps in one launch (dW2
+//             summed per batch row);
 //             else per block, last first: layer 1's recurrence again from
 //             the block's checkpoint (the currents are kept), recording
 //             v1 and s1, then
@@ -1098,6 +1099,21 @@ __global__ void scan2_bwd(ScanArgs a, const float *__restrict__ rec_v2,
     carry2[i] = dn;
 }
 
+// dL/dcount[b][o] = scale * (softmax(count[b])[o] - onehot(label[b])[o]) / B,
+// the same operations wherever it is computed (loss_grad_kernel, the fused
+// backward scan), so both give the same bits.
+__device__ inline float loss_grad_at(const float *__restrict__ counts,
+                                     const int32_t *__restrict__ labels,
+                                     int B, int O, float scale, int b, int o) {
+    const float *c = counts + b * O;
+    float m = c[0];
+    for (int k = 1; k < O; ++k) m = fmaxf(m, c[k]);
+    float s = 0.0f;
+    for (int k = 0; k < O; ++k) s += expf(c[k] - m);
+    const float p = expf(c[o] - m) / s;
+    return scale * (p - (o == labels[b] ? 1.0f : 0.0f)) / (float)B;
+}
+
 // The whole backward in one launch (n_hid a multiple of 256, checkpoints
 // of at most 16 steps): a block of 256 threads = 256 hidden units of one
 // batch row b; it walks the blocks of steps from the last, and per block
@@ -1119,7 +1135,8 @@ template <int MODE, int OUT>
 __global__ __launch_bounds__(256) void scan_bwd_fused(
         ScanArgs a, int nblk, long slot, const uint16_t *__restrict__ w2,
         const float *__restrict__ a1, const float *__restrict__ state,
-        const float *__restrict__ rec_v2, const float *__restrict__ g,
+        const float *__restrict__ rec_v2, const float *__restrict__ counts,
+        const int32_t *__restrict__ labels, float scale,
         float *__restrict__ lam, float *__restrict__ part2) {
     constexpr int NO = OUT > 0 ? OUT : kMaxOut;   // output slots walked
     __shared__ float d2s[16][kMaxOut];
@@ -1130,7 +1147,9 @@ __global__ __launch_bounds__(256) void scan_bwd_fused(
     const int b = i / H, u = i % H;
     const bool out_thread = threadIdx.x < O;
     const int i2 = b * O + (out_thread ? threadIdx.x : 0);
-    const float gb = out_thread ? g[i2] : 0.0f;
+    const float gb = out_thread ? loss_grad_at(counts, labels, a.B, O, scale, b,
+                                               threadIdx.x)
+                                : 0.0f;
     float wc[NO];
 #pragma unroll
     for (int o = 0; o < NO; ++o)
@@ -1313,8 +1332,21 @@ __global__ __launch_bounds__(256) void dw2_reduce(
     const int per = (chunks + 3) / 4;
     const int c0 = w * per, c1 = min(chunks, c0 + per);
     float acc = 0.0f;
-    if (i < O * H)
-        for (int c = c0; c < c1; ++c) acc += part[(long)c * O * H + i];
+    if (i < O * H) {
+        // eight loads in flight before their in-order adds: the sum is the
+        // same as one slot at a time, without a memory latency per slot
+        const long st = (long)O * H;
+        const float *p = part + i;
+        int c = c0;
+        for (; c + 8 <= c1; c += 8) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = p[(c + j) * st];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc += v[j];
+        }
+        for (; c < c1; ++c) acc += p[c * st];
+    }
     q[w][lane] = acc;
     __syncthreads();
     if (w == 0 && i < O * H) dw2[i] = ((q[0][lane] + q[1][lane]) + q[2][lane]) + q[3][lane];
@@ -1341,8 +1373,17 @@ __global__ void loss_kernel(const float *__restrict__ counts,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        // in batch order; four LDS words a read
         float s = 0.0f;
-        for (int b = 0; b < B; ++b) s += lb[b];
+        int b = 0;
+        for (; b + 4 <= B; b += 4) {
+            const float4 v = *(const float4 *)(lb + b);
+            s += v.x;
+            s += v.y;
+            s += v.z;
+            s += v.w;
+        }
+        for (; b < B; ++b) s += lb[b];
         *loss = s / (float)B;
     }
 }
@@ -1353,15 +1394,8 @@ __global__ void loss_grad_kernel(const float *__restrict__ counts,
                                  int O, float scale, float *__restrict__ g) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= B) return;
-    const float *c = counts + b * O;
-    float m = c[0];
-    for (int o = 1; o < O; ++o) m = fmaxf(m, c[o]);
-    float s = 0.0f;
-    for (int o = 0; o < O; ++o) s += expf(c[o] - m);
-    for (int o = 0; o < O; ++o) {
-        const float p = expf(c[o] - m) / s;
-        g[b * O + o] = scale * (p - (o == labels[b] ? 1.0f : 0.0f)) / (float)B;
-    }
+    for (int o = 0; o < O; ++o)
+        g[b * O + o] = loss_grad_at(counts, labels, B, O, scale, b, o);
 }
 
 __global__ void fill_kernel(float *p, long n, float v) {
@@ -1619,15 +1653,14 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     const int M = c.checkpoint * B;
     const long slot = slot_floats(B, H, O);
     const ScanArgs a = scan_args(c);
-    hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256), 0, s,
-                       cnt, labels, B, O, grad_scale, g);
     const int chunks = (int)dw2_slots(c);
     const int nblk = c.steps / c.checkpoint;
     if (fused_backward(c)) {
         // the mode and, for 10 outputs, their number fixed at compile time
         typedef void (*BK)(ScanArgs, int, long, const uint16_t *,
                            const float *, const float *, const float *,
-                           const float *, float *, float *);
+                           const float *, const int32_t *, float, float *,
+                           float *);
         const bool spike = c.mode == SG_SNN_SPIKE;
         const BK k = spike ? (O == 10 ? (BK)scan_bwd_fused<SG_SNN_SPIKE, 10>
                                       : (BK)scan_bwd_fused<SG_SNN_SPIKE, 0>)
@@ -1635,9 +1668,11 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
                                       : (BK)scan_bwd_fused<SG_SNN_RATE, 0>);
         hipLaunchKernelGGL(k, dim3((unsigned)((long)B * H / 256)), dim3(256),
                            0, s, a, nblk, slot, w2,
-                           (const float *)(ws + l.a1), state, rv2, g, lam,
-                           part2);
+                           (const float *)(ws + l.a1), state, rv2, cnt, labels,
+                           grad_scale, lam, part2);
     } else {
+        hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256),
+                           0, s, cnt, labels, B, O, grad_scale, g);
         fill(part2, (long)chunks * O * H, 0.0f, s);
         fill(carry1, (long)B * H, 0.0f, s);
         fill(carry2, (long)B * O, 0.0f, s);
