@@ -180,8 +180,8 @@ def _workloads():
                      data="synthetic (Philox-seeded weights U[0,0.5); "
                           "device-drawn N(5,2) input into L1)"),
         # launch_bytes: one launch reads and writes every word once, for
-        # the several steps it takes (8 on the 1024 board, 4 on the 16384
-        # one; "game of life launches" in the report)
+        # the several steps it takes (8 on both boards; "game of life
+        # launches" in the report)
         "gol1024": dict(make=lambda: workloads.gol(1024), kernel="gol_step",
                         label="gol_step (8 steps per launch on bit-packed "
                               "rows: 3x3 count + rule + shift)",
@@ -191,7 +191,7 @@ def _workloads():
                         data="synthetic (Philox Bernoulli(0.5) cells)"),
         "gol16384": dict(make=lambda: workloads.gol(16384),
                          kernel="gol_step",
-                         label="gol_step (4 steps per launch on bit-packed "
+                         label="gol_step (8 steps per launch on bit-packed "
                                "rows: 3x3 count + rule + shift)",
                          launch_bytes=8 * 16384 * 16384,
                          step_bytes=8 * 16384 * 16384, cpu_steps=1,

@@ -319,8 +319,8 @@ int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
                     const float *kernel9, int survival_min, int survival_max,
                     int birth_min, int birth_max, void *stream);
 /* The same for `steps` consecutive steps in one pass (1 to
- * sg_hip_gol_max_steps(kernel9, rows, cols): 4, or 8 for kernels of 0s and
- * 1s on boards small enough for the launch to bound the step), for a layer
+ * sg_hip_gol_max_steps(kernel9, rows, cols): 8 for kernels of 0s and 1s,
+ * else 4), for a layer
  * whose words nothing else reads between them: words_out holds the words
  * after the last step, `input` (if not NULL) that step's neighbour sums. */
 int sg_hip_gol_max_steps(const float *kernel9, int rows, int cols);

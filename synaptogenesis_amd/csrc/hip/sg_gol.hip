@@ -303,133 +303,224 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_multi_kernel(
     }
 }
 
+// v_writelane_b32: old with lane `lane` set to the wavefront-uniform value
+// (the compiler's intrinsic, which this HIP has no builtin name for).
+__device__ int gol_writelane(int value, int lane, int old)
+    __asm("llvm.amdgcn.writelane.i32");
+
+// Lane i gets lane i - 1's value (wave_shr:1) / lane i + 1's (wave_shl:1);
+// the lane with no source gets 0.
+__device__ __forceinline__ uint64_t gol_row_above(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138,
+                                               0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32),
+                                               0x138, 0xf, 0xf, true);
+    return (uint64_t)(uint32_t)hi << 32 | (uint32_t)lo;
+}
+__device__ __forceinline__ uint64_t gol_row_below(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x130,
+                                               0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32),
+                                               0x130, 0xf, 0xf, true);
+    return (uint64_t)(uint32_t)hi << 32 | (uint32_t)lo;
+}
+
 // Kernels whose nine weights are each 0 or 1 (the game of life's own:
 // ones around a cleared centre): K steps (1 <= K <= 8) on rows of 64 cells
 // packed into 64-bit masks.  A block owns a 64 x 64 region of cells (rows
 // row0 .., columns col0 ..) and writes its inner (64 - 2K) x (64 - 2K)
 // tile: each step's new bits are exact one cell further in from the
 // region's edge than the previous step's.  Its four wavefronts stage 16
-// region rows each, lane l = column col0 + l, and pack each row's newest
-// bits, the K survival bits (bit s - 1 of the word for step s) and, on a
-// bounded board, the cells inside it, by ballot.  One wavefront then takes
-// all K steps, lane i = region row i, with the rows above and below from
-// its neighbour lanes: the neighbour count of 64 cells at once as four bit
-// planes (a ripple add of each weighted neighbour mask), the rule as the
+// region rows each, lane l = column col0 + l (a region inside the board
+// loads from a row base with no per-lane wrap or bounds arithmetic), and
+// pack each row's newest bits, the K survival bits (bit s - 1 of the word
+// for step s) and, on a bounded board, the cells inside it, by ballot; lane
+// j of the wavefront collects row j's masks.  One wavefront then takes all
+// K steps, lane i = region row i, with the rows above and below from its
+// neighbour lanes.  LIFE (neighbours = the eight around the cell, survival
+// on 2-3, birth on 3): each row's horizontal sums by full adders, their sum
+// as a ones mask and four twos masks; the count is 2 or 3 where exactly one
+// twos mask is set, and the cell lives there if the ones mask or its
+// survival bit is set.  Otherwise the count of 64 cells at once as four bit
+// planes (a ripple add of each weighted neighbour mask) and the rule as the
 // union of the count's equality masks for the values the survival and
-// birth ranges admit.  The count equals the integer sum of the
-// one-weighted bits, which is the float sum of the one-step kernel
-// exactly; cells outside a bounded board hold 0 at every step.  Finally
-// each wavefront writes its rows' inner words: the staged word shifted by
-// K with the K new bits on top, and the last step's count as the input.
-template <int K>
+// birth ranges admit.  The count equals the integer sum of the one-weighted
+// bits, which is the float sum of the one-step kernel exactly; cells
+// outside a bounded board hold 0 at every step.  Finally each wavefront
+// writes its rows' inner words: the staged word shifted by K with the K new
+// bits on top (each lane reads the half of each step's row mask that holds
+// its column), and the last step's count as the input.
+template <int K, bool LIFE>
 __global__ __launch_bounds__(256) void gol_packed_kernel(
         const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
         float *__restrict__ input, int rows, int cols, int wrap,
         uint32_t wmask, uint32_t surv, uint32_t birth) {
     constexpr int TI = 64 - 2 * K;           // inner tile
-    __shared__ uint64_t newest[64], inside[64], keep[K][64];
-    __shared__ uint64_t step_bits[K][64], planes[4][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NP = LIFE ? 5 : 4;         // count masks of the last step
+    constexpr int NM = K + 2;                // masks staged per row
+    __shared__ uint64_t staged[NM][64];      // newest, inside, survival
+    __shared__ uint32_t halves[64][2][K];    // step masks, [row][half][step]
+    __shared__ uint64_t planes[NP][64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int row0 = blockIdx.y * TI - K, col0 = blockIdx.x * TI - K;
     const int c = col0 + lane;
-    // the 16 rows' addresses first (cells outside a bounded board read
-    // cell 0 and are cleared), then all 16 loads at once
-    const int cw = wrap ? gol_wrap_k(c, cols) : c;
-    const bool col_ok = wrap || (c >= 0 && c < cols);
+    const bool interior = row0 >= 0 && row0 + 64 <= rows && col0 >= 0 &&
+                          col0 + 64 <= cols;
     uint32_t words[16];
-    bool okv[16];
+    uint32_t mlo[NM], mhi[NM];               // lane j: row j's masks
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int r = row0 + wave * 16 + j;
-        const int rw = wrap ? gol_wrap_k(r, rows) : r;
-        okv[j] = col_ok && (wrap || (r >= 0 && r < rows));
-        words[j] = in[okv[j] ? (long)rw * cols + cw : 0];
-    }
+    for (int m = 0; m < NM; ++m) mlo[m] = mhi[m] = 0u;
+    auto collect = [&](int j, int m, uint64_t v) {
+        mlo[m] = (uint32_t)gol_writelane((int)(uint32_t)v, j, (int)mlo[m]);
+        mhi[m] = (uint32_t)gol_writelane((int)(uint32_t)(v >> 32), j,
+                                         (int)mhi[m]);
+    };
+    if (interior) {
+        mlo[1] = mhi[1] = ~0u;               // every cell inside the board
+        const uint32_t *base = in + (long)(row0 + wave * 16) * cols + col0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int i = wave * 16 + j;
-        const bool ok = okv[j];
-        const uint32_t v = ok ? words[j] : 0u;
-        words[j] = v;
-        const uint64_t nb = __ballot((v >> 31) != 0u);
-        const uint64_t ib = __ballot(ok);
-        uint64_t kb[K];
+        for (int j = 0; j < 16; ++j) words[j] = base[(long)j * cols + lane];
 #pragma unroll
-        for (int s = 0; s < K; ++s) kb[s] = __ballot(((v >> s) & 1u) != 0u);
-        if (lane == 0) {
-            newest[i] = nb;
-            inside[i] = ib;
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t v = words[j];
+            collect(j, 0, __ballot((int)v < 0));
 #pragma unroll
-            for (int s = 0; s < K; ++s) keep[s][i] = kb[s];
+            for (int s = 0; s < K; ++s)
+                collect(j, 2 + s, __ballot((v >> s) & 1u));
         }
+    } else {
+        // addresses first (cells outside a bounded board read cell 0 and
+        // are cleared), then all 16 loads at once
+        const int cw = wrap ? gol_wrap_k(c, cols) : c;
+        const bool col_ok = wrap || (c >= 0 && c < cols);
+        bool okv[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int r = row0 + wave * 16 + j;
+            const int rw = wrap ? gol_wrap_k(r, rows) : r;
+            okv[j] = col_ok && (wrap || (r >= 0 && r < rows));
+            words[j] = in[okv[j] ? (long)rw * cols + cw : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint32_t v = okv[j] ? words[j] : 0u;
+            words[j] = v;
+            collect(j, 0, __ballot((int)v < 0));
+            collect(j, 1, __ballot(okv[j]));
+#pragma unroll
+            for (int s = 0; s < K; ++s)
+                collect(j, 2 + s, __ballot((v >> s) & 1u));
+        }
+    }
+    if (lane < 16) {
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+            staged[m][wave * 16 + lane] = (uint64_t)mhi[m] << 32 | mlo[m];
     }
     __syncthreads();
     if (wave == 0) {
-        uint64_t b = newest[lane];
-        const uint64_t ib = inside[lane];
+        uint64_t b = staged[0][lane];
+        const uint64_t ib = staged[1][lane];
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-            // rows above and below, every lane taking part in both
-            // shuffles (a lane left out of one supplies 0 to its
-            // neighbour); the edge rows get their own row instead, which
-            // only touches the rows no output depends on
-            const uint64_t a = __shfl_up(b, 1);
-            const uint64_t d = __shfl_down(b, 1);
-            const uint64_t nbr[9] = {a << 1, a, a >> 1, b << 1, b, b >> 1,
-                                     d << 1, d, d >> 1};
-            uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+            // rows above and below by whole-wavefront DPP shifts (every
+            // lane active); the edge rows get 0, which only touches the
+            // rows no output depends on
+            const uint64_t a = gol_row_above(b);
+            const uint64_t d = gol_row_below(b);
+            const uint64_t p = staged[2 + s][lane];
+            if (LIFE) {
+                const uint64_t al = a << 1, ar = a >> 1, bl = b << 1,
+                               br = b >> 1, dl = d << 1, dr = d >> 1;
+                uint64_t t = al ^ a;
+                const uint64_t sa = t ^ ar, ca = (al & a) | (ar & t);
+                const uint64_t sb = bl ^ br, cb = bl & br;
+                t = dl ^ d;
+                const uint64_t sd = t ^ dr, cd = (dl & d) | (dr & t);
+                t = sa ^ sb;
+                const uint64_t one = t ^ sd, c1 = (sa & sb) | (sd & t);
+                // count = one + 2 (ca + cb + cd + c1): 2 or 3 where exactly
+                // one of the four twos masks is set
+                const uint64_t p1 = ca ^ cb, q1 = cd ^ c1;
+                const uint64_t t1 =
+                    (p1 ^ q1) & ~((ca & cb) | (cd & c1) | (p1 & q1));
+                b = t1 & (one | p) & ib;
+                if (s == K - 1 && input) {
+                    planes[0][lane] = one;
+                    planes[1][lane] = ca;
+                    planes[2][lane] = cb;
+                    planes[3][lane] = cd;
+                    planes[NP - 1][lane] = c1;
+                }
+            } else {
+                const uint64_t nbr[9] = {a << 1, a, a >> 1, b << 1, b,
+                                         b >> 1, d << 1, d, d >> 1};
+                uint64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                if (!((wmask >> q) & 1u)) continue;
-                const uint64_t x = nbr[q];
-                const uint64_t k0 = c0 & x;
-                c0 ^= x;
-                const uint64_t k1 = c1 & k0;
-                c1 ^= k0;
-                const uint64_t k2 = c2 & k1;
-                c2 ^= k1;
-                c3 |= k2;
-            }
-            uint64_t sv = 0, bv = 0;
+                for (int q = 0; q < 9; ++q) {
+                    if (!((wmask >> q) & 1u)) continue;
+                    const uint64_t x = nbr[q];
+                    const uint64_t k0 = c0 & x;
+                    c0 ^= x;
+                    const uint64_t k1 = c1 & k0;
+                    c1 ^= k0;
+                    const uint64_t k2 = c2 & k1;
+                    c2 ^= k1;
+                    c3 |= k2;
+                }
+                uint64_t sv = 0, bv = 0;
 #pragma unroll
-            for (int v = 0; v <= 9; ++v) {
-                if (!(((surv | birth) >> v) & 1u)) continue;
-                const uint64_t eq = ((v & 1) ? c0 : ~c0) & ((v & 2) ? c1 : ~c1) &
-                                    ((v & 4) ? c2 : ~c2) & ((v & 8) ? c3 : ~c3);
-                if ((surv >> v) & 1u) sv |= eq;
-                if ((birth >> v) & 1u) bv |= eq;
+                for (int v = 0; v <= 9; ++v) {
+                    if (!(((surv | birth) >> v) & 1u)) continue;
+                    const uint64_t eq = ((v & 1) ? c0 : ~c0) &
+                                        ((v & 2) ? c1 : ~c1) &
+                                        ((v & 4) ? c2 : ~c2) &
+                                        ((v & 8) ? c3 : ~c3);
+                    if ((surv >> v) & 1u) sv |= eq;
+                    if ((birth >> v) & 1u) bv |= eq;
+                }
+                b = ((p & sv) | (~p & bv)) & ib;
+                if (s == K - 1 && input) {
+                    planes[0][lane] = c0;
+                    planes[1][lane] = c1;
+                    planes[2][lane] = c2;
+                    planes[3][lane] = c3;
+                }
             }
-            const uint64_t p = keep[s][lane];
-            b = ((p & sv) | (~p & bv)) & ib;
-            step_bits[s][lane] = b;
-            if (s == K - 1) {
-                planes[0][lane] = c0;
-                planes[1][lane] = c1;
-                planes[2][lane] = c2;
-                planes[3][lane] = c3;
-            }
+            halves[lane][0][s] = (uint32_t)b;
+            halves[lane][1][s] = (uint32_t)(b >> 32);
         }
     }
     __syncthreads();
     if (lane < K || lane >= 64 - K) return;
     if (c >= cols) return;
+    const int half = lane >> 5, bit = lane & 31;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const int i = wave * 16 + j;
         const int r = row0 + i;
         if (i < K || i >= 64 - K || r >= rows) continue;
         uint32_t word = words[j] >> K;
+        const uint32_t *h = halves[i][half];
 #pragma unroll
         for (int s = 0; s < K; ++s)
-            word |= (uint32_t)((step_bits[s][i] >> lane) & 1ull)
-                    << (32 - K + s);
+            word |= __builtin_amdgcn_ubfe(h[s], bit, 1) << (32 - K + s);
         const long idx = (long)r * cols + c;
         __builtin_nontemporal_store(word, &out[idx]);
         if (input) {
-            const int n = (int)((planes[0][i] >> lane) & 1ull) |
-                          (int)((planes[1][i] >> lane) & 1ull) << 1 |
-                          (int)((planes[2][i] >> lane) & 1ull) << 2 |
-                          (int)((planes[3][i] >> lane) & 1ull) << 3;
+            int n;
+            if (LIFE)
+                n = (int)((planes[0][i] >> lane) & 1ull) +
+                    2 * ((int)((planes[1][i] >> lane) & 1ull) +
+                         (int)((planes[2][i] >> lane) & 1ull) +
+                         (int)((planes[3][i] >> lane) & 1ull) +
+                         (int)((planes[NP - 1][i] >> lane) & 1ull));
+            else
+                n = (int)((planes[0][i] >> lane) & 1ull) |
+                    (int)((planes[1][i] >> lane) & 1ull) << 1 |
+                    (int)((planes[2][i] >> lane) & 1ull) << 2 |
+                    (int)((planes[3][i] >> lane) & 1ull) << 3;
             input[idx] = (float)n;
         }
     }
@@ -464,15 +555,15 @@ extern "C" int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
 }
 
 extern "C" int sg_hip_gol_max_steps(const float *k, int rows, int cols) {
+    (void)rows;
+    (void)cols;
     for (int q = 0; q < 9; ++q)
         if (k[q] != 0.0f && k[q] != 1.0f) return 4;
-    // the packed form: 8 steps per launch where the launch, not the board's
-    // traffic, bounds the step (a 48 x 48 tile of 8 steps reads 1.78 words
-    // per cell, a 56 x 56 tile of 4 steps 1.31); 16 steps (32 x 32 tiles)
-    // measured no faster on a 1024 board: one wavefront per block steps the
-    // rows, and the step's instructions, not the launch, then bound it
-    const long tiles8 = (long)((cols + 47) / 48) * ((rows + 47) / 48);
-    return tiles8 < 2048 ? 8 : 4;
+    // the packed form: its instructions, not the board's traffic, bound a
+    // launch, and 8 steps a launch (48 x 48 tiles) measured fastest on
+    // boards of 1024 and 16384 (4 steps: 1.5x the time a step on both; 10
+    // to 16 steps: no faster on 1024, slower on 16384)
+    return 8;
 }
 
 extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
@@ -498,14 +589,23 @@ extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
             if (v >= bmin && v <= bmax) birth |= 1u << v;
         }
         const int ti = 64 - 2 * steps;
-        dim3 g((cols + ti - 1) / ti, (rows + ti - 1) / ti);
+        const dim3 g((cols + ti - 1) / ti, (rows + ti - 1) / ti);
         typedef void (*PK)(const uint32_t *, uint32_t *, float *, int, int,
                            int, uint32_t, uint32_t, uint32_t);
-        static const PK kerns[8] = {
-            gol_packed_kernel<1>, gol_packed_kernel<2>, gol_packed_kernel<3>,
-            gol_packed_kernel<4>, gol_packed_kernel<5>, gol_packed_kernel<6>,
-            gol_packed_kernel<7>, gol_packed_kernel<8>};
-        const PK kern = kerns[steps - 1];
+        // the game of life's own rule: eight neighbours, survival on 2-3,
+        // birth on 3
+        const bool life = wmask == 0x1efu && surv == 0xcu && birth == 0x8u;
+        static const PK life_k[8] = {
+            gol_packed_kernel<1, true>, gol_packed_kernel<2, true>,
+            gol_packed_kernel<3, true>, gol_packed_kernel<4, true>,
+            gol_packed_kernel<5, true>, gol_packed_kernel<6, true>,
+            gol_packed_kernel<7, true>, gol_packed_kernel<8, true>};
+        static const PK any_k[8] = {
+            gol_packed_kernel<1, false>, gol_packed_kernel<2, false>,
+            gol_packed_kernel<3, false>, gol_packed_kernel<4, false>,
+            gol_packed_kernel<5, false>, gol_packed_kernel<6, false>,
+            gol_packed_kernel<7, false>, gol_packed_kernel<8, false>};
+        const PK kern = (life ? life_k : any_k)[steps - 1];
         hipLaunchKernelGGL(kern, g, dim3(256), 0, (hipStream_t)stream,
                            words_in, words_out, input, rows, cols, wrap,
                            wmask, surv, birth);

@@ -133,7 +133,7 @@ GOL_MULTI = [
     (300, 260, False, 10, "int", 318),
     (1024, 1024, True, 21, "std", 319),
     (5, 3, True, 8, "std", 320),        # narrower than the halo
-    (4096, 4096, True, 6, "std", 321),  # large: 4 steps per launch
+    (4096, 4096, True, 6, "std", 321),  # large board
     (640, 700, False, 35, "std", 322),
     (33, 31, True, 50, "std", 323),
 ]
