@@ -208,29 +208,53 @@ __global__ __launch_bounds__(256) void gemm_fwd(const uint16_t *__restrict__ X,
 // rows: xp[g][k][0..7] = X[8 g .. 8 g + 7][k], for g < gp (rows past M are
 // 0; gp = the groups rounded up to whole 32-row slices), so one 16-byte
 // read is the eight reduction values an MFMA operand lane holds.  Thread =
-// (group, 8 columns).
+// (group, 8 columns), whose output is the 128 contiguous bytes of xp at
+// 128 i when K is a multiple of 8: the block's 32 KB then go out through
+// LDS as 16-byte stores of consecutive lanes to consecutive addresses
+// (9.7 us for 784 columns x 12800 rows, 4.1 TB/s, against 12.3 us with
+// each lane storing its own 128 bytes).
 __global__ __launch_bounds__(256) void pack_x(const uint16_t *__restrict__ X,
                                               uint16_t *__restrict__ xp,
                                               int M, int K, long gp) {
+    __shared__ u32x4 staged[256 * 8];
     const int nc = (K + 7) / 8;
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= gp * nc) return;
-    const int g = (int)(i / nc), c = (int)(i % nc);
-    u32x4 r[8];
+    const long n = gp * nc;
+    const long i0 = blockIdx.x * (long)blockDim.x;
+    const long i = i0 + threadIdx.x;
+    const bool whole = K % 8 == 0;
+    u32x4 o[8];
+    if (i < n) {
+        const int g = (int)(i / nc), c = (int)(i % nc);
+        u32x4 r[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = load8(X, M, K, 8 * g + j, 8 * c);
+        for (int j = 0; j < 8; ++j) r[j] = load8(X, M, K, 8 * g + j, 8 * c);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int k = 8 * c + e;
-        if (k >= K) break;
-        u32x4 o;
+        for (int e = 0; e < 8; ++e) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            const uint32_t lo = (r[2 * p][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-            const uint32_t hi = (r[2 * p + 1][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
-            o[p] = lo | (hi << 16);
+            for (int p = 0; p < 4; ++p) {
+                const uint32_t lo =
+                    (r[2 * p][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                const uint32_t hi =
+                    (r[2 * p + 1][e >> 1] >> (16 * (e & 1))) & 0xFFFFu;
+                o[e][p] = lo | (hi << 16);
+            }
         }
-        *(u32x4 *)&xp[((long)g * K + k) * 8] = o;
+        if (!whole) {
+            for (int e = 0; e < 8 && 8 * c + e < K; ++e)
+                *(u32x4 *)&xp[((long)g * K + 8 * c + e) * 8] = o[e];
+            return;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) staged[threadIdx.x * 8 + e] = o[e];
+    }
+    if (!whole) return;   // uniform over the block
+    __syncthreads();
+    const long left = (n - i0) * 8;   // 16-byte chunks of this block
+    u32x4 *dst = (u32x4 *)xp + i0 * 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int q = threadIdx.x + 256 * j;
+        if (q < left) dst[q] = staged[q];
     }
 }
 
