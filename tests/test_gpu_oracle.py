@@ -122,7 +122,10 @@ def _gol_max_steps(kernel9, rows, cols):
 
 GOL_MULTI = [
     # rows, cols, wrap, steps, kernel, seed; kernel "std" (ones around a
-    # cleared centre: the packed form), "int" (integers -2 .. 3) or "float"
+    # cleared centre with the game of life's rule: the packed form's own
+    # rule), "rule" (the same kernel, survival on 1-4 and birth on 3-5),
+    # "01" (other 0/1 weights: both the packed form's general rule),
+    # "int" (integers -2 .. 3) or "float"
     (17, 23, True, 37, "std", 311),     # boards smaller than a tile + halo
     (9, 11, False, 13, "std", 312),
     (24, 20, True, 6, "float", 313),
@@ -136,6 +139,10 @@ GOL_MULTI = [
     (4096, 4096, True, 6, "std", 321),  # large board
     (640, 700, False, 35, "std", 322),
     (33, 31, True, 50, "std", 323),
+    (300, 260, True, 21, "rule", 324),
+    (130, 70, False, 17, "rule", 325),
+    (1024, 1024, True, 19, "01", 326),
+    (70, 130, False, 12, "01", 327),
 ]
 
 
@@ -146,12 +153,13 @@ GOL_MULTI = [
 def test_game_of_life_several_steps_per_launch(gpu_lib, rows, cols, wrap,
                                                steps, kern, seed):
     """A board that nothing else reads runs several steps per launch
-    (groups of 8 with the standard kernel, of 4 with others, then the
+    (groups of 8 with kernels of 0s and 1s, of 4 with others, then the
     remainder): final history words and the last
     step's neighbour sums bit for bit against the oracle, on wrapping and
-    bounded boards, with the standard kernel (the bit-packed form), other
-    integer kernels and a float kernel (the float sum in row-major order,
-    bounded boards skipping outside cells)."""
+    bounded boards, with the standard kernel and rule (the bit-packed
+    form's full-adder rule), other rules and other 0/1 kernels (its general
+    rule), other integer kernels and a float kernel (the float sum in
+    row-major order, bounded boards skipping outside cells)."""
     print("game of life %dx%d wrap=%s steps=%d kernel=%s" %
           (rows, cols, wrap, steps, kern), flush=True)
     case = netdefs.gol("gol_multi_%d" % seed, rows, cols, steps, seed,
@@ -162,13 +170,22 @@ def test_game_of_life_several_steps_per_launch(gpu_lib, rows, cols, wrap,
         k = rng.normal(0.0, 1.2, 9).astype(np.float32)
     elif kern == "int":
         k = rng.integers(-2, 4, 9).astype(np.float32)
-    if kern != "std":
+    elif kern == "01":
+        k = rng.integers(0, 2, 9).astype(np.float32)
+        k[4] = 0.0
+        if np.array_equal(k, _STD_KERNEL):
+            k[0] = 0.0
+    elif kern == "rule":
+        case.net["structures"][0]["layers"][0].update(
+            {"survival_min": "1", "survival_max": "4", "birth_min": "3",
+             "birth_max": "5"})
+    if kern in ("float", "int", "01"):
         k[4] = 0.0   # the centre, cleared by "diagonal": false
         case.weights = {"life": k}
     captures, finals, report = harness.run_case(case, gpu_lib)
     assert report.get("iterations") == str(steps)
     per = _gol_max_steps(case.weights.get("life", _STD_KERNEL), rows, cols)
-    assert per == 4 if kern != "std" else per in (4, 8)
+    assert per == (8 if kern in ("std", "rule", "01") else 4)
     assert report.get("game of life launches") == str((steps + per - 1) //
                                                       per)
     problems = stepper.compare(case, captures, finals)
