@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kGolTx * kGolWy) void gol_step_kernel(
     }
 }
 
-// x within K <= 4 of [0, n) wrapped into it: one add or subtract, a
+// x within K <= 8 of [0, n) wrapped into it: one add or subtract, a
 // division only for boards narrower than the halo.
 __device__ __forceinline__ int gol_wrap_k(int x, int n) {
     if (x < 0) x += n;
