@@ -21,7 +21,7 @@
 //   backward  n_hid a multiple of 256 and checkpoints of at most 16
 //             steps: scan_bwd_fused, the loss gradient and all of the
 //             passes below for every block of steps in one launch (dW2
-//             summed per batch row);
+//             summed per batch row), with pack_x as extra blocks of it;
 //             else per block, last first: layer 1's recurrence again from
 //             the block's checkpoint (the currents are kept), recording
 //             v1 and s1, then
@@ -213,13 +213,16 @@ __global__ __launch_bounds__(256) void gemm_fwd(const uint16_t *__restrict__ X,
 // LDS as 16-byte stores of consecutive lanes to consecutive addresses
 // (9.7 us for 784 columns x 12800 rows, 4.1 TB/s, against 12.3 us with
 // each lane storing its own 128 bytes).
-__global__ __launch_bounds__(256) void pack_x(const uint16_t *__restrict__ X,
-                                              uint16_t *__restrict__ xp,
-                                              int M, int K, long gp) {
+// One 256-thread block `block` of it (pack_x, or extra blocks of the
+// fused backward scan's launch).
+__device__ __forceinline__ void pack_x_block(const uint16_t *__restrict__ X,
+                                             uint16_t *__restrict__ xp,
+                                             int M, int K, long gp,
+                                             long block) {
     __shared__ u32x4 staged[256 * 8];
     const int nc = (K + 7) / 8;
     const long n = gp * nc;
-    const long i0 = blockIdx.x * (long)blockDim.x;
+    const long i0 = block * 256;
     const long i = i0 + threadIdx.x;
     const bool whole = K % 8 == 0;
     u32x4 o[8];
@@ -256,6 +259,12 @@ __global__ __launch_bounds__(256) void pack_x(const uint16_t *__restrict__ X,
         const int q = threadIdx.x + 256 * j;
         if (q < left) dst[q] = staged[q];
     }
+}
+
+__global__ __launch_bounds__(256) void pack_x(const uint16_t *__restrict__ X,
+                                              uint16_t *__restrict__ xp,
+                                              int M, int K, long gp) {
+    pack_x_block(X, xp, M, K, gp, blockIdx.x);
 }
 
 // part[z][H][K] = L[rows of range z][H]^T . X[rows of range z][K] (dW1):
@@ -1161,7 +1170,15 @@ __global__ __launch_bounds__(256) void scan_bwd_fused(
         const float *__restrict__ a1, const float *__restrict__ state,
         const float *__restrict__ rec_v2, const float *__restrict__ counts,
         const int32_t *__restrict__ labels, float scale,
-        float *__restrict__ lam, float *__restrict__ part2) {
+        float *__restrict__ lam, float *__restrict__ part2,
+        const uint16_t *__restrict__ X, uint16_t *__restrict__ xp, int MT,
+        int n_in, long gp, int scan_blocks) {
+    // the blocks past the scan's regroup x for the dW1 product (pack_x):
+    // memory work beside the scan's arithmetic, in the same launch
+    if ((int)blockIdx.x >= scan_blocks) {
+        pack_x_block(X, xp, MT, n_in, gp, (long)blockIdx.x - scan_blocks);
+        return;
+    }
     constexpr int NO = OUT > 0 ? OUT : kMaxOut;   // output slots walked
     __shared__ float d2s[16][kMaxOut];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1679,21 +1696,28 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
     const ScanArgs a = scan_args(c);
     const int chunks = (int)dw2_slots(c);
     const int nblk = c.steps / c.checkpoint;
+    const int MT = c.steps * B;
+    const long gp = ((MT + 7) / 8 + 3) / 4 * 4;
+    uint16_t *xp = (uint16_t *)(ws + l.xp);
     if (fused_backward(c)) {
         // the mode and, for 10 outputs, their number fixed at compile time
         typedef void (*BK)(ScanArgs, int, long, const uint16_t *,
                            const float *, const float *, const float *,
                            const float *, const int32_t *, float, float *,
-                           float *);
+                           float *, const uint16_t *, uint16_t *, int, int,
+                           long, int);
         const bool spike = c.mode == SG_SNN_SPIKE;
         const BK k = spike ? (O == 10 ? (BK)scan_bwd_fused<SG_SNN_SPIKE, 10>
                                       : (BK)scan_bwd_fused<SG_SNN_SPIKE, 0>)
                            : (O == 10 ? (BK)scan_bwd_fused<SG_SNN_RATE, 10>
                                       : (BK)scan_bwd_fused<SG_SNN_RATE, 0>);
-        hipLaunchKernelGGL(k, dim3((unsigned)((long)B * H / 256)), dim3(256),
-                           0, s, a, nblk, slot, w2,
+        const int scan_blocks = (int)((long)B * H / 256);
+        const long pack_blocks = blocks(gp * ((c.n_in + 7) / 8));
+        hipLaunchKernelGGL(k, dim3((unsigned)(scan_blocks + pack_blocks)),
+                           dim3(256), 0, s, a, nblk, slot, w2,
                            (const float *)(ws + l.a1), state, rv2, cnt, labels,
-                           grad_scale, lam, part2);
+                           grad_scale, lam, part2, x, xp, MT, c.n_in, gp,
+                           scan_blocks);
     } else {
         hipLaunchKernelGGL(loss_grad_kernel, dim3((B + 255) / 256), dim3(256),
                            0, s, cnt, labels, B, O, grad_scale, g);
@@ -1714,12 +1738,11 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
         }
     }
     // dW1 over every step's rows in one GEMM, split into kTnSplit row
-    // ranges of whole 32-row slices
-    const int MT = c.steps * B;
-    const long gp = ((MT + 7) / 8 + 3) / 4 * 4;
-    uint16_t *xp = (uint16_t *)(ws + l.xp);
-    hipLaunchKernelGGL(pack_x, dim3(blocks(gp * ((c.n_in + 7) / 8))),
-                       dim3(256), 0, s, x, xp, MT, c.n_in, gp);
+    // ranges of whole 32-row slices (x regrouped by the fused scan's
+    // launch, else here)
+    if (!fused_backward(c))
+        hipLaunchKernelGGL(pack_x, dim3(blocks(gp * ((c.n_in + 7) / 8))),
+                           dim3(256), 0, s, x, xp, MT, c.n_in, gp);
     const int per = ((MT + kTnSplit - 1) / kTnSplit + kDr - 1) / kDr * kDr;
     const dim3 dgrid((c.n_in + kDk - 1) / kDk, (H + kDh - 1) / kDh, kTnSplit);
     if (H % 4 == 0)
