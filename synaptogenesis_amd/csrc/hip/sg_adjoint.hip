@@ -8,7 +8,8 @@
 //   forward   gemm_fwd_dma A1[T*batch][n_hid] = X . W1^T for every step
 //                          (MFMA BF16; gemm_fwd where n_in is not a
 //                          multiple of 8), kept for the backward
-//             T <= 1024: scan1_fwd_all, out_current_mfma (S1 . W2^T as
+//             T <= 1024: scan1_fwd_all (W2 put in fragment order by
+//             extra blocks of its launch), out_current_mfma (S1 . W2^T as
 //             one MFMA product) and scan2_fwd_all, the passes below over
 //             all steps; else per block:
 //             scan1_fwd    one thread per (b, hidden unit): v1, s1 over
@@ -39,7 +40,7 @@
 //                          where n_hid is not a multiple of 4), split
 //                          into row ranges
 // and add_parts / dw2_reduce add the ranges and chunks in a fixed order into dW1
-// and dW2.
+// and dW2 (dw2_reduce's work as one more z slice of gemm_dw1_dma's launch).
 // Every sum runs in a fixed order: results are deterministic.
 
 #include <hip/hip_runtime.h>
@@ -584,6 +585,41 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
         }
 }
 
+// dW2[o][u] = the partial sums (chunks or batch rows): block `blk` of 256
+// threads covers 64 outputs, its wavefront w adds the w-th quarter of the
+// slots in slot order, and the quarters are added in order (dw2_reduce, or
+// extra blocks of the dW1 product's launch).
+__device__ __forceinline__ void dw2_reduce_block(
+        const float *__restrict__ part, float *__restrict__ dw2, int chunks,
+        int H, int O, int blk) {
+    __shared__ float q[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blk * 64 + lane;
+    const int per = (chunks + 3) / 4;
+    const int c0 = w * per, c1 = min(chunks, c0 + per);
+    float acc = 0.0f;
+    if (i < O * H) {
+        // eight loads in flight before their in-order adds: the sum is the
+        // same as one slot at a time, without a memory latency per slot
+        const long st = (long)O * H;
+        const float *p = part + i;
+        int c = c0;
+        for (; c + 8 <= c1; c += 8) {
+            float v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = p[(c + j) * st];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc += v[j];
+        }
+        for (; c < c1; ++c) acc += p[c * st];
+    }
+    q[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && i < O * H)
+        dw2[i] = ((q[0][lane] + q[1][lane]) + q[2][lane]) + q[3][lane];
+    __syncthreads();   // q is reused by the block's next output group
+}
+
 // dW1, H a multiple of 4.  A slice is 32 rows of L (row = 128 FP32 in
 // 4-float units stored at unit u ^ (4 * ((row >> 3) & 3))) and the xp
 // entries of its four groups ([4][112][8], 7 KB, padded to 8 KB so that
@@ -591,7 +627,9 @@ __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_fwd_dma(
 // multiple of 32); rows past M read row M - 1 of L, against xp's zeros.
 __global__ __launch_bounds__(256) void gemm_dw1_dma(
         const float *__restrict__ L, const uint16_t *__restrict__ xp,
-        float *__restrict__ part, int M, int H, int K, int per, long gp) {
+        float *__restrict__ part, int M, int H, int K, int per, long gp,
+        const float *__restrict__ part2, float *__restrict__ dw2, int chunks2,
+        int O) {
     __shared__ __align__(16) float L0[kDr * kDh], L1[kDr * kDh],
         L2[kDr * kDh];
     __shared__ __align__(16) uint16_t X0[8 * 512], X1[8 * 512], X2[8 * 512];
@@ -602,6 +640,13 @@ __global__ __launch_bounds__(256) void gemm_dw1_dma(
     const int hw = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
     const int lg = (nb % 8 == 0) ? (hw % 8) * (nb / 8) + hw / 8 : hw;
     const int bx = lg % nx, by = (lg / nx) % ny, bz = lg / (nx * ny);
+    if (dw2 && bz == (int)gridDim.z - 1) {
+        // the last z slice sums dW2 (dw2_reduce's work) beside the product
+        const int ng = (O * H + 63) / 64;
+        for (int g = bx + nx * by; g < ng; g += nx * ny)
+            dw2_reduce_block(part2, dw2, chunks2, H, O, g);
+        return;
+    }
     const int h0 = by * kDh, kb = bx * kDk;
     const int rb = bz * per, re = min(M, rb + per);
     const int t = threadIdx.x;
@@ -917,6 +962,18 @@ __global__ __launch_bounds__(1024) void out_scan2_fwd(
     if (count_mode != 0) counts[i] = cnt;
 }
 
+// w2f[(kk * 64 + lane) * 8 + e] = W2[lane % 16][32 kk + 8 (lane / 16) + e]
+// (0 past O or H): the B fragment of units 32 kk .. 32 kk + 31; entry i,
+// written by extra blocks of scan1_fwd_all's launch.
+__device__ __forceinline__ void frag_w2_at(const uint16_t *__restrict__ w2,
+                                           uint16_t *__restrict__ w2f, int H,
+                                           int O, int nk, int i) {
+    if (i >= nk * 64 * 8) return;
+    const int e = i & 7, lane = (i >> 3) & 63, kk = i >> 9;
+    const int o = lane & 15, u = 32 * kk + 8 * (lane >> 4) + e;
+    w2f[i] = (o < O && u < H) ? w2[o * H + u] : (uint16_t)0;
+}
+
 // The forward over all T steps in two passes (T <= 1024; layer 1 does not
 // read layer 2).  scan1_fwd_all: layer 1, thread = (b, u), the recurrence
 // of scan1_fwd over every step; s1 of every step to s1_all, the state after
@@ -924,7 +981,18 @@ __global__ __launch_bounds__(1024) void out_scan2_fwd(
 __global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
                               const float *__restrict__ a1,
                               float *__restrict__ state,
-                              float *__restrict__ s1_all) {
+                              float *__restrict__ s1_all,
+                              const uint16_t *__restrict__ w2,
+                              uint16_t *__restrict__ w2f, int nk,
+                              int scan_blocks) {
+    // the blocks past the scan's put W2 in fragment order for the output
+    // currents in the same launch
+    if ((int)blockIdx.x >= scan_blocks) {
+        frag_w2_at(w2, w2f, a.H, a.O, nk,
+                   ((int)blockIdx.x - scan_blocks) * blockDim.x +
+                       threadIdx.x);
+        return;
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int H = a.H;
     if (i >= a.B * H) return;
@@ -966,17 +1034,6 @@ __global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
 // partial tiles are added in wavefront order.  Every sum runs in a fixed
 // order.
 constexpr int kOutWaves = 4;
-
-// w2f[(kk * 64 + lane) * 8 + e] = W2[lane % 16][32 kk + 8 (lane / 16) + e]
-// (0 past O or H): the B fragment of units 32 kk .. 32 kk + 31.
-__global__ void frag_w2(const uint16_t *__restrict__ w2,
-                        uint16_t *__restrict__ w2f, int H, int O, int nk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nk * 64 * 8) return;
-    const int e = i & 7, lane = (i >> 3) & 63, kk = i >> 9;
-    const int o = lane & 15, u = 32 * kk + 8 * (lane >> 4) + e;
-    w2f[i] = (o < O && u < H) ? w2[o * H + u] : (uint16_t)0;
-}
 
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void out_current_mfma(
@@ -1361,36 +1418,10 @@ __global__ void dw2_partial(const float *__restrict__ g2,
         if (o < O) part[((long)c * O + o) * H + u] += acc[o];
 }
 
-// dW2[o][u] = the partial sums (chunks or batch rows): a block covers 64
-// outputs, its wavefront w adds the w-th quarter of the slots in slot
-// order, and the quarters are added in order.
 __global__ __launch_bounds__(256) void dw2_reduce(
         const float *__restrict__ part, float *__restrict__ dw2, int chunks,
         int H, int O) {
-    __shared__ float q[4][64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = blockIdx.x * 64 + lane;
-    const int per = (chunks + 3) / 4;
-    const int c0 = w * per, c1 = min(chunks, c0 + per);
-    float acc = 0.0f;
-    if (i < O * H) {
-        // eight loads in flight before their in-order adds: the sum is the
-        // same as one slot at a time, without a memory latency per slot
-        const long st = (long)O * H;
-        const float *p = part + i;
-        int c = c0;
-        for (; c + 8 <= c1; c += 8) {
-            float v[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) v[j] = p[(c + j) * st];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) acc += v[j];
-        }
-        for (; c < c1; ++c) acc += p[c * st];
-    }
-    q[w][lane] = acc;
-    __syncthreads();
-    if (w == 0 && i < O * H) dw2[i] = ((q[0][lane] + q[1][lane]) + q[2][lane]) + q[3][lane];
+    dw2_reduce_block(part, dw2, chunks, H, O, blockIdx.x);
 }
 
 // Mean softmax cross-entropy over the batch (one block; sums in batch
@@ -1632,16 +1663,18 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
         // two passes over all steps; s1 of every step goes to the adjoint
         // states' buffer, which only the backward writes
         float *s1_all = (float *)(ws + l.lam);
-        hipLaunchKernelGGL(scan1_fwd_all, dim3(blocks((long)c.batch * c.n_hid)),
+        uint16_t *w2f = (uint16_t *)(ws + l.w2f);
+        const int nk = (c.n_hid + 31) / 32;
+        // W2 in fragment order by extra blocks of layer 1's scan
+        const unsigned scan_blocks = blocks((long)c.batch * c.n_hid);
+        hipLaunchKernelGGL(scan1_fwd_all,
+                           dim3(scan_blocks + blocks((long)nk * 512)),
                            dim3(256), 0, s, a, c.steps, c.checkpoint, slot,
-                           (const float *)(ws + l.a1), state, s1_all);
+                           (const float *)(ws + l.a1), state, s1_all, w2, w2f,
+                           nk, (int)scan_blocks);
         {
             // S1 . W2^T over the whole grid (MFMA), then layer 2's scan
             float *a2all = (float *)(ws + l.a2all);
-            uint16_t *w2f = (uint16_t *)(ws + l.w2f);
-            const int nk = (c.n_hid + 31) / 32;
-            hipLaunchKernelGGL(frag_w2, dim3(blocks((long)nk * 512)),
-                               dim3(256), 0, s, w2, w2f, c.n_hid, c.n_out, nk);
             const dim3 og((unsigned)((MT + 15) / 16));
             if (c.mode == SG_SNN_SPIKE)
                 hipLaunchKernelGGL(out_current_mfma<false>, og, dim3(256), 0,
@@ -1745,16 +1778,19 @@ int sg_snn_backward(const sg_snn_cfg *cfg, const uint16_t *x,
                            dim3(256), 0, s, x, xp, MT, c.n_in, gp);
     const int per = ((MT + kTnSplit - 1) / kTnSplit + kDr - 1) / kDr * kDr;
     const dim3 dgrid((c.n_in + kDk - 1) / kDk, (H + kDh - 1) / kDh, kTnSplit);
-    if (H % 4 == 0)
-        hipLaunchKernelGGL(gemm_dw1_dma, dgrid, dim3(256), 0, s, lam, xp,
-                           part1, MT, H, c.n_in, per, gp);
-    else
+    if (H % 4 == 0) {
+        // one more z slice of blocks sums dW2
+        const dim3 g2(dgrid.x, dgrid.y, kTnSplit + 1);
+        hipLaunchKernelGGL(gemm_dw1_dma, g2, dim3(256), 0, s, lam, xp, part1,
+                           MT, H, c.n_in, per, gp, part2, dw2, chunks, O);
+    } else {
         hipLaunchKernelGGL(gemm_dw1, dgrid, dim3(256), 0, s, lam, xp, part1,
                            MT, H, c.n_in, per);
+        hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 63) / 64), dim3(256), 0,
+                           s, part2, dw2, chunks, H, O);
+    }
     hipLaunchKernelGGL(add_parts, dim3(1024), dim3(256), 0, s, part1, dw1,
                        (long)H * c.n_in, kTnSplit);
-    hipLaunchKernelGGL(dw2_reduce, dim3((O * H + 63) / 64), dim3(256), 0, s,
-                       part2, dw2, chunks, H, O);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(std::string("launch: ") + hipGetErrorString(e));
     return 0;
