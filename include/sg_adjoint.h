@@ -27,7 +27,10 @@
  * input currents W1 x of all steps are one MFMA GEMM (BF16 operands, FP32
  * accumulation), kept in the workspace; the recurrences run per batch row;
  * dW1 is one MFMA GEMM over every step's adjoint states, which enter as
- * three BF16 terms (hi + mid + lo) so the product keeps FP32 precision.
+ * two BF16 terms (hi + mid: 16 significant bits; where n_hid is not a
+ * multiple of 4, three); against the FP64 model its error is that of the
+ * FP32 recurrences (dW1 relative L2 2.5e-7 in spike mode, 3.1e-6 in rate
+ * mode on a 784 -> 1024 -> 10 network, batch 128, 100 steps).
  * The forward keeps the recurrent state only at every `checkpoint`-th
  * step; the backward recomputes layer 1's recurrence of each block of
  * steps from its checkpoint (the input currents, and layer 2's potentials
