@@ -991,12 +991,14 @@ __device__ __forceinline__ void frag_w2_at(const uint16_t *__restrict__ w2,
 
 // The forward over all T steps in two passes (T <= 1024; layer 1 does not
 // read layer 2).  scan1_fwd_all: layer 1, thread = (b, u), the recurrence
-// of scan1_fwd over every step; s1 of every step to s1_all, the state after
-// each block of ck steps to that block's checkpoint slot.
+// of scan1_fwd over every step; s1 of every step to s1_all (to s1_bf as
+// BF16 in spike mode), the state after each block of ck steps to that
+// block's checkpoint slot.
 __global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
                               const float *__restrict__ a1,
                               float *__restrict__ state,
                               float *__restrict__ s1_all,
+                              uint16_t *__restrict__ s1_bf,
                               const uint16_t *__restrict__ w2,
                               uint16_t *__restrict__ w2f, int nk,
                               int scan_blocks) {
@@ -1028,7 +1030,12 @@ __global__ void scan1_fwd_all(ScanArgs a, int T, int ck, long slot,
             if (t0 + j >= T) break;
             v = a.beta * v - a.reset_theta * s + in[j];
             s = act(v - a.theta, a.mode, a.slope);
-            s1_all[(long)(t0 + j) * BH + i] = s;
+            // spikes (0 / 1, exact in BF16) as BF16, else FP32
+            if (s1_bf)
+                s1_bf[(long)(t0 + j) * BH + i] =
+                    s != 0.0f ? (uint16_t)0x3F80u : (uint16_t)0u;
+            else
+                s1_all[(long)(t0 + j) * BH + i] = s;
             if (--left == 0) {
                 left = ck;
                 next[i] = v;
@@ -1052,14 +1059,16 @@ constexpr int kOutWaves = 4;
 
 template <bool SPLIT>
 __global__ __launch_bounds__(256) void out_current_mfma(
-        const float *__restrict__ s1, const uint16_t *__restrict__ w2f,
+        const void *__restrict__ s1v, const uint16_t *__restrict__ w2f,
         float *__restrict__ a2, long rows, int H, int O, int nk) {
     __shared__ f32x4 part[kOutWaves - 1][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int li = lane & 15, lk = lane >> 4;
     const long r0 = (long)blockIdx.x * 16;
     const long row = min(r0 + li, rows - 1);   // past the end: never stored
-    const float *srow = s1 + row * H;
+    // S1 in FP32 (SPLIT) or as BF16 spikes
+    const float *srow = (const float *)s1v + row * H;
+    const uint16_t *brow = (const uint16_t *)s1v + row * H;
     const int per = (nk + kOutWaves - 1) / kOutWaves;
     const int k0 = w * per, k1 = min(nk, k0 + per);
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -1067,13 +1076,23 @@ __global__ __launch_bounds__(256) void out_current_mfma(
     constexpr int F = 8;   // fragments whose loads are in flight together
     for (int kb = k0; kb < k1; kb += F) {
         float v[F][8];
-        bf16x8 b[F];
+        bf16x8 b[F], sb[F];
 #pragma unroll
         for (int q = 0; q < F; ++q) {
             const int kk = kb + q;
             if (kk >= k1) break;
             const int u0 = 32 * kk + 8 * lk;
-            if (vec && u0 + 8 <= H) {
+            if (!SPLIT) {
+                if (vec && u0 + 8 <= H) {
+                    sb[q] = *(const bf16x8 *)&brow[u0];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        sb[q][e] = u0 + e < H ? __builtin_bit_cast(
+                                                    __bf16, brow[u0 + e])
+                                              : (__bf16)0.0f;
+                }
+            } else if (vec && u0 + 8 <= H) {
                 const float4 x0 = *(const float4 *)&srow[u0];
                 const float4 x1 = *(const float4 *)&srow[u0 + 4];
                 v[q][0] = x0.x; v[q][1] = x0.y; v[q][2] = x0.z; v[q][3] = x0.w;
@@ -1095,10 +1114,7 @@ __global__ __launch_bounds__(256) void out_current_mfma(
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mid, b[q], acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, b[q], acc, 0, 0, 0);
             } else {
-                bf16x8 a;
-#pragma unroll
-                for (int e = 0; e < 8; ++e) a[e] = (__bf16)v[q][e];
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[q], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sb[q], b[q], acc, 0, 0, 0);
             }
         }
     }
@@ -1685,8 +1701,10 @@ int sg_snn_forward(const sg_snn_cfg *cfg, const uint16_t *x,
         hipLaunchKernelGGL(scan1_fwd_all,
                            dim3(scan_blocks + blocks((long)nk * 512)),
                            dim3(256), 0, s, a, c.steps, c.checkpoint, slot,
-                           (const float *)(ws + l.a1), state, s1_all, w2, w2f,
-                           nk, (int)scan_blocks);
+                           (const float *)(ws + l.a1), state, s1_all,
+                           c.mode == SG_SNN_SPIKE ? (uint16_t *)s1_all
+                                                  : nullptr,
+                           w2, w2f, nk, (int)scan_blocks);
         {
             // S1 . W2^T over the whole grid (MFMA), then layer 2's scan
             float *a2all = (float *)(ws + l.a2all);
