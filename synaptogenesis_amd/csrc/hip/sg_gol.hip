@@ -527,6 +527,105 @@ __global__ __launch_bounds__(64 * W) void gol_packed_kernel(
     }
 }
 
+
+// The game of life's own rule (0/1 kernel of the eight neighbours,
+// survival on 2-3, birth on 3), K steps (1 <= K <= 8), one wavefront per
+// 64 x 64 region with lane = column col0 + lane: each lane loads its
+// column's 64 words and builds its own 64-row masks (bit i = region row i)
+// of the newest bits, the K survival bits and the cells inside the board;
+// the rows above and below are bit shifts of a mask, the columns beside
+// it the neighbour lanes' masks by DPP; the rule is the full-adder form of
+// gol_packed_kernel with the columns in the rows' place (the neighbourhood
+// is symmetric).  Masks and words stay in the lane that loaded them: no
+// LDS, no barrier.  The inner (64 - 2K) x (64 - 2K) tile is written as in
+// gol_packed_kernel.
+template <int K>
+__global__ __launch_bounds__(64) void gol_column_kernel(
+        const uint32_t *__restrict__ in, uint32_t *__restrict__ out,
+        float *__restrict__ input, int rows, int cols, int wrap) {
+    constexpr int TI = 64 - 2 * K;
+    const int lane = threadIdx.x;
+    const int row0 = blockIdx.y * TI - K, col0 = blockIdx.x * TI - K;
+    const int c = col0 + lane;
+    const bool interior = row0 >= 0 && row0 + 64 <= rows && col0 >= 0 &&
+                          col0 + 64 <= cols;
+    uint32_t words[64];
+    uint64_t ib = ~0ull;
+    if (interior) {
+        const uint32_t *base = in + (long)row0 * cols + c;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) words[r] = base[(long)r * cols];
+    } else {
+        const int cw = wrap ? gol_wrap_k(c, cols) : c;
+        const bool col_ok = wrap || (c >= 0 && c < cols);
+        ib = 0ull;
+#pragma unroll
+        for (int r = 0; r < 64; ++r) {
+            const int rr = row0 + r;
+            const int rw = wrap ? gol_wrap_k(rr, rows) : rr;
+            const bool ok = col_ok && (wrap || (rr >= 0 && rr < rows));
+            words[r] = ok ? in[(long)rw * cols + cw] : 0u;
+            ib |= (uint64_t)ok << r;
+        }
+    }
+    uint32_t nlo = 0u, nhi = 0u, klo[K], khi[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) klo[s] = khi[s] = 0u;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        nlo |= (words[r] >> 31) << r;
+        nhi |= (words[r + 32] >> 31) << r;
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            klo[s] |= ((words[r] >> s) & 1u) << r;
+            khi[s] |= ((words[r + 32] >> s) & 1u) << r;
+        }
+    }
+    uint64_t b = (uint64_t)nhi << 32 | nlo;
+    uint64_t step[K];
+    uint64_t one = 0, ca = 0, cb = 0, cd = 0, c1 = 0;
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const uint64_t L = gol_row_above(b);   // lane - 1: the column left
+        const uint64_t R = gol_row_below(b);   // lane + 1: the column right
+        const uint64_t p = (uint64_t)khi[s] << 32 | klo[s];
+        const uint64_t lu = L << 1, ld = L >> 1, bu = b << 1, bd = b >> 1,
+                       ru = R << 1, rd = R >> 1;
+        uint64_t t = lu ^ L;
+        const uint64_t sa = t ^ ld;
+        ca = (lu & L) | (ld & t);
+        const uint64_t sb = bu ^ bd;
+        cb = bu & bd;
+        t = ru ^ R;
+        const uint64_t sd = t ^ rd;
+        cd = (ru & R) | (rd & t);
+        t = sa ^ sb;
+        one = t ^ sd;
+        c1 = (sa & sb) | (sd & t);
+        const uint64_t p1 = ca ^ cb, q1 = cd ^ c1;
+        const uint64_t t1 = (p1 ^ q1) & ~((ca & cb) | (cd & c1) | (p1 & q1));
+        b = t1 & (one | p) & ib;
+        step[s] = b;
+    }
+    if (lane < K || lane >= 64 - K || c >= cols) return;
+#pragma unroll
+    for (int r = K; r < 64 - K; ++r) {
+        if (row0 + r >= rows) break;
+        uint32_t word = words[r] >> K;
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+            word |= (uint32_t)((step[s] >> r) & 1ull) << (32 - K + s);
+        const long idx = (long)(row0 + r) * cols + c;
+        __builtin_nontemporal_store(word, &out[idx]);
+        if (input)
+            input[idx] = (float)((int)((one >> r) & 1ull) +
+                                 2 * ((int)((ca >> r) & 1ull) +
+                                      (int)((cb >> r) & 1ull) +
+                                      (int)((cd >> r) & 1ull) +
+                                      (int)((c1 >> r) & 1ull)));
+    }
+}
+
 }  // namespace sg
 
 extern "C" int sg_hip_gol_step(const uint32_t *words_in, uint32_t *words_out,
@@ -610,6 +709,23 @@ extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
             gol_packed_kernel<3, false, W>, gol_packed_kernel<4, false, W>,
             gol_packed_kernel<5, false, W>, gol_packed_kernel<6, false, W>,
             gol_packed_kernel<7, false, W>, gol_packed_kernel<8, false, W>};
+        if (life && (long)g.x * g.y >= 2048) {
+            // one wavefront per region, lane = column (gol_column_kernel):
+            // fewer instructions per region (16384 board: 585 against 761
+            // us a launch) but one wavefront's chain per region, which
+            // bounds small boards (1024 board: 13.9 against 8.0 us)
+            typedef void (*CK)(const uint32_t *, uint32_t *, float *, int,
+                               int, int);
+            static const CK col_k[8] = {
+                gol_column_kernel<1>, gol_column_kernel<2>,
+                gol_column_kernel<3>, gol_column_kernel<4>,
+                gol_column_kernel<5>, gol_column_kernel<6>,
+                gol_column_kernel<7>, gol_column_kernel<8>};
+            hipLaunchKernelGGL(col_k[steps - 1], g, dim3(64), 0,
+                               (hipStream_t)stream, words_in, words_out,
+                               input, rows, cols, wrap);
+            return hipGetLastError();
+        }
         const PK kern = (life ? life_k : any_k)[steps - 1];
         hipLaunchKernelGGL(kern, g, dim3(64 * W), 0, (hipStream_t)stream,
                            words_in, words_out, input, rows, cols, wrap,

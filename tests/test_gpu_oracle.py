@@ -139,6 +139,8 @@ GOL_MULTI = [
     (4096, 4096, True, 6, "std", 321),  # large board
     (640, 700, False, 35, "std", 322),
     (33, 31, True, 50, "std", 323),
+    (2600, 3000, False, 9, "std", 328),   # large bounded, ragged: one
+    (3000, 2600, True, 11, "std", 329),   # wavefront per region
     (300, 260, True, 21, "rule", 324),
     (130, 70, False, 17, "rule", 325),
     (1024, 1024, True, 19, "01", 326),
