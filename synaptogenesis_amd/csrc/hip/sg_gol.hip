@@ -528,6 +528,17 @@ __global__ __launch_bounds__(64 * W) void gol_packed_kernel(
 }
 
 
+// An 8 x 8 bit matrix (byte i = row i, bit j = column j) transposed by
+// three swap stages: byte j of the result holds column j (bit i = row i).
+__device__ __forceinline__ uint64_t gol_transpose8(uint64_t x) {
+    uint64_t t = (x ^ (x >> 7)) & 0x00AA00AA00AA00AAull;
+    x = x ^ t ^ (t << 7);
+    t = (x ^ (x >> 14)) & 0x0000CCCC0000CCCCull;
+    x = x ^ t ^ (t << 14);
+    t = (x ^ (x >> 28)) & 0x00000000F0F0F0F0ull;
+    return x ^ t ^ (t << 28);
+}
+
 // The game of life's own rule (0/1 kernel of the eight neighbours,
 // survival on 2-3, birth on 3), K steps (1 <= K <= 8), one wavefront per
 // 64 x 64 region with lane = column col0 + lane: each lane loads its
@@ -575,10 +586,23 @@ __global__ __launch_bounds__(64) void gol_column_kernel(
     for (int r = 0; r < 32; ++r) {
         nlo |= (words[r] >> 31) << r;
         nhi |= (words[r + 32] >> 31) << r;
+    }
+    // the survival masks eight rows at a time: the rows' low bytes as the
+    // rows of an 8 x 8 bit matrix, transposed (three swap stages), so that
+    // byte s holds bit s of the eight rows
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t *w = words + 8 * j;
+        const uint32_t xl = (w[0] & 0xFFu) | (w[1] & 0xFFu) << 8 |
+                            (w[2] & 0xFFu) << 16 | w[3] << 24;
+        const uint32_t xh = (w[4] & 0xFFu) | (w[5] & 0xFFu) << 8 |
+                            (w[6] & 0xFFu) << 16 | w[7] << 24;
+        const uint64_t x = gol_transpose8((uint64_t)xh << 32 | xl);
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-            klo[s] |= ((words[r] >> s) & 1u) << r;
-            khi[s] |= ((words[r + 32] >> s) & 1u) << r;
+            const uint32_t byte = (uint32_t)(x >> (8 * s)) & 0xFFu;
+            if (j < 4) klo[s] |= byte << (8 * j);
+            else khi[s] |= byte << (8 * (j - 4));
         }
     }
     uint64_t b = (uint64_t)nhi << 32 | nlo;
@@ -608,13 +632,24 @@ __global__ __launch_bounds__(64) void gol_column_kernel(
         step[s] = b;
     }
     if (lane < K || lane >= 64 - K || c >= cols) return;
+    // each row's K new bits as one byte: per eight rows, byte s = step s's
+    // bits of the rows, transposed so that byte i = row i's bits
+    uint64_t rb[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        rb[j] = 0ull;
+        if (8 * j + 7 < K || 8 * j >= 64 - K) continue;   // no output row
+        uint64_t x = 0ull;
+#pragma unroll
+        for (int s = 0; s < K; ++s) x |= ((step[s] >> (8 * j)) & 0xFFull) << (8 * s);
+        rb[j] = gol_transpose8(x);
+    }
 #pragma unroll
     for (int r = K; r < 64 - K; ++r) {
         if (row0 + r >= rows) break;
-        uint32_t word = words[r] >> K;
-#pragma unroll
-        for (int s = 0; s < K; ++s)
-            word |= (uint32_t)((step[s] >> r) & 1ull) << (32 - K + s);
+        const uint32_t bits =
+            (uint32_t)(rb[r >> 3] >> (8 * (r & 7))) & ((1u << K) - 1u);
+        const uint32_t word = (words[r] >> K) | bits << (32 - K);
         const long idx = (long)(row0 + r) * cols + c;
         __builtin_nontemporal_store(word, &out[idx]);
         if (input)
