@@ -310,14 +310,14 @@ __device__ int gol_writelane(int value, int lane, int old)
 
 // Lane i gets lane i - 1's value (wave_shr:1) / lane i + 1's (wave_shl:1);
 // the lane with no source gets 0.
-__device__ __forceinline__ uint64_t gol_row_above(uint64_t v) {
+__device__ __forceinline__ uint64_t gol_from_prev_lane(uint64_t v) {
     const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138,
                                                0xf, 0xf, true);
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32),
                                                0x138, 0xf, 0xf, true);
     return (uint64_t)(uint32_t)hi << 32 | (uint32_t)lo;
 }
-__device__ __forceinline__ uint64_t gol_row_below(uint64_t v) {
+__device__ __forceinline__ uint64_t gol_from_next_lane(uint64_t v) {
     const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x130,
                                                0xf, 0xf, true);
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32),
@@ -428,8 +428,8 @@ __global__ __launch_bounds__(64 * W) void gol_packed_kernel(
             // rows above and below by whole-wavefront DPP shifts (every
             // lane active); the edge rows get 0, which only touches the
             // rows no output depends on
-            const uint64_t a = gol_row_above(b);
-            const uint64_t d = gol_row_below(b);
+            const uint64_t a = gol_from_prev_lane(b);
+            const uint64_t d = gol_from_next_lane(b);
             const uint64_t p = staged[2 + s][lane];
             if (LIFE) {
                 const uint64_t al = a << 1, ar = a >> 1, bl = b << 1,
@@ -610,8 +610,8 @@ __global__ __launch_bounds__(64) void gol_column_kernel(
     uint64_t one = 0, ca = 0, cb = 0, cd = 0, c1 = 0;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
-        const uint64_t L = gol_row_above(b);   // lane - 1: the column left
-        const uint64_t R = gol_row_below(b);   // lane + 1: the column right
+        const uint64_t L = gol_from_prev_lane(b);   // the column left
+        const uint64_t R = gol_from_next_lane(b);   // the column right
         const uint64_t p = (uint64_t)khi[s] << 32 | klo[s];
         const uint64_t lu = L << 1, ld = L >> 1, bu = b << 1, bd = b >> 1,
                        ru = R << 1, rd = R >> 1;
