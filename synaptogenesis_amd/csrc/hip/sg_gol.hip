@@ -744,11 +744,13 @@ extern "C" int sg_hip_gol_steps(const uint32_t *words_in, uint32_t *words_out,
             gol_packed_kernel<3, false, W>, gol_packed_kernel<4, false, W>,
             gol_packed_kernel<5, false, W>, gol_packed_kernel<6, false, W>,
             gol_packed_kernel<7, false, W>, gol_packed_kernel<8, false, W>};
-        if (life && (long)g.x * g.y >= 2048) {
+        if (life && (long)g.x * g.y >= 1024) {
             // one wavefront per region, lane = column (gol_column_kernel):
-            // fewer instructions per region (16384 board: 585 against 761
-            // us a launch) but one wavefront's chain per region, which
-            // bounds small boards (1024 board: 13.9 against 8.0 us)
+            // fewer instructions per region but one wavefront's chain per
+            // region, which bounds small boards.  8 steps a launch, row
+            // form against column form: 484 regions 7.8 / 10.7 us, 1024
+            // regions 10.9 / 11.0, 1849 16.7 / 16.0, 7396 49.0 / 41.7, the
+            // 16384 board 761 / 449
             typedef void (*CK)(const uint32_t *, uint32_t *, float *, int,
                                int, int);
             static const CK col_k[8] = {
